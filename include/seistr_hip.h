@@ -1,0 +1,136 @@
+/*
+ * seistr_hip.h -- C ABI of the MI355X (gfx950) structural hot path.
+ *
+ * Every entry point takes a context (HIP stream + cached workspace), DEVICE
+ * pointers to Fortran-order float32 arrays (n1 fastest, then n2, then n3)
+ * and the reference's own parameters, and writes caller-owned device
+ * outputs.  Calls are asynchronous on the context's stream unless stated.
+ * Return value: SH_OK (0) or a negative status; sh_last_error() gives the
+ * message.  Nothing here allocates host-visible state shared between
+ * contexts: one context per thread/stream.
+ *
+ * Each declaration cites the reference entry point it replaces (paths
+ * relative to the reference repository).
+ */
+#ifndef SEISTR_HIP_H
+#define SEISTR_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SH_OK 0
+#define SH_EINVAL (-1)   /* bad shape or parameter */
+#define SH_ENULL (-2)    /* required pointer is NULL */
+#define SH_EALLOC (-3)   /* device allocation failed */
+#define SH_EHIP (-4)     /* HIP runtime error */
+#define SH_EUNSUP (-5)   /* option the library does not implement */
+
+typedef struct sh_ctx sh_ctx;
+
+/* Create a context bound to a HIP stream (NULL = default stream). */
+sh_ctx *sh_ctx_create(void *hip_stream);
+void sh_ctx_destroy(sh_ctx *ctx);
+int sh_ctx_set_stream(sh_ctx *ctx, void *hip_stream);
+/* Release cached device workspace. */
+int sh_ctx_trim(sh_ctx *ctx);
+const char *sh_last_error(const sh_ctx *ctx);
+int sh_version(void);
+/* Block until all work queued on the context's stream has finished. */
+int sh_sync(sh_ctx *ctx);
+
+/* Per-direction iteration record of a dip call: for GN iteration k,
+ * cg_iters[d][k] = shaping-CG iterations run (-1: zero denominator),
+ * ls_step[d][k] = accepted line-search halving (8 = none accepted). */
+typedef struct {
+    int ndir, niter;
+    int cg_iters[2][64];
+    int ls_step[2][64];
+} sh_dip_report;
+
+/* Local slopes by shaping-regularised plane-wave destruction.
+ * Replaces dipcfun.dipc (pyseistr/src/dip_cfuns.c:1694-1989), the C entry
+ * behind dip2dc (pyseistr/dip2d.py:115) and dip3dc (pyseistr/dip3d.py:59).
+ * n3 == 1: inline dip only, dip[N]; otherwise inline then crossline,
+ * dip[2N].  mask (nullable, length N): samples whose PWD stencil touches a
+ * zero of mask are excluded (mask32, dip_cfuns.c:914).  init (nullable, N
+ * or 2N): initial dip instead of the reference's zeros (the commented
+ * idip/xdip inputs, dip_cfuns.c:1861-1875).  eps_dv is accepted and has no
+ * effect, as in the reference (its shared `eps` static is reset to 1).
+ * rep (nullable, host memory): iteration record; filling it synchronises. */
+int sh_dip(sh_ctx *ctx, const float *data, const float *mask, const float *init,
+           int n1, int n2, int n3, int niter, int liter, int order, float eps_dv,
+           int r1, int r2, int r3, float *dip, sh_dip_report *rep);
+
+/* 3-D plane-wave spray, slots visited by distance from the source trace.
+ * Replaces the spray loop of csomean3d/csomf3d (pyseistr/src/
+ * sof3d_cfuns.c:1639-1705) and the pure pwspray3d (pyseistr/pwspray3d.py:2).
+ * u has the reference's layout u[(trace*np + slot)*n1 + i1],
+ * np = (2ns2+1)(2ns3+1); eps is used as given (predictor e = eps*eps). */
+int sh_pwspray3d(sh_ctx *ctx, const float *din, const float *dipi, const float *dipx,
+                 int n1, int n2, int n3, int ns2, int ns3, int order, float eps, float *u);
+
+/* Structure-oriented mean: spray, then the plain mean over all np slots.
+ * Replaces sof3dcfun.csomean3d (pyseistr/src/sof3d_cfuns.c:1355-1552);
+ * eps is ignored and 0.01 used, as in the reference (1402). */
+int sh_somean3d(sh_ctx *ctx, const float *din, const float *dipi, const float *dipx,
+                int n1, int n2, int n3, int r1, int r2, float eps, int order, float *out);
+
+/* Structure-oriented median: spray, then the median of the nmf linear slot
+ * indices centred at (np-1)/2 (option 1) or the space-varying median
+ * (option != 1).  Replaces sof3dcfun.csomf3d (pyseistr/src/
+ * sof3d_cfuns.c:1554-1753); eps forced to 0.01 as in the reference (1586). */
+int sh_somf3d(sh_ctx *ctx, const float *din, const float *dipi, const float *dipx,
+              int n1, int n2, int n3, int r1, int r2, int nmf, int option, int order,
+              float eps, float *out);
+
+/* 2-D plane-wave spray of one n1 x n2 slice, forward (x -> u) or adjoint
+ * (u -> x).  Replaces pwspray_lop (pyseistr/src/sof_cfuns.c:948-1031) and
+ * the pure pwspray2d (pyseistr/pwspray2d.py:130).  u layout
+ * u[(i2*(2ns+1) + slot)*n1 + i1]. */
+int sh_pwspray2d(sh_ctx *ctx, int adj, const float *dip, int n1, int n2, int ns, int order,
+                 float eps, float *x, float *u);
+
+/* Normalised triangle-weighted structure-oriented smoothing per n3 slice
+ * (or its adjoint).  Replaces sofcfun.csomean2d (pyseistr/src/
+ * sof_cfuns.c:1433-1532) behind somean2dc (pyseistr/somean2d.py:36). */
+int sh_somean2d(sh_ctx *ctx, const float *dn, const float *dip, int n1, int n2, int n3,
+                int ns, int order, int adj, float eps, float *out);
+
+/* Structure-oriented median (option 1) or space-varying median per n3
+ * slice.  Replaces sofcfun.csomf2d (pyseistr/src/sof_cfuns.c:1534-1672)
+ * behind somf2dc (pyseistr/somf2d.py:60). */
+int sh_somf2d(sh_ctx *ctx, const float *dn, const float *dip, int n1, int n2, int n3,
+              int ns, int nmf, int option, int order, float eps, float *out);
+
+/* PWD-regularised interpolation of unknown samples.  Replaces
+ * soint3dcfun.csoint3d (pyseistr/src/soint3d_cfuns.c:2405-2508) behind
+ * soint3dc (pyseistr/soint3d.py:65).  iters (nullable, host): solver
+ * iterations run; filling it synchronises. */
+int sh_soint3d(sh_ctx *ctx, const float *din, const float *mask, const float *dipi,
+               const float *dipx, int n1, int n2, int n3, int nw, int nj1, int nj2,
+               int niter, int drift, int seed, int hasmask, float var, float *out, int *iters);
+
+/* 2-D analogue without preconditioning.  Replaces soint2dcfun.csoint2d
+ * (pyseistr/src/soint2d_cfuns.c:2260-2419) for prec == 0; twoplane != 0
+ * returns the data unchanged, as the reference does. */
+int sh_soint2d(sh_ctx *ctx, const float *din, const float *mask, const float *dip,
+               int n1, int n2, int nw, int nj1, int niter, int drift, int hasmask,
+               int twoplane, float *out, int *iters);
+
+/* Masked shaping-CG interpolation with structure-oriented smoothing as the
+ * shaping operator.  Replaces soint3dcfun.csint3d (pyseistr/src/
+ * soint3d_cfuns.c:2510-2615) behind sint3dc (pyseistr/sint.py:97). */
+int sh_sint3d(sh_ctx *ctx, const float *din, const float *dipi, const float *dipx,
+              const float *mask, int n1, int n2, int n3, int niter, int ns1, int ns2,
+              int order1, int order2, float eps, float *out, int *iters);
+
+/* 2-D analogue.  Replaces soint2dcfun.csint2d (pyseistr/src/
+ * soint2d_cfuns.c:2421-2538) behind sint2dc (pyseistr/sint.py:61). */
+int sh_sint2d(sh_ctx *ctx, const float *din, const float *dip, const float *mask,
+              int n1, int n2, int niter, int ns, int order, float eps, float *out, int *iters);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SEISTR_HIP_H */

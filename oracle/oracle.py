@@ -61,7 +61,7 @@ def aderfilter(p, nw=2, dtype=np.float32):
 
 def pwd(x, p, shape, axis, nw=2, nj=1, drift=0, der=0, dtype=np.float32):
     """allpass1 (axis 2) / allpass2 (axis 3) on a flat F-order volume."""
-    n1, n2, n3 = shape
+    n1, n2, n3 = (int(v) for v in shape)
     x = _arr(x, dtype); p = _arr(p, dtype)
     y = np.zeros_like(x)
     _lib(dtype).or_pwd(n1, n2, n3, axis, nw, nj, drift, der, _ptr(x), _ptr(p), _ptr(y))
@@ -69,7 +69,7 @@ def pwd(x, p, shape, axis, nw=2, nj=1, drift=0, der=0, dtype=np.float32):
 
 
 def allpass3(x_or_y, pp, qq, shape, adj=0, nw=2, nj1=1, nj2=1, drift=0, dtype=np.float32):
-    n1, n2, n3 = shape
+    n1, n2, n3 = (int(v) for v in shape)
     n = n1 * n2 * n3
     pp = _arr(pp, dtype); qq = _arr(qq, dtype)
     if adj:
@@ -81,7 +81,7 @@ def allpass3(x_or_y, pp, qq, shape, adj=0, nw=2, nj1=1, nj2=1, drift=0, dtype=np
 
 
 def allpass21(x_or_y, pp, shape, adj=0, nw=2, nj=1, drift=0, dtype=np.float32):
-    n1, n2 = shape
+    n1, n2 = (int(v) for v in shape)
     pp = _arr(pp, dtype)
     a = _arr(x_or_y, dtype); b = np.zeros(n1 * n2, dtype)
     x, y = (b, a) if adj else (a, b)
@@ -91,14 +91,14 @@ def allpass21(x_or_y, pp, shape, adj=0, nw=2, nj=1, drift=0, dtype=np.float32):
 
 def trianglen(x, shape, rect, dtype=np.float32):
     x = _arr(x, dtype).copy()
-    _lib(dtype).or_trianglen(*shape, *rect, _ptr(x))
+    _lib(dtype).or_trianglen(*(int(v) for v in shape), *(int(r) for r in rect), _ptr(x))
     return x
 
 
 def dip(din, shape, niter=5, liter=20, order=2, eps_dv=0.01, rect=(1, 1, 1), mask=None,
         p_init=None, dtype=np.float32, report=False):
     """dipc: returns flat dips (N for n3==1, else 2N = inline || crossline)."""
-    n1, n2, n3 = shape
+    n1, n2, n3 = (int(v) for v in shape)
     n = n1 * n2 * n3
     nd = 1 if n3 == 1 else 2
     din = _arr(din, dtype)
@@ -106,8 +106,8 @@ def dip(din, shape, niter=5, liter=20, order=2, eps_dv=0.01, rect=(1, 1, 1), mas
     pi = _arr(p_init, dtype) if p_init is not None else None
     out = np.zeros(nd * n, dtype)
     rep = np.zeros(2 * niter * nd, np.int32)
-    _lib(dtype).or_dip(_ptr(din), _ptr(m), n1, n2, n3, niter, liter, order, _real(dtype)(eps_dv),
-                       *rect, _ptr(pi), _ptr(out), _ptr(rep))
+    _lib(dtype).or_dip(_ptr(din), _ptr(m), n1, n2, n3, int(niter), int(liter), int(order), _real(dtype)(eps_dv),
+                       *(int(r) for r in rect), _ptr(pi), _ptr(out), _ptr(rep))
     return (out, rep.reshape(nd, niter, 2)) if report else out
 
 
@@ -132,7 +132,7 @@ def predict2(tr1, tr2, pp1, pp2, forw1=0, forw2=0, nw=2, e=1e-4, dtype=np.float3
 
 def pwspray2d(x, dip, shape, ns, order=2, eps=0.01, adj=0, dtype=np.float32):
     """Forward: x (n1*n2) -> u (n1*(2ns+1)*n2); adjoint: u -> x."""
-    n1, n2 = shape
+    n1, n2 = (int(v) for v in shape)
     ns2 = 2 * ns + 1
     d = _arr(dip, dtype)
     if adj:
@@ -144,7 +144,7 @@ def pwspray2d(x, dip, shape, ns, order=2, eps=0.01, adj=0, dtype=np.float32):
 
 
 def pwsmooth2d(x, dip, shape, ns, order=2, eps=0.01, adj=0, dtype=np.float32):
-    n1, n2 = shape
+    n1, n2 = (int(v) for v in shape)
     d = _arr(dip, dtype)
     a = _arr(x, dtype).copy(); b = np.zeros(n1 * n2, dtype)
     trace, smooth = (b, a) if adj else (a, b)
@@ -153,14 +153,14 @@ def pwsmooth2d(x, dip, shape, ns, order=2, eps=0.01, adj=0, dtype=np.float32):
 
 
 def somean2d(dn, dip, shape, ns, order=2, eps=0.01, adj=0, dtype=np.float32):
-    n1, n2, n3 = shape
+    n1, n2, n3 = (int(v) for v in shape)
     a = _arr(dn, dtype); d = _arr(dip, dtype); o = np.zeros_like(a)
     _lib(dtype).or_somean2d(_ptr(a), _ptr(d), n1, n2, n3, ns, order, adj, _real(dtype)(eps), _ptr(o))
     return o
 
 
 def somf2d(dn, dip, shape, ns, order=2, eps=0.01, option=1, nmf=None, dtype=np.float32):
-    n1, n2, n3 = shape
+    n1, n2, n3 = (int(v) for v in shape)
     a = _arr(dn, dtype); d = _arr(dip, dtype); o = np.zeros_like(a)
     nmf = 2 * ns + 1 if nmf is None else nmf
     _lib(dtype).or_somf2d(_ptr(a), _ptr(d), n1, n2, n3, ns, nmf, option, order, _real(dtype)(eps), _ptr(o))
@@ -176,7 +176,7 @@ def spray3_plan(ns2, ns3):
 
 def pwspray3d(dn, dipi, dipx, shape, ns2, ns3, order=2, eps=0.01, dtype=np.float32):
     """Returns u flat with layout [(trace*np + slot)*n1 + i1]."""
-    n1, n2, n3 = shape
+    n1, n2, n3 = (int(v) for v in shape)
     np_ = (2 * ns2 + 1) * (2 * ns3 + 1)
     a = _arr(dn, dtype); p = _arr(dipi, dtype); q = _arr(dipx, dtype)
     u = np.zeros(n1 * n2 * n3 * np_, dtype)
@@ -185,14 +185,14 @@ def pwspray3d(dn, dipi, dipx, shape, ns2, ns3, order=2, eps=0.01, dtype=np.float
 
 
 def somean3d(dn, dipi, dipx, shape, r1, r2, order=2, dtype=np.float32):
-    n1, n2, n3 = shape
+    n1, n2, n3 = (int(v) for v in shape)
     a = _arr(dn, dtype); p = _arr(dipi, dtype); q = _arr(dipx, dtype); o = np.zeros_like(a)
     _lib(dtype).or_somean3d(_ptr(a), _ptr(p), _ptr(q), n1, n2, n3, r1, r2, order, _ptr(o))
     return o
 
 
 def somf3d(dn, dipi, dipx, shape, r1, r2, order=2, option=1, nmf=None, dtype=np.float32):
-    n1, n2, n3 = shape
+    n1, n2, n3 = (int(v) for v in shape)
     a = _arr(dn, dtype); p = _arr(dipi, dtype); q = _arr(dipx, dtype); o = np.zeros_like(a)
     nmf = 2 * r1 * r2 + 1 if nmf is None else nmf
     _lib(dtype).or_somf3d(_ptr(a), _ptr(p), _ptr(q), n1, n2, n3, r1, r2, nmf, option, order, _ptr(o))
@@ -214,7 +214,7 @@ def randn(seed, n):
 
 def soint3d(din, mask, dipi, dipx, shape, order=1, niter=100, njs=(1, 1), drift=0, seed=202223,
             hasmask=1, var=0.0, dtype=np.float32):
-    n1, n2, n3 = shape
+    n1, n2, n3 = (int(v) for v in shape)
     a = _arr(din, dtype); m = _arr(mask, dtype); p = _arr(dipi, dtype); q = _arr(dipx, dtype)
     o = np.zeros_like(a)
     _lib(dtype).or_soint3d(_ptr(a), _ptr(m), _ptr(p), _ptr(q), n1, n2, n3, order, njs[0], njs[1], niter,
@@ -224,7 +224,7 @@ def soint3d(din, mask, dipi, dipx, shape, order=1, niter=100, njs=(1, 1), drift=
 
 def soint2d(din, mask, dip, shape, order=1, niter=100, njs=(1, 1), drift=0, hasmask=1, twoplane=0,
             dtype=np.float32):
-    n1, n2 = shape
+    n1, n2 = (int(v) for v in shape)
     a = _arr(din, dtype); m = _arr(mask, dtype); p = _arr(dip, dtype); o = np.zeros_like(a)
     _lib(dtype).or_soint2d(_ptr(a), _ptr(m), _ptr(p), n1, n2, order, njs[0], niter, drift, hasmask,
                            twoplane, _ptr(o))
@@ -232,7 +232,7 @@ def soint2d(din, mask, dip, shape, order=1, niter=100, njs=(1, 1), drift=0, hasm
 
 
 def sint2d(din, mask, dip, shape, niter=100, eps=0.01, ns=1, order=1, dtype=np.float32):
-    n1, n2 = shape
+    n1, n2 = (int(v) for v in shape)
     a = _arr(din, dtype); m = _arr(mask, dtype); p = _arr(dip, dtype); o = np.zeros_like(a)
     _lib(dtype).or_sint2d(_ptr(a), _ptr(p), _ptr(m), n1, n2, niter, ns, order, _real(dtype)(eps), _ptr(o))
     return o
@@ -240,7 +240,7 @@ def sint2d(din, mask, dip, shape, niter=100, eps=0.01, ns=1, order=1, dtype=np.f
 
 def sint3d(din, mask, dipi, dipx, shape, niter=100, eps=0.01, ns1=1, ns2=1, order1=1, order2=1,
            dtype=np.float32):
-    n1, n2, n3 = shape
+    n1, n2, n3 = (int(v) for v in shape)
     a = _arr(din, dtype); m = _arr(mask, dtype); p = _arr(dipi, dtype); q = _arr(dipx, dtype)
     o = np.zeros_like(a)
     _lib(dtype).or_sint3d(_ptr(a), _ptr(p), _ptr(q), _ptr(m), n1, n2, n3, niter, ns1, ns2, order1, order2,

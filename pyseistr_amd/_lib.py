@@ -1,0 +1,148 @@
+"""ctypes binding of libseistr_hip.so (the gfx950 C ABI in include/seistr_hip.h).
+
+The library is the only compute path: if it is missing or no GPU is visible
+the calls raise; there is no CPU fallback.  PyTorch is used for device memory
+and the current HIP stream only.
+"""
+import ctypes
+import os
+import threading
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libseistr_hip.so")
+
+_lib = None
+_tls = threading.local()
+
+c_int, c_float, c_void_p = ctypes.c_int, ctypes.c_float, ctypes.c_void_p
+c_int_p = ctypes.POINTER(ctypes.c_int)
+
+
+class SeistrError(RuntimeError):
+    """Raised for a non-zero status of the HIP library."""
+
+
+class DipReport(ctypes.Structure):
+    _fields_ = [("ndir", c_int), ("niter", c_int),
+                ("cg_iters", (c_int * 64) * 2), ("ls_step", (c_int * 64) * 2)]
+
+
+_SIGS = {
+    "sh_ctx_create": (c_void_p, [c_void_p]),
+    "sh_ctx_destroy": (None, [c_void_p]),
+    "sh_ctx_set_stream": (c_int, [c_void_p, c_void_p]),
+    "sh_ctx_trim": (c_int, [c_void_p]),
+    "sh_last_error": (ctypes.c_char_p, [c_void_p]),
+    "sh_version": (c_int, []),
+    "sh_sync": (c_int, [c_void_p]),
+    "sh_dip": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_int, c_int,
+                       c_float, c_int, c_int, c_int, c_void_p, ctypes.POINTER(DipReport)]),
+    "sh_pwspray3d": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_int,
+                             c_int, c_float, c_void_p]),
+    "sh_somean3d": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_int,
+                            c_float, c_int, c_void_p]),
+    "sh_somf3d": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_int,
+                          c_int, c_int, c_int, c_float, c_void_p]),
+    "sh_pwspray2d": (c_int, [c_void_p, c_int, c_void_p, c_int, c_int, c_int, c_int, c_float, c_void_p,
+                             c_void_p]),
+    "sh_somean2d": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_int, c_int,
+                            c_float, c_void_p]),
+    "sh_somf2d": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_int, c_int, c_int,
+                          c_float, c_void_p]),
+    "sh_soint3d": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_int,
+                           c_int, c_int, c_int, c_int, c_int, c_int, c_float, c_void_p, c_int_p]),
+    "sh_soint2d": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_int,
+                           c_int, c_int, c_int, c_void_p, c_int_p]),
+    "sh_sint3d": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_int,
+                          c_int, c_int, c_int, c_int, c_float, c_void_p, c_int_p]),
+    "sh_sint2d": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_int,
+                          c_float, c_void_p, c_int_p]),
+}
+
+EXPORTED = tuple(_SIGS)
+
+
+def load():
+    """Load and declare the library (no GPU needed for loading)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise SeistrError(f"{LIB_PATH} is not built; run __graft_entry__.build() "
+                              "(hipcc --offload-arch=gfx950)")
+        lib = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            if not hasattr(lib, name):
+                continue  # reported by missing_symbols()
+            f = getattr(lib, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def missing_symbols():
+    lib = load()
+    return [name for name in _SIGS if not hasattr(lib, name)]
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise SeistrError("pyseistr_amd needs a ROCm GPU (torch.cuda.is_available() is False)")
+    return torch
+
+
+def context():
+    """Per-thread library context bound to torch's current stream."""
+    torch = _torch()
+    lib = load()
+    dev = torch.cuda.current_device()
+    ctxs = getattr(_tls, "ctxs", None)
+    if ctxs is None:
+        ctxs = _tls.ctxs = {}
+    ctx = ctxs.get(dev)
+    if ctx is None:
+        ctx = lib.sh_ctx_create(None)
+        if not ctx:
+            raise SeistrError("sh_ctx_create failed")
+        ctxs[dev] = ctx
+    stream = torch.cuda.current_stream().cuda_stream
+    lib.sh_ctx_set_stream(ctx, c_void_p(stream))
+    return ctx
+
+
+def check(ctx, rc, what):
+    if rc != 0:
+        msg = load().sh_last_error(ctx)
+        msg = msg.decode() if msg else ""
+        if rc == -1:
+            raise ValueError(f"{what}: {msg}")
+        if rc == -5:
+            raise NotImplementedError(f"{what}: {msg}")
+        raise SeistrError(f"{what}: {msg} (status {rc})")
+
+
+def ptr(t):
+    return c_void_p(t.data_ptr()) if t is not None else None
+
+
+def to_device(a):
+    """numpy (any order) -> flat Fortran-order float32 device tensor."""
+    torch = _torch()
+    if isinstance(a, torch.Tensor):
+        if a.is_cuda and a.dtype == torch.float32 and a.dim() == 1 and a.is_contiguous():
+            return a
+        raise TypeError("device tensors must be flat contiguous float32 (Fortran-order data)")
+    flat = np.ascontiguousarray(np.asarray(a, dtype=np.float32).ravel(order="F"))
+    return torch.from_numpy(flat).cuda()
+
+
+def empty(n):
+    torch = _torch()
+    return torch.empty(int(n), dtype=torch.float32, device="cuda")
+
+
+def to_host(t):
+    return t.cpu().numpy()

@@ -1,0 +1,109 @@
+// Context management, error reporting and the deterministic partial-sum
+// reduction shared by all modules.
+#include "common.hpp"
+
+int sh_fail(sh_ctx *ctx, int code, const char *fmt, ...) {
+    if (ctx) {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof buf, fmt, ap);
+        va_end(ap);
+        ctx->err = buf;
+    }
+    return code;
+}
+
+int sh_slot(sh_ctx *ctx, int id, size_t bytes, void **out) {
+    if (id < 0 || id >= SLOT_COUNT) return sh_fail(ctx, SH_EINVAL, "bad workspace slot %d", id);
+    if (bytes == 0) bytes = 256;
+    if (ctx->slot_size[id] < bytes) {
+        if (ctx->slot_ptr[id]) {
+            // the stream may still use the old buffer
+            hipError_t e = hipStreamSynchronize(ctx->stream);
+            if (e != hipSuccess) return sh_fail(ctx, SH_EHIP, "hipStreamSynchronize: %s", hipGetErrorString(e));
+            (void)hipFree(ctx->slot_ptr[id]);
+            ctx->slot_ptr[id] = nullptr;
+            ctx->slot_size[id] = 0;
+        }
+        size_t sz = (bytes + 4095) & ~(size_t)4095;
+        void *p = nullptr;
+        hipError_t e = hipMalloc(&p, sz);
+        if (e != hipSuccess || !p)
+            return sh_fail(ctx, SH_EALLOC, "hipMalloc(%zu) for workspace slot %d: %s", sz, id,
+                           hipGetErrorString(e));
+        ctx->slot_ptr[id] = p;
+        ctx->slot_size[id] = sz;
+    }
+    *out = ctx->slot_ptr[id];
+    return SH_OK;
+}
+
+int sh_launch_check(sh_ctx *ctx, const char *what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return sh_fail(ctx, SH_EHIP, "launch of %s failed: %s", what, hipGetErrorString(e));
+    return SH_OK;
+}
+
+__global__ void __launch_bounds__(1024) reduce_partials_kernel(const double *part, int nblk, int K, double *out) {
+    __shared__ double sh[16];
+    for (int k = 0; k < K; k++) {
+        double s = 0.0;
+        for (int i = threadIdx.x; i < nblk; i += blockDim.x) s += part[(size_t)k * nblk + i];
+        double t = block_sum(s, sh);
+        if (threadIdx.x == 0) out[k] = t;
+        __syncthreads();
+    }
+}
+
+int reduce_partials(sh_ctx *ctx, const double *part, int nblk, int K, double *out) {
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(1024), 0, ctx->stream, part, nblk, K, out);
+    return sh_launch_check(ctx, "reduce_partials");
+}
+
+extern "C" {
+
+sh_ctx *sh_ctx_create(void *hip_stream) {
+    sh_ctx *ctx = new (std::nothrow) sh_ctx();
+    if (!ctx) return nullptr;
+    ctx->stream = (hipStream_t)hip_stream;
+    ctx->slot_ptr.assign(SLOT_COUNT, nullptr);
+    ctx->slot_size.assign(SLOT_COUNT, 0);
+    (void)hipGetDevice(&ctx->device);
+    return ctx;
+}
+
+int sh_ctx_trim(sh_ctx *ctx) {
+    if (!ctx) return SH_ENULL;
+    SH_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    for (int i = 0; i < SLOT_COUNT; i++) {
+        if (ctx->slot_ptr[i]) (void)hipFree(ctx->slot_ptr[i]);
+        ctx->slot_ptr[i] = nullptr;
+        ctx->slot_size[i] = 0;
+    }
+    return SH_OK;
+}
+
+void sh_ctx_destroy(sh_ctx *ctx) {
+    if (!ctx) return;
+    sh_ctx_trim(ctx);
+    delete ctx;
+}
+
+int sh_ctx_set_stream(sh_ctx *ctx, void *hip_stream) {
+    if (!ctx) return SH_ENULL;
+    ctx->stream = (hipStream_t)hip_stream;
+    return SH_OK;
+}
+
+const char *sh_last_error(const sh_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int sh_version(void) { return 100; }
+
+int sh_sync(sh_ctx *ctx) {
+    if (!ctx) return SH_ENULL;
+    SH_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return SH_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,209 @@
+// Shared infrastructure of the gfx950 structural-filter library: context and
+// workspace, error reporting, deterministic block reductions, and the
+// plane-wave-destruction filter coefficients.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/seistr_hip.h"
+
+#define SH_HD __host__ __device__
+
+// ---------------------------------------------------------------------------
+// Context: stream, last error, and a set of cached device buffers ("slots")
+// that grow on demand and are reused across calls on the same context.
+// ---------------------------------------------------------------------------
+struct sh_ctx {
+    hipStream_t stream = nullptr;
+    std::string err;
+    std::vector<void *> slot_ptr;
+    std::vector<size_t> slot_size;
+    int device = 0;
+};
+
+int sh_fail(sh_ctx *ctx, int code, const char *fmt, ...);
+
+#define SH_HIP(ctx, call)                                                        \
+    do {                                                                         \
+        hipError_t e_ = (call);                                                  \
+        if (e_ != hipSuccess)                                                    \
+            return sh_fail((ctx), SH_EHIP, "%s failed: %s", #call,               \
+                           hipGetErrorString(e_));                               \
+    } while (0)
+
+#define SH_TRY(expr)                                                             \
+    do {                                                                         \
+        int rc_ = (expr);                                                        \
+        if (rc_ != SH_OK) return rc_;                                            \
+    } while (0)
+
+// Device buffer cached in slot `id` of at least `bytes`; contents persist
+// between calls only as scratch (callers must initialise what they read).
+int sh_slot(sh_ctx *ctx, int id, size_t bytes, void **out);
+
+// Launch-error check after a kernel launch.
+int sh_launch_check(sh_ctx *ctx, const char *what);
+
+// Slot ids (each module owns a range so that nested calls do not collide).
+enum {
+    SLOT_RED = 0,        // reduction partials (double)
+    SLOT_SCAL = 1,       // device scalars
+    SLOT_DIP0 = 2,       // dip module: 2..19
+    SLOT_SPRAY0 = 20,    // spray module: 20..39
+    SLOT_INTERP0 = 40,   // interpolation module: 40..59
+    SLOT_COUNT = 64
+};
+
+// ---------------------------------------------------------------------------
+// Deterministic reductions.  Each block writes its partial sums (double) to
+// part[k*nblk + block]; reduce_partials sums them in a fixed order.
+// ---------------------------------------------------------------------------
+__device__ inline double wave_sum(double v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    return v;
+}
+
+// Sum `v` over the block (blockDim.x a multiple of 64, at most 1024).
+// Returns the total in thread 0; `sh` needs 16 doubles.
+__device__ inline double block_sum(double v, double *sh) {
+    int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    int nw = (blockDim.x * blockDim.y * blockDim.z + 63) >> 6;
+    v = wave_sum(v);
+    __syncthreads();
+    if (lane == 0) sh[wid] = v;
+    __syncthreads();
+    double t = 0.0;
+    if (threadIdx.x == 0)
+        for (int w = 0; w < nw; w++) t += sh[w];
+    return t;
+}
+
+// Linear thread id for 2-D/3-D blocks, used with block_sum.
+__device__ inline int flat_tid() {
+    return threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
+}
+
+__device__ inline double block_sum_flat(double v, double *sh) {
+    int t = flat_tid();
+    int lane = t & 63, wid = t >> 6;
+    int nw = (blockDim.x * blockDim.y * blockDim.z + 63) >> 6;
+    v = wave_sum(v);
+    __syncthreads();
+    if (lane == 0) sh[wid] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (t == 0)
+        for (int w = 0; w < nw; w++) s += sh[w];
+    __syncthreads();
+    return s;
+}
+
+// Sum K arrays of partials (each nblk long, stacked) into out[k] with one
+// block of 1024 threads; fixed summation order.
+__global__ void reduce_partials_kernel(const double *part, int nblk, int K, double *out);
+int reduce_partials(sh_ctx *ctx, const double *part, int nblk, int K, double *out);
+
+// ---------------------------------------------------------------------------
+// All-pass PWD filter coefficients (maximally flat, order nw).  The table
+// and factor arithmetic follow apfilt_init/passfilter/aderfilter
+// (pyseistr/src/dip_cfuns.c:833-910): factors (nf-j-p) and (p+j+1) are
+// formed in float, products are accumulated in double, the result rounded
+// to float, with contraction disabled so that the coefficients equal the
+// reference's bit for bit.
+// ---------------------------------------------------------------------------
+constexpr double apf_b(int nw, int k) {
+    int nf = 2 * nw;
+    double v = 1.0;
+    for (int j = 0; j < nf; j++)
+        v *= (j < nf - k) ? (k + j + 1.0) / (2 * (2 * j + 1) * (j + 1)) : 1.0 / (2 * (2 * j + 1));
+    return v;
+}
+
+template <int NW>
+struct ApfTable {
+    double b[2 * NW + 1];
+    constexpr ApfTable() : b() {
+        for (int k = 0; k <= 2 * NW; k++) b[k] = apf_b(NW, k);
+    }
+};
+
+#pragma clang fp contract(off)
+template <int NW>
+SH_HD inline double apf_factor(int k, int j, float p) {
+    constexpr int NF = 2 * NW;
+    return (j < NF - k) ? (double)((float)(NF - j) - p) : (double)((p + (float)j) + 1.0f);
+}
+
+template <int NW>
+SH_HD inline void apf_pass(float p, float *a) {
+    constexpr ApfTable<NW> T;
+    constexpr int NF = 2 * NW;
+#pragma unroll
+    for (int k = 0; k <= NF; k++) {
+        double v = T.b[k];
+#pragma unroll
+        for (int j = 0; j < NF; j++) v *= apf_factor<NW>(k, j, p);
+        a[k] = (float)v;
+    }
+}
+
+template <int NW>
+SH_HD inline void apf_der(float p, float *a) {
+    constexpr ApfTable<NW> T;
+    constexpr int NF = 2 * NW;
+#pragma unroll
+    for (int k = 0; k <= NF; k++) {
+        double acc = 0.0;
+#pragma unroll
+        for (int i = 0; i < NF; i++) {
+            double term = -1.0;
+#pragma unroll
+            for (int j = 0; j < NF; j++) {
+                if (j != i) term *= apf_factor<NW>(k, j, p);
+                else if (j < NF - k) term *= -1;
+            }
+            acc += term;
+        }
+        a[k] = (float)(acc * T.b[k]);
+    }
+}
+
+// Nearest-integer shift of the drift variant (PS_NINT, dip_cfuns.c:9).
+SH_HD inline int nint_ref(float x) { return (int)((x) > 0.0 ? (x) + 0.5 : (x)-0.5); }
+
+inline int ceil_div(long a, long b) { return (int)((a + b - 1) / b); }
+
+// Dispatch a template on the PWD order (1..4); returns SH_EUNSUP otherwise.
+#define SH_DISPATCH_NW(ctx, nw, NAME, ...)                                       \
+    switch (nw) {                                                                \
+    case 1: { constexpr int NAME = 1; __VA_ARGS__; } break;                      \
+    case 2: { constexpr int NAME = 2; __VA_ARGS__; } break;                      \
+    case 3: { constexpr int NAME = 3; __VA_ARGS__; } break;                      \
+    case 4: { constexpr int NAME = 4; __VA_ARGS__; } break;                      \
+    default: return sh_fail((ctx), SH_EUNSUP, "PWD order %d not supported (1..4)", (nw)); \
+    }
+
+// ---------------------------------------------------------------------------
+// 2-D structure-oriented smoothing of n3 independent n1 x n2 slices held in
+// the trace-fast slice layout [slice][sample][trace] (spray.hip); used by
+// somean2d and by the sint solvers (interp.hip).
+// ---------------------------------------------------------------------------
+struct Pwsmooth2 {
+    int n1, n2, n3, ns, order;
+    float e;            // predictor regularisation (eps^2)
+    const float *dip;   // [slice][k][trace]
+    float *ww;          // normalisation 1 / smooth(ones)
+    float *U;           // slots [slot][slice][k][trace]
+    float *C;           // adjoint chains [2][...]
+    float *scr, *tr;    // per-thread solver scratch and running trace
+    long nthr;
+};
+int pws2_setup(sh_ctx *ctx, Pwsmooth2 &P, int slot_base);
+int pws2_apply(sh_ctx *ctx, Pwsmooth2 &P, int adj, int add, float *trace, float *smooth);
+int sh_transpose_internal(sh_ctx *ctx, const float *src, float *dst, int n1, long ntr, bool to_fast);

@@ -1,0 +1,614 @@
+// Local-slope estimation on gfx950: plane-wave destruction (PWD) stencils,
+// triangle shaping smoothing, the shaping-regularised conjugate-gradient
+// smooth division and the Gauss-Newton loop with backtracking line search.
+//
+// Algorithm: dip3 / ps_divne / ps_conjgrad / ps_trianglen_lop / allpass1-2
+// of pyseistr/src/dip_cfuns.c (cited per kernel).  Design:
+//  * every kernel streams HBM once, coalesced along n1;
+//  * the CG keeps all scalars on the device, so a whole dip estimate is
+//    queued without host round trips; convergence and line-search
+//    acceptance are device flags that turn later launches into no-ops;
+//  * the triangle smoother is the direct (2nb-1)-tap stencil with even
+//    reflection, staged through LDS tiles with halos, the vector updates of
+//    the CG fused into the first/last smoothing pass.
+#include "common.hpp"
+
+#include <algorithm>
+
+namespace {
+
+constexpr int SM_CH1 = 512;   // samples of a line per block, axis-1 pass
+constexpr int SM_CHS = 128;   // samples along the axis per block, strided pass
+constexpr int SM_HMAX = 127;  // largest supported smoothing half-width (rect-1)
+
+// Device-resident scalar state of one dip direction.
+struct DipScal {
+    double usum;        // sum u2^2 of the current residual
+    double divn_norm;   // sqrt(N / sum den^2)
+    double gn, gnp, g0; // CG gradient norms
+    double alpha;       // gn / gnp (direction update)
+    double alpha2;      // -gn / beta (model update, applied lazily)
+    int accepted, ls_k;
+    int cg_done, pending, cur, cg_iters, divn_zero, pad;
+    int rec[2 * 64];    // per GN iteration: cg iterations, accepted step
+};
+
+enum { F_USUM = 0, F_DIVN, F_R0, F_GN, F_BETA, F_LS };
+
+// fold position q (may lie outside [0,n)) back by even reflection
+__device__ inline int fold_idx(int q, int n) {
+    if (q >= 0 && q < n) return q;
+    if (q < 0) q = -1 - q;
+    int m = q % (2 * n);
+    return (m < n) ? m : 2 * n - 1 - m;
+}
+
+// ---------------------------------------------------------------------------
+// Reductions + scalar logic, one block of 1024 threads.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(1024) dip_finalize(const double *part, int nblk, int K, DipScal *sc,
+                                                     int mode, int it, long n, float tol, int k_ls) {
+    __shared__ double sh[16];
+    __shared__ double tot[4];
+    for (int k = 0; k < K; k++) {
+        double s = 0.0;
+        for (int i = threadIdx.x; i < nblk; i += blockDim.x) s += part[(size_t)k * nblk + i];
+        double t = block_sum(s, sh);
+        if (threadIdx.x == 0) tot[k] = t;
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    switch (mode) {
+    case F_USUM:
+        sc->usum = tot[0];
+        break;
+    case F_DIVN:  // dip_cfuns.c:810-817: zero denominator -> zero ratio
+        sc->accepted = 0;
+        sc->cur = 0;
+        sc->pending = 0;
+        sc->cg_iters = 0;
+        if (tot[0] == 0.0) {
+            sc->divn_zero = 1;
+            sc->cg_done = 1;
+            sc->cg_iters = -1;
+        } else {
+            sc->divn_zero = 0;
+            sc->cg_done = 0;
+            sc->divn_norm = sqrt((double)n / tot[0]);
+        }
+        break;
+    case F_R0:  // dip_cfuns.c:300-304: zero residual -> zero model
+        if (!sc->divn_zero && tot[0] == 0.0) sc->cg_done = 1;
+        break;
+    case F_GN:  // dip_cfuns.c:337-363
+        if (sc->cg_done) break;
+        if (sc->pending) sc->cur ^= 1;
+        sc->pending = 0;
+        sc->gn = tot[0];
+        if (it == 0) {
+            sc->g0 = sc->gn;
+        } else {
+            double alpha = sc->gn / sc->gnp, dg = sc->gn / sc->g0;
+            if (alpha < (double)tol || dg < (double)tol) {
+                sc->cg_done = 1;
+                break;
+            }
+            sc->alpha = alpha;
+        }
+        break;
+    case F_BETA:  // dip_cfuns.c:367-380 (eps = 1)
+        if (sc->cg_done) break;
+        {
+            double beta = tot[0] + 1.0 * (tot[1] - tot[2]);
+            sc->alpha2 = -sc->gn / beta;
+            sc->gnp = sc->gn;
+            sc->pending = 1;
+            sc->cg_iters = it + 1;
+        }
+        break;
+    case F_LS:  // dip_cfuns.c:1667-1687: accept on strict decrease
+        if (sc->accepted) break;
+        if (tot[0] < sc->usum || k_ls == 7) {
+            sc->accepted = (tot[0] < sc->usum) ? 1 : 2;
+            sc->ls_k = (tot[0] < sc->usum) ? k_ls : 8;
+            sc->usum = tot[0];
+            sc->rec[2 * it] = sc->cg_iters;
+            sc->rec[2 * it + 1] = sc->ls_k;
+        }
+        break;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// PWD stencil kernels (allpass1 / allpass2, dip_cfuns.c:1135-1200, 1399-1465)
+// MODE 0: u2 = C(p) x, sum u2^2
+// MODE 1: u1 = C'(p) x; divne prologue on (num=u2, den=u1): mask, then
+//         scale by 1/hypot(den, 1) (dip_cfuns.c:1656-1663, 802-808), sum den^2
+// MODE 2: line-search trial p = clip(p0 + lam dp), u2 = C(p) x, sum u2^2
+// ---------------------------------------------------------------------------
+template <int NW, int MODE>
+__global__ void __launch_bounds__(256)
+pwd_kernel(const float *__restrict__ x, const float *__restrict__ p, const float *__restrict__ dp,
+           float lam, float pmin, float pmax, float *__restrict__ pout, float *__restrict__ u1,
+           float *__restrict__ u2, const unsigned char *__restrict__ mask, int n1, int n2, int n3,
+           int axis, double *part, const DipScal *sc) {
+    if (MODE == 2 && sc->accepted) return;
+    __shared__ double sh[16];
+    long n = (long)n1 * n2 * n3;
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double acc = 0.0;
+    if (i < n) {
+        int i1 = (int)(i % n1);
+        long r = i / n1;
+        int i2 = (int)(r % n2), i3 = (int)(r / n2);
+        long ip = (axis == 2) ? n1 : (long)n1 * n2;
+        bool valid = i1 >= NW && i1 < n1 - NW && (axis == 2 ? i2 < n2 - 1 : i3 < n3 - 1);
+        float pv;
+        if (MODE == 2) {
+            pv = p[i] + lam * dp[i];
+            if (pv < pmin) pv = pmin;
+            if (pv > pmax) pv = pmax;
+            pout[i] = pv;
+        } else {
+            pv = p[i];
+        }
+        float y = 0.0f;
+        if (valid) {
+            float a[2 * NW + 1];
+            if (MODE == 1) apf_der<NW>(pv, a);
+            else apf_pass<NW>(pv, a);
+#pragma unroll
+            for (int iw = 0; iw <= 2 * NW; iw++) {
+                int is = iw - NW;
+                y += (x[i + is + ip] - x[i - is]) * a[iw];
+            }
+        }
+        if (MODE == 1) {
+            float num = u2[i], den = y;
+            if (mask && mask[i]) { num = 0.0f; den = 0.0f; }
+            double norm = 1.0 / hypot((double)den, 1.0);
+            num = (float)(num * norm);
+            den = (float)(den * norm);
+            u2[i] = num;
+            u1[i] = den;
+            acc = (double)den * den;
+        } else {
+            u2[i] = y;
+            acc = (double)y * y;
+        }
+    }
+    double t = block_sum(acc, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+// CG start (ps_divne tail + ps_conjgrad head, dip_cfuns.c:819-826, 284-303):
+// w = den*norm, r = -(num*norm), x = p = 0; partial sum r^2.
+__global__ void __launch_bounds__(256)
+cg_start_kernel(const float *__restrict__ num, const float *__restrict__ den, float *__restrict__ w,
+                float *__restrict__ r, float *__restrict__ x, float *__restrict__ pc, long n,
+                double *part, const DipScal *sc) {
+    __shared__ double sh[16];
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double acc = 0.0;
+    if (i < n) {
+        float wv = 0.0f, rv = 0.0f;
+        if (!sc->divn_zero) {
+            double nrm = sc->divn_norm;
+            wv = (float)(den[i] * nrm);
+            rv = -(float)(num[i] * nrm);
+        }
+        w[i] = wv;
+        r[i] = rv;
+        x[i] = 0.0f;
+        pc[i] = 0.0f;
+        acc = (double)rv * rv;
+    }
+    double t = block_sum(acc, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+// dp = x (+ alpha2 sx when the last update is still pending)
+__global__ void __launch_bounds__(256)
+cg_result_kernel(const float *__restrict__ xa, const float *__restrict__ xb, const float *__restrict__ sx,
+                 float *__restrict__ dp, long n, const DipScal *sc) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float v = (sc->cur ? xb : xa)[i];
+    if (sc->pending) v = v + (float)sc->alpha2 * sx[i];
+    dp[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// Triangle smoothing passes with fused CG prologue/epilogue.
+// ---------------------------------------------------------------------------
+enum { PRO_PLAIN = 0, PRO_CG = 1 };
+enum { EPI_PLAIN = 0, EPI_GP = 1, EPI_S = 2 };
+
+struct CgArgs {
+    const float *w;
+    float *xa, *xb, *ra, *rb;  // model and residual, ping-pong by sc->cur
+    float *sx, *sr, *sp;       // conjugate directions
+    float *pc;                 // shaped model p
+    float *gp;                 // shaped gradient
+};
+
+// gx = -eps*x + w*r with the pending model update applied first
+// (dip_cfuns.c:307-316 and 376-378; eps = 1).
+template <int PRO>
+__device__ inline float sm_load(const float *in, long gi, bool own, const CgArgs &A, bool pend, float a2,
+                                int cur) {
+    if (PRO == PRO_PLAIN) return in[gi];
+    const float *xi = cur ? A.xb : A.xa;
+    const float *ri = cur ? A.rb : A.ra;
+    float xv = xi[gi], rv = ri[gi];
+    if (pend) {
+        xv = xv + a2 * A.sx[gi];
+        rv = rv + a2 * A.sr[gi];
+        if (own) {
+            (cur ? A.xa : A.xb)[gi] = xv;
+            (cur ? A.ra : A.rb)[gi] = rv;
+        }
+    }
+    return (-xv) + rv * A.w[gi];
+}
+
+template <int EPI>
+__device__ inline void sm_store(float s, float *out, long gi, const CgArgs &A, bool pend, float a2, float a1,
+                                int first, double *acc) {
+    if (EPI == EPI_PLAIN) {
+        out[gi] = s;
+    } else if (EPI == EPI_GP) {  // gp = eps*p + S(gx), p updated lazily
+        float pv = A.pc[gi];
+        if (pend) {
+            pv = pv + a2 * A.sp[gi];
+            A.pc[gi] = pv;
+        }
+        float g = pv + s;
+        A.gp[gi] = g;
+        acc[0] += (double)g * g;
+    } else {  // gx = S(gp), gr = w gx, conjugate directions (dip_cfuns.c:334-365)
+        float gx = s, gr = gx * A.w[gi], gpv = A.gp[gi];
+        float sp, sx, sr;
+        if (first) {
+            sp = gpv; sx = gx; sr = gr;
+        } else {
+            sp = gpv + a1 * A.sp[gi];
+            sx = gx + a1 * A.sx[gi];
+            sr = gr + a1 * A.sr[gi];
+        }
+        A.sp[gi] = sp; A.sx[gi] = sx; A.sr[gi] = sr;
+        acc[0] += (double)sr * sr;
+        acc[1] += (double)sp * sp;
+        acc[2] += (double)sx * sx;
+    }
+}
+
+__device__ inline void sm_weights(float *wts, int nb) {
+    for (int k = threadIdx.x + blockDim.x * threadIdx.y; k < nb; k += blockDim.x * blockDim.y)
+        wts[k] = (float)((double)(nb - k) / ((double)nb * nb));
+}
+
+template <int EPI>
+__device__ inline void sm_partials(double *acc, double *part, int nblk, int bid, double *sh) {
+    if (EPI == EPI_PLAIN) return;
+    constexpr int K = (EPI == EPI_GP) ? 1 : 3;
+    for (int k = 0; k < K; k++) {
+        double t = block_sum_flat(acc[k], sh);
+        if (flat_tid() == 0) part[(size_t)k * nblk + bid] = t;
+    }
+}
+
+// Axis 1 (contiguous): block = one chunk of one trace.
+template <int PRO, int EPI>
+__global__ void __launch_bounds__(256)
+smooth_contig(const float *__restrict__ in, float *__restrict__ out, int n1, int nb, CgArgs A,
+              const DipScal *sc, double *part, int first) {
+    if (sc->cg_done) return;
+    __shared__ float tile[SM_CH1 + 2 * SM_HMAX];
+    __shared__ float wts[SM_HMAX + 1];
+    __shared__ double sh[16];
+    int h = nb - 1;
+    int nch = (n1 + SM_CH1 - 1) / SM_CH1;
+    long base = (long)(blockIdx.x / nch) * n1;
+    int c0 = (blockIdx.x % nch) * SM_CH1;
+    int len = min(SM_CH1, n1 - c0);
+    bool pend = (PRO == PRO_CG || EPI == EPI_GP) ? (sc->pending != 0) : false;
+    float a2 = pend ? (float)sc->alpha2 : 0.0f;
+    float a1 = (EPI == EPI_S) ? (float)sc->alpha : 0.0f;
+    int cur = (PRO == PRO_CG) ? sc->cur : 0;
+    sm_weights(wts, nb);
+    for (int t = threadIdx.x; t < len + 2 * h; t += blockDim.x) {
+        int pos = c0 - h + t;
+        int q = fold_idx(pos, n1);
+        bool own = pos >= c0 && pos < c0 + len;
+        tile[t] = sm_load<PRO>(in, base + q, own, A, pend, a2, cur);
+    }
+    __syncthreads();
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int t = threadIdx.x; t < len; t += blockDim.x) {
+        const float *c = tile + t + h;
+        float s = wts[0] * c[0];
+        for (int k = 1; k <= h; k++) s += wts[k] * (c[-k] + c[k]);
+        sm_store<EPI>(s, out, base + c0 + t, A, pend, a2, a1, first, acc);
+    }
+    sm_partials<EPI>(acc, part, gridDim.x, blockIdx.x, sh);
+}
+
+// Axes 2/3 (strided): block = 32 consecutive i1 x a chunk along the axis.
+// grid = (ceil(n1/32), ceil(naxis/SM_CHS), nother); index = i1 + o*ostride + q*stride.
+template <int PRO, int EPI>
+__global__ void __launch_bounds__(256)
+smooth_strided(const float *__restrict__ in, float *__restrict__ out, int n1, int naxis, long stride,
+               long ostride, int nb, CgArgs A, const DipScal *sc, double *part, int first) {
+    if (sc->cg_done) return;
+    extern __shared__ float tile[];  // [(SM_CHS + 2h) rows][32]
+    __shared__ float wts[SM_HMAX + 1];
+    __shared__ double sh[16];
+    int h = nb - 1;
+    int tx = threadIdx.x, ty = threadIdx.y;
+    int i1 = blockIdx.x * 32 + tx;
+    int c0 = blockIdx.y * SM_CHS;
+    int len = min(SM_CHS, naxis - c0);
+    long base = (long)blockIdx.z * ostride + i1;
+    bool pend = (PRO == PRO_CG || EPI == EPI_GP) ? (sc->pending != 0) : false;
+    float a2 = pend ? (float)sc->alpha2 : 0.0f;
+    float a1 = (EPI == EPI_S) ? (float)sc->alpha : 0.0f;
+    int cur = (PRO == PRO_CG) ? sc->cur : 0;
+    sm_weights(wts, nb);
+    if (i1 < n1) {
+        for (int row = ty; row < len + 2 * h; row += blockDim.y) {
+            int pos = c0 - h + row;
+            int q = fold_idx(pos, naxis);
+            bool own = pos >= c0 && pos < c0 + len;
+            tile[row * 32 + tx] = sm_load<PRO>(in, base + (long)q * stride, own, A, pend, a2, cur);
+        }
+    }
+    __syncthreads();
+    double acc[3] = {0.0, 0.0, 0.0};
+    if (i1 < n1) {
+        for (int row = ty; row < len; row += blockDim.y) {
+            const float *c = tile + (row + h) * 32 + tx;
+            float s = wts[0] * c[0];
+            for (int k = 1; k <= h; k++) s += wts[k] * (c[-k * 32] + c[k * 32]);
+            sm_store<EPI>(s, out, base + (long)(c0 + row) * stride, A, pend, a2, a1, first, acc);
+        }
+    }
+    sm_partials<EPI>(acc, part, gridDim.x * gridDim.y * gridDim.z,
+                     blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z), sh);
+}
+
+// mask32 (dip_cfuns.c:914-966, both = false)
+__global__ void mask_kernel(const float *__restrict__ md, unsigned char *__restrict__ m0,
+                            unsigned char *__restrict__ m1, int n1, int n2, int n3, int nw) {
+    long n = (long)n1 * n2 * n3;
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int i1 = (int)(i % n1);
+    long r = i / n1;
+    int i2 = (int)(r % n2), i3 = (int)(r / n2);
+    unsigned char a = 0, b = 0;
+    if (i1 >= nw && i1 < n1 - nw) {
+        for (int iw = 0; iw <= 2 * nw; iw++) {
+            int is = iw - nw;
+            if (i2 < n2 - 1 && (md[i - is] == 0.0f || md[i + n1 + is] == 0.0f)) a = 1;
+            if (i3 < n3 - 1 && (md[i - is] == 0.0f || md[i + (long)n1 * n2 + is] == 0.0f)) b = 1;
+        }
+    }
+    m0[i] = a;
+    m1[i] = b;
+}
+
+__global__ void fill_kernel(float *__restrict__ y, float v, long n) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// Host orchestration
+// ---------------------------------------------------------------------------
+struct DipWork {
+    float *p[2], *u1, *u2, *w, *xa, *xb, *ra, *rb, *sp, *sx, *sr, *gp, *pc, *t1, *t2, *dp;
+    unsigned char *m0, *m1;
+    double *part;
+    DipScal *sc;
+    int nblk_max;
+};
+
+struct SmoothPlan {
+    int naxes;
+    int axis[3];
+    int nb[3];
+};
+
+struct DipGeom {
+    int n1, n2, n3;
+    long n;
+};
+
+// One triangle-shaping application in -> out with the given prologue and
+// epilogue on the first / last active axis; returns number of blocks of
+// the final pass (for its partial sums).
+int smooth_apply(sh_ctx *ctx, const DipGeom &g, const SmoothPlan &sp, const float *in, float *out, int pro,
+                 int epi, const CgArgs &A, DipWork &W, int first, int *nblk_out) {
+    const float *src = in;
+    for (int a = 0; a < sp.naxes; a++) {
+        bool is_first = a == 0, is_last = a == sp.naxes - 1;
+        int P = is_first ? pro : PRO_PLAIN;
+        int E = is_last ? epi : EPI_PLAIN;
+        float *dst = is_last ? out : ((a % 2 == 0) ? W.t1 : W.t2);
+        int ax = sp.axis[a], nb = sp.nb[a];
+        int nblk;
+        if (ax == 0) {
+            dim3 grid((unsigned)((long)ceil_div(g.n1, SM_CH1) * g.n2 * g.n3));
+            nblk = grid.x;
+#define SM_C(PP, EE)                                                                          \
+    if (P == PP && E == EE)                                                                   \
+        hipLaunchKernelGGL((smooth_contig<PP, EE>), grid, dim3(256), 0, ctx->stream, src, dst, g.n1, nb, A, \
+                           W.sc, W.part, first);
+            SM_C(0, 0) SM_C(0, 1) SM_C(0, 2) SM_C(1, 0) SM_C(1, 1) SM_C(1, 2)
+#undef SM_C
+        } else {
+            int naxis = ax == 1 ? g.n2 : g.n3;
+            long stride = ax == 1 ? g.n1 : (long)g.n1 * g.n2;
+            int nother = ax == 1 ? g.n3 : g.n2;
+            long ostride = ax == 1 ? (long)g.n1 * g.n2 : g.n1;
+            dim3 grid(ceil_div(g.n1, 32), ceil_div(naxis, SM_CHS), nother);
+            nblk = grid.x * grid.y * grid.z;
+            size_t lds = (size_t)(SM_CHS + 2 * (nb - 1)) * 32 * sizeof(float);
+#define SM_S(PP, EE)                                                                          \
+    if (P == PP && E == EE)                                                                   \
+        hipLaunchKernelGGL((smooth_strided<PP, EE>), grid, dim3(32, 8), lds, ctx->stream, src, dst, g.n1, \
+                           naxis, stride, ostride, nb, A, W.sc, W.part, first);
+            SM_S(0, 0) SM_S(0, 1) SM_S(0, 2) SM_S(1, 0) SM_S(1, 1) SM_S(1, 2)
+#undef SM_S
+        }
+        SH_TRY(sh_launch_check(ctx, "smooth"));
+        if (nblk > W.nblk_max) return sh_fail(ctx, SH_EINVAL, "internal: partial buffer too small");
+        if (is_last) *nblk_out = nblk;
+        src = dst;
+    }
+    return SH_OK;
+}
+
+int finalize(sh_ctx *ctx, DipWork &W, int nblk, int K, int mode, int it, long n, int k_ls = 0) {
+    hipLaunchKernelGGL(dip_finalize, dim3(1), dim3(1024), 0, ctx->stream, W.part, nblk, K, W.sc, mode, it, n,
+                       1.e-6f, k_ls);
+    return sh_launch_check(ctx, "dip_finalize");
+}
+
+template <int NW>
+int pwd_launch(sh_ctx *ctx, int mode, const DipGeom &g, int axis, const float *x, const float *p,
+               const float *dp, float lam, float *pout, DipWork &W, const unsigned char *mask) {
+    int nblk = ceil_div(g.n, 256);
+    const float pmax = 3.402823466e+38f, pmin = -3.402823466e+38f;
+    if (mode == 0)
+        hipLaunchKernelGGL((pwd_kernel<NW, 0>), dim3(nblk), dim3(256), 0, ctx->stream, x, p, dp, lam, pmin, pmax,
+                           pout, W.u1, W.u2, mask, g.n1, g.n2, g.n3, axis, W.part, W.sc);
+    else if (mode == 1)
+        hipLaunchKernelGGL((pwd_kernel<NW, 1>), dim3(nblk), dim3(256), 0, ctx->stream, x, p, dp, lam, pmin, pmax,
+                           pout, W.u1, W.u2, mask, g.n1, g.n2, g.n3, axis, W.part, W.sc);
+    else
+        hipLaunchKernelGGL((pwd_kernel<NW, 2>), dim3(nblk), dim3(256), 0, ctx->stream, x, p, dp, lam, pmin, pmax,
+                           pout, W.u1, W.u2, mask, g.n1, g.n2, g.n3, axis, W.part, W.sc);
+    return sh_launch_check(ctx, "pwd_kernel");
+}
+
+// One dip direction: dip3 (dip_cfuns.c:1619-1691).  p[0] holds the initial
+// dip on entry; the result is left in W.p[*final_idx].
+template <int NW>
+int dip_direction(sh_ctx *ctx, const DipGeom &g, const SmoothPlan &plan, int axis, const float *x,
+                  const unsigned char *mask, int niter, int liter, DipWork &W, int *final_idx) {
+    long n = g.n;
+    int nb_pw = ceil_div(n, 256);
+    int cur = 0;  // index of the current dip buffer
+    CgArgs A{W.w, W.xa, W.xb, W.ra, W.rb, W.sx, W.sr, W.sp, W.pc, W.gp};
+    SH_TRY(pwd_launch<NW>(ctx, 0, g, axis, x, W.p[cur], nullptr, 0.0f, nullptr, W, mask));
+    SH_TRY(finalize(ctx, W, nb_pw, 1, F_USUM, 0, n));
+    for (int it = 0; it < niter; it++) {
+        // u1 = C'(p) x, divne prologue
+        SH_TRY(pwd_launch<NW>(ctx, 1, g, axis, x, W.p[cur], nullptr, 0.0f, nullptr, W, mask));
+        SH_TRY(finalize(ctx, W, nb_pw, 1, F_DIVN, it, n));
+        hipLaunchKernelGGL(cg_start_kernel, dim3(nb_pw), dim3(256), 0, ctx->stream, W.u2, W.u1, W.w, W.ra, W.xa,
+                           W.pc, n, W.part, W.sc);
+        SH_TRY(sh_launch_check(ctx, "cg_start"));
+        SH_TRY(finalize(ctx, W, nb_pw, 1, F_R0, it, n));
+        for (int k = 0; k < liter; k++) {
+            int nblk = 0;
+            SH_TRY(smooth_apply(ctx, g, plan, nullptr, nullptr, PRO_CG, EPI_GP, A, W, k == 0, &nblk));
+            SH_TRY(finalize(ctx, W, nblk, 1, F_GN, k, n));
+            SH_TRY(smooth_apply(ctx, g, plan, W.gp, nullptr, PRO_PLAIN, EPI_S, A, W, k == 0, &nblk));
+            SH_TRY(finalize(ctx, W, nblk, 3, F_BETA, k, n));
+        }
+        hipLaunchKernelGGL(cg_result_kernel, dim3(nb_pw), dim3(256), 0, ctx->stream, W.xa, W.xb, W.sx, W.dp, n,
+                           W.sc);
+        SH_TRY(sh_launch_check(ctx, "cg_result"));
+        float lam = 1.0f;
+        for (int k = 0; k < 8; k++) {
+            SH_TRY(pwd_launch<NW>(ctx, 2, g, axis, x, W.p[cur], W.dp, lam, W.p[1 - cur], W, mask));
+            SH_TRY(finalize(ctx, W, nb_pw, 1, F_LS, it, n, k));
+            lam *= 0.5f;
+        }
+        cur = 1 - cur;
+    }
+    *final_idx = cur;
+    return SH_OK;
+}
+
+}  // namespace
+
+extern "C" int sh_dip(sh_ctx *ctx, const float *data, const float *mask, const float *init, int n1, int n2,
+                      int n3, int niter, int liter, int order, float eps_dv, int r1, int r2, int r3,
+                      float *dip, sh_dip_report *rep) {
+    (void)eps_dv;
+    if (!ctx) return SH_ENULL;
+    if (!data || !dip) return sh_fail(ctx, SH_ENULL, "sh_dip: data and dip must be non-NULL");
+    if (n1 < 1 || n2 < 1 || n3 < 1) return sh_fail(ctx, SH_EINVAL, "sh_dip: bad shape %d x %d x %d", n1, n2, n3);
+    if (niter < 0 || niter > 64 || liter < 0) return sh_fail(ctx, SH_EINVAL, "sh_dip: niter must be 0..64");
+    int rect[3] = {r1, r2, r3};
+    for (int a = 0; a < 3; a++)
+        if (rect[a] > SM_HMAX + 1) return sh_fail(ctx, SH_EUNSUP, "sh_dip: rect %d exceeds %d", rect[a], SM_HMAX + 1);
+    DipGeom g{n1, n2, n3, (long)n1 * n2 * n3};
+    SmoothPlan plan{};
+    for (int a = 0; a < 3; a++)
+        if (rect[a] > 1) {
+            plan.axis[plan.naxes] = a;
+            plan.nb[plan.naxes] = rect[a];
+            plan.naxes++;
+        }
+    if (plan.naxes == 0) {  // no smoothing: identity through a 1-tap pass
+        plan.naxes = 1;
+        plan.axis[0] = 0;
+        plan.nb[0] = 1;
+    }
+    long n = g.n;
+    size_t fb = (size_t)n * sizeof(float);
+    // workspace
+    DipWork W{};
+    float *fbuf = nullptr;
+    const int NF = 17;
+    SH_TRY(sh_slot(ctx, SLOT_DIP0, fb * NF, (void **)&fbuf));
+    float **slots[NF] = {&W.p[0], &W.p[1], &W.u1, &W.u2, &W.w, &W.xa, &W.xb, &W.ra, &W.rb,
+                         &W.sp, &W.sx, &W.sr, &W.gp, &W.pc, &W.t1, &W.t2, &W.dp};
+    for (int s = 0; s < NF; s++) *slots[s] = fbuf + (size_t)s * n;
+    long nblk_max = std::max<long>(ceil_div(n, 256), (long)ceil_div(n1, SM_CH1) * n2 * n3);
+    nblk_max = std::max<long>(nblk_max, (long)ceil_div(n1, 32) * ceil_div(n2, SM_CHS) * n3);
+    nblk_max = std::max<long>(nblk_max, (long)ceil_div(n1, 32) * ceil_div(n3, SM_CHS) * n2);
+    W.nblk_max = (int)nblk_max;
+    SH_TRY(sh_slot(ctx, SLOT_RED, (size_t)nblk_max * 3 * sizeof(double), (void **)&W.part));
+    SH_TRY(sh_slot(ctx, SLOT_SCAL, sizeof(DipScal), (void **)&W.sc));
+    SH_HIP(ctx, hipMemsetAsync(W.sc, 0, sizeof(DipScal), ctx->stream));
+    if (mask) {
+        SH_TRY(sh_slot(ctx, SLOT_DIP0 + 1, (size_t)n * 2, (void **)&W.m0));
+        W.m1 = W.m0 + n;
+        hipLaunchKernelGGL(mask_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, ctx->stream, mask, W.m0, W.m1, n1, n2,
+                           n3, order);
+        SH_TRY(sh_launch_check(ctx, "mask_kernel"));
+    }
+    int ndir = (n3 == 1) ? 1 : 2;
+    if (rep) {
+        rep->ndir = ndir;
+        rep->niter = niter;
+    }
+    for (int d = 0; d < ndir; d++) {
+        if (init)
+            SH_HIP(ctx, hipMemcpyAsync(W.p[0], init + (size_t)d * n, fb, hipMemcpyDeviceToDevice, ctx->stream));
+        else {
+            hipLaunchKernelGGL(fill_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, ctx->stream, W.p[0], 0.0f, n);
+            SH_TRY(sh_launch_check(ctx, "fill"));
+        }
+        int fin = 0;
+        const unsigned char *m = mask ? (d == 0 ? W.m0 : W.m1) : nullptr;
+        SH_DISPATCH_NW(ctx, order, NW, SH_TRY(dip_direction<NW>(ctx, g, plan, d == 0 ? 2 : 3, data, m, niter, liter, W, &fin)));
+        SH_HIP(ctx, hipMemcpyAsync(dip + (size_t)d * n, W.p[fin], fb, hipMemcpyDeviceToDevice, ctx->stream));
+        if (rep) {
+            DipScal hs;
+            SH_HIP(ctx, hipMemcpyAsync(&hs, W.sc, sizeof hs, hipMemcpyDeviceToHost, ctx->stream));
+            SH_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            for (int it = 0; it < niter; it++) {
+                rep->cg_iters[d][it] = hs.rec[2 * it];
+                rep->ls_step[d][it] = hs.rec[2 * it + 1];
+            }
+        }
+    }
+    return SH_OK;
+}

@@ -1,0 +1,638 @@
+// Structure-oriented interpolation on gfx950.
+//
+// soint (PWD-regularised least squares, Claerbout's cgstep solver):
+//   csoint3d / allpass3_lop / ps_cgstep / ps_solver
+//   (pyseistr/src/soint3d_cfuns.c:625-731, 826-1174, 2405-2508) and the 2-D
+//   csoint2d / allpass21_lop (pyseistr/src/soint2d_cfuns.c:296-344, 2260-2419).
+// sint (masked shaping CG with structure-oriented smoothing):
+//   csint3d / pwsmooth3_lop / ps_conjgrad (soint3d_cfuns.c:2231-2300,
+//   1222-1349, 2510-2615) and csint2d (soint2d_cfuns.c:2421-2538).
+//
+// Design: the PWD operator and its adjoint are gather kernels over
+// precomputed filter taps (the adjoint enumerates the contributions to each
+// output sample in the reference's scatter order, so both are bit-exact);
+// solver scalars are formed on the host from deterministic device
+// reductions, one small read-back per iteration.
+#include "common.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+void sh_mt_randn(unsigned seed, size_t n, float *out);
+
+namespace {
+
+constexpr int TB = 256;
+
+struct Geo {
+    int n1, n2, n3, nw, nj1, nj2, nparts;
+    long n;
+};
+
+// taps[part][iw][i] = a_iw(p_part[i]) (apfilt, soint3d_cfuns.c:159-250)
+template <int NW>
+__global__ void taps_kernel(const float *__restrict__ pp, float *__restrict__ taps, long n) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float a[2 * NW + 1];
+    apf_pass<NW>(pp[i], a);
+#pragma unroll
+    for (int iw = 0; iw <= 2 * NW; iw++) taps[(long)iw * n + i] = a[iw];
+}
+
+// y[part*n + i] = sum_iw (x[i+ip+is] - x[i-is]) a_iw(i)  (allpass3_lop fwd)
+// plus partial dots of y with ss and rr (for cgstep) and y.y.
+template <int NW>
+__global__ void __launch_bounds__(TB)
+pwd_fwd(const float *__restrict__ x, const float *__restrict__ taps, float *__restrict__ y,
+        const float *__restrict__ ss, const float *__restrict__ rr, Geo g, double *part, int nblk,
+        const float *base) {
+    __shared__ double sh[16];
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double d_gg = 0, d_gs = 0, d_gr = 0;
+    long ny = g.n * g.nparts;
+    if (i < ny) {
+        int part_id = (int)(i / g.n);
+        long k = i - part_id * g.n;
+        int ix = (int)(k % g.n1);
+        long r = k / g.n1;
+        int iy = (int)(r % g.n2), iz = (int)(r / g.n2);
+        int nj = part_id ? g.nj2 : g.nj1;
+        long ip = part_id ? (long)g.n1 * g.n2 : g.n1;
+        bool valid = ix >= NW * nj && ix < g.n1 - NW * nj && (part_id ? iz < g.n3 - 1 : iy < g.n2 - 1);
+        float v = base ? base[i] : 0.0f;
+        if (valid) {
+            const float *tp = taps + (long)part_id * (2 * NW + 1) * g.n;
+#pragma unroll
+            for (int iw = 0; iw <= 2 * NW; iw++) {
+                int is = (iw - NW) * nj;
+                v += (x[k + ip + is] - x[k - is]) * tp[(long)iw * g.n + k];
+            }
+        }
+        y[i] = v;
+        d_gg = (double)v * v;
+        d_gs = (double)v * ss[i];
+        d_gr = (double)v * rr[i];
+    }
+    double t0 = block_sum(d_gg, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = t0;
+    double t1 = block_sum(d_gs, sh);
+    if (threadIdx.x == 0) part[nblk + blockIdx.x] = t1;
+    double t2 = block_sum(d_gr, sh);
+    if (threadIdx.x == 0) part[2 * nblk + blockIdx.x] = t2;
+}
+
+// g[k] = (A^T rr)[k], zero where known (allpass3_lop adj + ps_solver
+// 1056-1060).  Contributions are summed in the order the reference's
+// scatter loop delivers them (increasing source index i, inline part
+// first).  Partial sums: g.g.
+template <int NW>
+__global__ void __launch_bounds__(TB)
+pwd_adj(const float *__restrict__ rr, const float *__restrict__ taps, const unsigned char *__restrict__ known,
+        float *__restrict__ gout, Geo g, double *part) {
+    __shared__ double sh[16];
+    long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double d = 0;
+    if (k < g.n) {
+        int kx = (int)(k % g.n1);
+        long r = k / g.n1;
+        int ky = (int)(r % g.n2), kz = (int)(r / g.n2);
+        float v = 0.0f;
+        for (int part_id = 0; part_id < g.nparts; part_id++) {
+            int nj = part_id ? g.nj2 : g.nj1;
+            int h = NW * nj;
+            long ip = part_id ? (long)g.n1 * g.n2 : g.n1;
+            const float *tp = taps + (long)part_id * (2 * NW + 1) * g.n;
+            const float *yy = rr + (long)part_id * g.n;
+            // "+" terms: i = k - ip - is, source row valid if it has a successor
+            bool prow = part_id ? (kz >= 1) : (ky >= 1);
+            if (prow) {
+#pragma unroll
+                for (int iw = 2 * NW; iw >= 0; iw--) {
+                    int is = (iw - NW) * nj;
+                    int ix = kx - is;
+                    if (ix >= h && ix < g.n1 - h) {
+                        long i = k - ip - is;
+                        v += yy[i] * tp[(long)iw * g.n + i];
+                    }
+                }
+            }
+            bool mrow = part_id ? (kz < g.n3 - 1) : (ky < g.n2 - 1);
+            if (mrow) {
+#pragma unroll
+                for (int iw = 0; iw <= 2 * NW; iw++) {
+                    int is = (iw - NW) * nj;
+                    int ix = kx + is;
+                    if (ix >= h && ix < g.n1 - h) {
+                        long i = k + is;
+                        v -= yy[i] * tp[(long)iw * g.n + i];
+                    }
+                }
+            }
+        }
+        if (known[k]) v = 0.0f;
+        gout[k] = v;
+        d = (double)v * v;
+    }
+    double t = block_sum(d, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+// partial sums over a vector: sum a*b (stacked into part[k*nblk + b])
+__global__ void __launch_bounds__(TB) dot_kernel(const float *__restrict__ a, const float *__restrict__ b, long n,
+                                                 double *part, int slot, int nblk) {
+    __shared__ double sh[16];
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double d = (i < n) ? (double)a[i] * b[i] : 0.0;
+    double t = block_sum(d, sh);
+    if (threadIdx.x == 0) part[(long)slot * nblk + blockIdx.x] = t;
+}
+
+// cgstep update (soint3d_cfuns.c:864-877): S = b S + a g, x += S (nx);
+// Ss = b Ss + a gg, rr += Ss (ny)
+__global__ void cgstep_update(float *__restrict__ S, const float *__restrict__ gv, float *__restrict__ x, long nx,
+                              float *__restrict__ Ss, const float *__restrict__ gg, float *__restrict__ rr, long ny,
+                              float a, float b) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nx) {
+        float s = S[i] * b;
+        s += a * gv[i];
+        S[i] = s;
+        x[i] += s;
+    }
+    if (i < ny) {
+        float s = Ss[i] * b;
+        s += a * gg[i];
+        Ss[i] = s;
+        rr[i] += s;
+    }
+}
+
+__global__ void known_kernel(const float *__restrict__ src, unsigned char *__restrict__ known, long n) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) known[i] = src[i] != 0.0f;
+}
+
+__global__ void neg_kernel(const float *__restrict__ a, float *__restrict__ b, long n) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = -a[i];
+}
+
+__global__ void zero_f(float *__restrict__ a, long n) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = 0.0f;
+}
+
+int sum_partials(sh_ctx *ctx, const double *part, int nblk, int K, double *dsum, double *host) {
+    SH_TRY(reduce_partials(ctx, part, nblk, K, dsum));
+    SH_HIP(ctx, hipMemcpyAsync(host, dsum, sizeof(double) * K, hipMemcpyDeviceToHost, ctx->stream));
+    SH_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return SH_OK;
+}
+
+// ps_solver + ps_cgstep with the "known" and "x0" options
+template <int NW>
+int soint_solve(sh_ctx *ctx, const Geo &g, const float *pp, const float *qq, const unsigned char *known, float *x,
+                const float *dd, int niter, int *iters_out) {
+    long n = g.n, ny = g.n * g.nparts;
+    int NA = 2 * NW + 1;
+    float *buf = nullptr;
+    // taps (nparts*NA*n), g, S (n each), rr, gg, Ss (ny each)
+    size_t nf = (size_t)g.nparts * NA * n + 2 * n + 3 * ny;
+    SH_TRY(sh_slot(ctx, SLOT_INTERP0, nf * sizeof(float), (void **)&buf));
+    float *taps = buf, *gv = taps + (size_t)g.nparts * NA * n, *S = gv + n, *rr = S + n, *gg = rr + ny, *Ss = gg + ny;
+    int nbn = ceil_div(n, TB), nby = ceil_div(ny, TB);
+    double *part = nullptr, *dsum = nullptr;
+    SH_TRY(sh_slot(ctx, SLOT_RED, (size_t)nby * 8 * sizeof(double), (void **)&part));
+    SH_TRY(sh_slot(ctx, SLOT_SCAL, 16 * sizeof(double), (void **)&dsum));
+    hipLaunchKernelGGL((taps_kernel<NW>), dim3(nbn), dim3(TB), 0, ctx->stream, pp, taps, n);
+    if (g.nparts == 2)
+        hipLaunchKernelGGL((taps_kernel<NW>), dim3(nbn), dim3(TB), 0, ctx->stream, qq, taps + (size_t)NA * n, n);
+    SH_TRY(sh_launch_check(ctx, "taps"));
+    hipLaunchKernelGGL(zero_f, dim3(nbn), dim3(TB), 0, ctx->stream, S, n);
+    hipLaunchKernelGGL(zero_f, dim3(nby), dim3(TB), 0, ctx->stream, Ss, ny);
+    // rr = -dd, then rr += A x0 tap by tap (ps_solver 1018-1026)
+    hipLaunchKernelGGL(neg_kernel, dim3(nby), dim3(TB), 0, ctx->stream, dd, rr, ny);
+    hipLaunchKernelGGL((pwd_fwd<NW>), dim3(nby), dim3(TB), 0, ctx->stream, x, taps, rr, Ss, Ss, g, part, nby, rr);
+    SH_TRY(sh_launch_check(ctx, "soint init"));
+    double h[8];
+    hipLaunchKernelGGL(dot_kernel, dim3(nby), dim3(TB), 0, ctx->stream, rr, rr, ny, part, 0, nby);
+    SH_TRY(sum_partials(ctx, part, nby, 1, dsum, h));
+    double dpr0 = h[0], dpg0 = 1.0;
+    bool started = false;
+    int it;
+    const double TOL = (double)1.e-12f, EPSILON = (double)1.e-12f;
+    for (it = 0; it < niter; it++) {
+        hipLaunchKernelGGL((pwd_adj<NW>), dim3(nbn), dim3(TB), 0, ctx->stream, rr, taps, known, gv, g, part);
+        SH_TRY(sh_launch_check(ctx, "pwd_adj"));
+        SH_TRY(sum_partials(ctx, part, nbn, 1, dsum, h));
+        double gnorm = h[0];
+        hipLaunchKernelGGL((pwd_fwd<NW>), dim3(nby), dim3(TB), 0, ctx->stream, gv, taps, gg, Ss, rr, g, part, nby,
+                           nullptr);
+        hipLaunchKernelGGL(dot_kernel, dim3(nby), dim3(TB), 0, ctx->stream, Ss, Ss, ny, part, 3, nby);
+        hipLaunchKernelGGL(dot_kernel, dim3(nby), dim3(TB), 0, ctx->stream, Ss, rr, ny, part, 4, nby);
+        hipLaunchKernelGGL(dot_kernel, dim3(nby), dim3(TB), 0, ctx->stream, rr, rr, ny, part, 5, nby);
+        SH_TRY(sh_launch_check(ctx, "pwd_fwd"));
+        SH_TRY(sum_partials(ctx, part, nby, 6, dsum, h));
+        double gdg = h[0], gds = h[1], gdr0 = h[2], sds = h[3], sdr0 = h[4], rnorm = h[5];
+        double dpr, dpg;
+        if (it == 0) {
+            dpg0 = gnorm;
+            dpr = 1.0;
+            dpg = 1.0;
+        } else {
+            dpr = rnorm / dpr0;
+            dpg = gnorm / dpg0;
+        }
+        if ((float)dpr < (float)TOL || (float)dpg < (float)TOL) break;
+        double alfa, beta;
+        bool forget = !started;
+        started = true;
+        if (forget) {
+            hipLaunchKernelGGL(zero_f, dim3(nbn), dim3(TB), 0, ctx->stream, S, n);
+            hipLaunchKernelGGL(zero_f, dim3(nby), dim3(TB), 0, ctx->stream, Ss, ny);
+            beta = 0.0;
+            alfa = gdg;
+            if (alfa <= 0.) continue;
+            alfa = -gdr0 / alfa;
+        } else {
+            if (gdg == 0. || sds == 0.) continue;
+            double determ = 1.0 - (gds / gdg) * (gds / sds);
+            if (determ > EPSILON) determ *= gdg * sds;
+            else determ = gdg * sds * EPSILON;
+            double gdr = -gdr0, sdr = -sdr0;
+            alfa = (sds * gdr - gds * sdr) / determ;
+            beta = (-gds * gdr + gdg * sdr) / determ;
+        }
+        hipLaunchKernelGGL(cgstep_update, dim3(nby), dim3(TB), 0, ctx->stream, S, gv, x, n, Ss, gg, rr, ny,
+                           (float)alfa, (float)beta);
+        SH_TRY(sh_launch_check(ctx, "cgstep_update"));
+    }
+    if (iters_out) *iters_out = it;
+    return SH_OK;
+}
+
+// ---------------------------------------------------------------------------
+// sint: masked shaping CG (ps_conjgrad with hasp0) over structure-oriented
+// smoothing.  Vectors live in the inline-slice layout [i3][k][i2].
+// ---------------------------------------------------------------------------
+// dst[i2][k][i3] = src[i3][k][i2]  (inline slices -> crossline slices)
+__global__ void permute_xi(const float *__restrict__ src, float *__restrict__ dst, int n1, int n2, int n3, int inv) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long n = (long)n1 * n2 * n3;
+    if (i >= n) return;
+    // i indexes the destination; decompose accordingly
+    if (!inv) {  // dst [i2][k][i3]
+        int i3 = (int)(i % n3);
+        long r = i / n3;
+        int k = (int)(r % n1), i2 = (int)(r / n1);
+        dst[i] = src[((long)i3 * n1 + k) * n2 + i2];
+    } else {  // dst [i3][k][i2] from src [i2][k][i3]
+        int i2 = (int)(i % n2);
+        long r = i / n2;
+        int k = (int)(r % n1), i3 = (int)(r / n1);
+        dst[i] = src[((long)i2 * n1 + k) * n3 + i3];
+    }
+}
+
+struct Shape3 {
+    Pwsmooth2 pin, px;  // inline slices (n1 x n2, n3 of them), crossline (n1 x n3, n2 of them)
+    bool three;
+    float *t1, *t2;     // layout conversion temporaries
+    int n1, n2, n3;
+    long n;
+};
+
+// x = S p  (add = 0)  or  p += S^T x  (adjoint, add = 1); pwsmooth3_lop
+int shape_apply(sh_ctx *ctx, Shape3 &Sh, int adj, float *p, float *x) {
+    int g = ceil_div(Sh.n, TB);
+    if (!Sh.three) return pws2_apply(ctx, Sh.pin, adj, adj ? 1 : 0, p, x);
+    if (!adj) {
+        SH_TRY(pws2_apply(ctx, Sh.pin, 0, 0, p, Sh.t1));  // itmp
+        hipLaunchKernelGGL(permute_xi, dim3(g), dim3(TB), 0, ctx->stream, Sh.t1, Sh.t2, Sh.n1, Sh.n2, Sh.n3, 0);
+        SH_TRY(pws2_apply(ctx, Sh.px, 0, 0, Sh.t2, Sh.t1));  // xtmp
+        hipLaunchKernelGGL(permute_xi, dim3(g), dim3(TB), 0, ctx->stream, Sh.t1, x, Sh.n1, Sh.n2, Sh.n3, 1);
+        return sh_launch_check(ctx, "permute");
+    }
+    hipLaunchKernelGGL(permute_xi, dim3(g), dim3(TB), 0, ctx->stream, x, Sh.t1, Sh.n1, Sh.n2, Sh.n3, 0);  // xtmp
+    SH_TRY(pws2_apply(ctx, Sh.px, 1, 0, Sh.t2, Sh.t1));  // itmp2 = Sx^T xtmp
+    hipLaunchKernelGGL(permute_xi, dim3(g), dim3(TB), 0, ctx->stream, Sh.t2, Sh.t1, Sh.n1, Sh.n2, Sh.n3, 1);
+    SH_TRY(sh_launch_check(ctx, "permute"));
+    return pws2_apply(ctx, Sh.pin, 1, 1, p, Sh.t1);  // trace += Si^T itmp
+}
+
+// gp = eps p; gx = -eps x + mask r   (ps_conjgrad 306-316 with ps_mask_lop)
+__global__ void sint_pre(const float *__restrict__ p, const float *__restrict__ x, const float *__restrict__ r,
+                         const unsigned char *__restrict__ m, float *__restrict__ gp, float *__restrict__ gx, float eps,
+                         long n) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    gp[i] = eps * p[i];
+    float v = -eps * x[i];
+    if (m[i]) v += r[i];
+    gx[i] = v;
+}
+
+// gr = mask gx; partial gp.gp
+__global__ void __launch_bounds__(TB) sint_post(const float *__restrict__ gp, const float *__restrict__ gx,
+                                                const unsigned char *__restrict__ m, float *__restrict__ gr, long n,
+                                                double *part) {
+    __shared__ double sh[16];
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double d = 0;
+    if (i < n) {
+        gr[i] = m[i] ? (0.0f + gx[i]) : 0.0f;
+        d = (double)gp[i] * gp[i];
+    }
+    double t = block_sum(d, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+// s = g (first) or s = g + a s; partial sr.sr, sp.sp, sx.sx
+__global__ void __launch_bounds__(TB) sint_dir(const float *__restrict__ gp, const float *__restrict__ gx,
+                                               const float *__restrict__ gr, float *__restrict__ sp, float *__restrict__ sx,
+                                               float *__restrict__ sr, float a, int first, long n, double *part, int nblk) {
+    __shared__ double sh[16];
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double d0 = 0, d1 = 0, d2 = 0;
+    if (i < n) {
+        float vp, vx, vr;
+        if (first) {
+            vp = gp[i]; vx = gx[i]; vr = gr[i];
+        } else {
+            vp = gp[i] + a * sp[i];
+            vx = gx[i] + a * sx[i];
+            vr = gr[i] + a * sr[i];
+        }
+        sp[i] = vp; sx[i] = vx; sr[i] = vr;
+        d0 = (double)vr * vr; d1 = (double)vp * vp; d2 = (double)vx * vx;
+    }
+    double t = block_sum(d0, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+    t = block_sum(d1, sh);
+    if (threadIdx.x == 0) part[nblk + blockIdx.x] = t;
+    t = block_sum(d2, sh);
+    if (threadIdx.x == 0) part[2 * nblk + blockIdx.x] = t;
+}
+
+__global__ void sint_update(float *__restrict__ p, float *__restrict__ x, float *__restrict__ r,
+                            const float *__restrict__ sp, const float *__restrict__ sx, const float *__restrict__ sr,
+                            float a, long n) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    p[i] += a * sp[i];
+    x[i] += a * sx[i];
+    r[i] += a * sr[i];
+}
+
+// r = -dat + mask x
+__global__ void sint_r0(const float *__restrict__ dat, const float *__restrict__ x, const unsigned char *__restrict__ m,
+                        float *__restrict__ r, long n) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float v = -dat[i];
+    if (m[i]) v += x[i];
+    r[i] = v;
+}
+
+// The shaping CG itself (p = model, x = estimated data, dat = data)
+int sint_cg(sh_ctx *ctx, Shape3 &Sh, const unsigned char *known, float *p, float *x, const float *dat, float eps1,
+            float tol, int niter, int *iters) {
+    long n = Sh.n;
+    int nb = ceil_div(n, TB);
+    float *buf = nullptr;
+    SH_TRY(sh_slot(ctx, SLOT_INTERP0 + 2, (size_t)n * 7 * sizeof(float), (void **)&buf));
+    float *r = buf, *sp = r + n, *gp = sp + n, *sx = gp + n, *gx = sx + n, *sr = gx + n, *gr = sr + n;
+    double *part = nullptr, *dsum = nullptr;
+    SH_TRY(sh_slot(ctx, SLOT_RED, (size_t)nb * 4 * sizeof(double), (void **)&part));
+    SH_TRY(sh_slot(ctx, SLOT_SCAL, 16 * sizeof(double), (void **)&dsum));
+    float eps = eps1 * eps1;
+    SH_TRY(shape_apply(ctx, Sh, 0, p, x));  // x = S p
+    hipLaunchKernelGGL(sint_r0, dim3(nb), dim3(TB), 0, ctx->stream, dat, x, known, r, n);
+    hipLaunchKernelGGL(dot_kernel, dim3(nb), dim3(TB), 0, ctx->stream, r, r, n, part, 0, nb);
+    SH_TRY(sh_launch_check(ctx, "sint init"));
+    double h[4];
+    SH_TRY(sum_partials(ctx, part, nb, 1, dsum, h));
+    int done_iters = 0;
+    if (h[0] == 0.) {
+        if (iters) *iters = 0;
+        return SH_OK;
+    }
+    double gn, gnp = 0, g0 = 0, alpha = 0;
+    for (int it = 0; it < niter; it++) {
+        hipLaunchKernelGGL(sint_pre, dim3(nb), dim3(TB), 0, ctx->stream, p, x, r, known, gp, gx, eps, n);
+        SH_TRY(sh_launch_check(ctx, "sint_pre"));
+        SH_TRY(shape_apply(ctx, Sh, 1, gp, gx));  // gp += S^T gx
+        SH_TRY(shape_apply(ctx, Sh, 0, gp, gx));  // gx = S gp
+        hipLaunchKernelGGL(sint_post, dim3(nb), dim3(TB), 0, ctx->stream, gp, gx, known, gr, n, part);
+        SH_TRY(sum_partials(ctx, part, nb, 1, dsum, h));
+        gn = h[0];
+        if (it == 0) {
+            g0 = gn;
+        } else {
+            alpha = gn / gnp;
+            double dg = gn / g0;
+            if (alpha < (double)tol || dg < (double)tol) break;
+        }
+        hipLaunchKernelGGL(sint_dir, dim3(nb), dim3(TB), 0, ctx->stream, gp, gx, gr, sp, sx, sr, (float)alpha,
+                           it == 0, n, part, nb);
+        SH_TRY(sum_partials(ctx, part, nb, 3, dsum, h));
+        double beta = h[0] + (double)eps * (h[1] - h[2]);
+        double a2 = -gn / beta;
+        hipLaunchKernelGGL(sint_update, dim3(nb), dim3(TB), 0, ctx->stream, p, x, r, sp, sx, sr, (float)a2, n);
+        SH_TRY(sh_launch_check(ctx, "sint_update"));
+        gnp = gn;
+        done_iters = it + 1;
+    }
+    if (iters) *iters = done_iters;
+    return SH_OK;
+}
+
+__global__ void count_kernel(const unsigned char *__restrict__ m, long n, double *part) {
+    __shared__ double sh[16];
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double d = (i < n && m[i]) ? 1.0 : 0.0;
+    double t = block_sum(d, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sh_soint3d(sh_ctx *ctx, const float *din, const float *mask, const float *dipi, const float *dipx, int n1,
+               int n2, int n3, int nw, int nj1, int nj2, int niter, int drift, int seed, int hasmask, float var,
+               float *out, int *iters) {
+    if (!ctx) return SH_ENULL;
+    if (!din || !dipi || !dipx || !out || (hasmask == 1 && !mask))
+        return sh_fail(ctx, SH_ENULL, "soint3d: NULL pointer");
+    if (n1 < 8 || n2 < 2 || n3 < 2 || nj1 < 1 || nj2 < 1) return sh_fail(ctx, SH_EINVAL, "soint3d: bad arguments");
+    if (drift) return sh_fail(ctx, SH_EUNSUP, "soint3d: drift=1 is not implemented on the GPU");
+    long n = (long)n1 * n2 * n3;
+    Geo g{n1, n2, n3, nw, nj1, nj2, 2, n};
+    unsigned char *known = nullptr;
+    float *dd = nullptr;
+    SH_TRY(sh_slot(ctx, SLOT_INTERP0 + 1, (size_t)n + (size_t)2 * n * sizeof(float) + 256, (void **)&dd));
+    known = (unsigned char *)(dd + 2 * n);
+    int nb = ceil_div(n, TB);
+    hipLaunchKernelGGL(known_kernel, dim3(nb), dim3(TB), 0, ctx->stream, hasmask == 1 ? mask : din, known, n);
+    // dd = sqrt(var) * N(0,1) (MT19937 + Box-Muller, host-generated; zeros for var = 0)
+    float a = sqrtf(var);
+    if (a != 0.0f) {
+        std::vector<float> h(2 * n);
+        sh_mt_randn((unsigned)seed, (size_t)2 * n, h.data());
+        for (auto &v : h) v = a * v;
+        SH_HIP(ctx, hipMemcpyAsync(dd, h.data(), sizeof(float) * 2 * n, hipMemcpyHostToDevice, ctx->stream));
+        SH_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    } else {
+        hipLaunchKernelGGL(zero_f, dim3(ceil_div(2 * n, TB)), dim3(TB), 0, ctx->stream, dd, 2 * n);
+    }
+    SH_HIP(ctx, hipMemcpyAsync(out, din, sizeof(float) * n, hipMemcpyDeviceToDevice, ctx->stream));
+    int it = 0;
+    SH_DISPATCH_NW(ctx, nw, NW, SH_TRY(soint_solve<NW>(ctx, g, dipi, dipx, known, out, dd, niter, &it)));
+    if (iters) *iters = it;
+    return SH_OK;
+}
+
+int sh_soint2d(sh_ctx *ctx, const float *din, const float *mask, const float *dip, int n1, int n2, int nw, int nj1,
+               int niter, int drift, int hasmask, int twoplane, float *out, int *iters) {
+    if (!ctx) return SH_ENULL;
+    if (!din || !dip || !out || (hasmask == 1 && !mask)) return sh_fail(ctx, SH_ENULL, "soint2d: NULL pointer");
+    if (n1 < 8 || n2 < 2 || nj1 < 1) return sh_fail(ctx, SH_EINVAL, "soint2d: bad arguments");
+    long n = (long)n1 * n2;
+    SH_HIP(ctx, hipMemcpyAsync(out, din, sizeof(float) * n, hipMemcpyDeviceToDevice, ctx->stream));
+    if (twoplane) {  // solver call commented out in the reference (soint2d_cfuns.c:2391-2392)
+        if (iters) *iters = 0;
+        return SH_OK;
+    }
+    if (drift) return sh_fail(ctx, SH_EUNSUP, "soint2d: drift=1 is not implemented on the GPU");
+    Geo g{n1, n2, 1, nw, nj1, nj1, 1, n};
+    float *dd = nullptr;
+    SH_TRY(sh_slot(ctx, SLOT_INTERP0 + 1, (size_t)n + (size_t)n * sizeof(float) + 256, (void **)&dd));
+    unsigned char *known = (unsigned char *)(dd + n);
+    int nb = ceil_div(n, TB);
+    hipLaunchKernelGGL(known_kernel, dim3(nb), dim3(TB), 0, ctx->stream, hasmask == 1 ? mask : din, known, n);
+    hipLaunchKernelGGL(zero_f, dim3(nb), dim3(TB), 0, ctx->stream, dd, n);
+    int it = 0;
+    SH_DISPATCH_NW(ctx, nw, NW, SH_TRY(soint_solve<NW>(ctx, g, dip, dip, known, out, dd, niter, &it)));
+    if (iters) *iters = it;
+    return SH_OK;
+}
+
+static int sint_common(sh_ctx *ctx, const float *din, const float *dipi, const float *dipx, const float *mask, int n1,
+                       int n2, int n3, int niter, int ns1, int ns2, int o1, int o2, float eps, float *out, int *iters) {
+    long n = (long)n1 * n2 * n3;
+    int nb = ceil_div(n, TB);
+    bool three = dipx != nullptr;
+    // layout buffers: data, mask, p (xx), x, dipi^T, dipx^X, t1, t2
+    float *b = nullptr;
+    SH_TRY(sh_slot(ctx, SLOT_INTERP0 + 3, (size_t)n * 8 * sizeof(float) + n + 256, (void **)&b));
+    float *dT = b, *mT = b + n, *pT = b + 2 * n, *xT = b + 3 * n, *diT = b + 4 * n, *dxX = b + 5 * n, *t1 = b + 6 * n,
+          *t2 = b + 7 * n;
+    unsigned char *known = (unsigned char *)(b + 8 * n);
+    for (int i3 = 0; i3 < n3; i3++) {
+        long o = (long)i3 * n1 * n2;
+        SH_TRY(sh_transpose_internal(ctx, din + o, dT + o, n1, n2, true));
+        SH_TRY(sh_transpose_internal(ctx, mask + o, mT + o, n1, n2, true));
+        SH_TRY(sh_transpose_internal(ctx, dipi + o, diT + o, n1, n2, true));
+        if (three) SH_TRY(sh_transpose_internal(ctx, dipx + o, t1 + o, n1, n2, true));
+    }
+    if (three) {
+        hipLaunchKernelGGL(permute_xi, dim3(nb), dim3(TB), 0, ctx->stream, t1, dxX, n1, n2, n3, 0);
+        SH_TRY(sh_launch_check(ctx, "permute"));
+    }
+    hipLaunchKernelGGL(known_kernel, dim3(nb), dim3(TB), 0, ctx->stream, mT, known, n);
+    // lam = sqrt(#known / N) (soint3d_cfuns.c:2586-2595)
+    double *part = nullptr, *dsum = nullptr;
+    SH_TRY(sh_slot(ctx, SLOT_RED, (size_t)nb * 4 * sizeof(double), (void **)&part));
+    SH_TRY(sh_slot(ctx, SLOT_SCAL, 16 * sizeof(double), (void **)&dsum));
+    hipLaunchKernelGGL(count_kernel, dim3(nb), dim3(TB), 0, ctx->stream, known, n, part);
+    double h[2];
+    SH_TRY(sum_partials(ctx, part, nb, 1, dsum, h));
+    float lam = (float)h[0];
+    lam = sqrtf(lam / (float)n);
+    Shape3 Sh{};
+    Sh.three = three;
+    Sh.n1 = n1; Sh.n2 = n2; Sh.n3 = n3; Sh.n = n;
+    Sh.t1 = t1; Sh.t2 = t2;
+    Sh.pin = Pwsmooth2{n1, n2, n3, ns1, o1, eps * eps, diT};
+    SH_TRY(pws2_setup(ctx, Sh.pin, SLOT_INTERP0 + 4));
+    if (three) {
+        Sh.px = Pwsmooth2{n1, n3, n2, ns2, o2, eps * eps, dxX};
+        SH_TRY(pws2_setup(ctx, Sh.px, SLOT_INTERP0 + 6));
+    }
+    SH_HIP(ctx, hipMemcpyAsync(pT, dT, sizeof(float) * n, hipMemcpyDeviceToDevice, ctx->stream));
+    SH_TRY(sint_cg(ctx, Sh, known, pT, xT, dT, lam, (float)(10 * 1.19209290E-07F), niter, iters));
+    for (int i3 = 0; i3 < n3; i3++) {
+        long o = (long)i3 * n1 * n2;
+        SH_TRY(sh_transpose_internal(ctx, xT + o, out + o, n1, n2, false));
+    }
+    return SH_OK;
+}
+
+int sh_sint3d(sh_ctx *ctx, const float *din, const float *dipi, const float *dipx, const float *mask, int n1, int n2,
+              int n3, int niter, int ns1, int ns2, int order1, int order2, float eps, float *out, int *iters) {
+    if (!ctx) return SH_ENULL;
+    if (!din || !dipi || !dipx || !mask || !out) return sh_fail(ctx, SH_ENULL, "sint3d: NULL pointer");
+    if (n1 < 8 || n2 < 1 || n3 < 1 || ns1 < 0 || ns2 < 0) return sh_fail(ctx, SH_EINVAL, "sint3d: bad arguments");
+    return sint_common(ctx, din, dipi, dipx, mask, n1, n2, n3, niter, ns1, ns2, order1, order2, eps, out, iters);
+}
+
+int sh_sint2d(sh_ctx *ctx, const float *din, const float *dip, const float *mask, int n1, int n2, int niter, int ns,
+              int order, float eps, float *out, int *iters) {
+    if (!ctx) return SH_ENULL;
+    if (!din || !dip || !mask || !out) return sh_fail(ctx, SH_ENULL, "sint2d: NULL pointer");
+    if (n1 < 8 || n2 < 1 || ns < 0) return sh_fail(ctx, SH_EINVAL, "sint2d: bad arguments");
+    return sint_common(ctx, din, dip, nullptr, mask, n1, n2, 1, niter, ns, 0, order, order, eps, out, iters);
+}
+
+}  // extern "C"
+
+// MT19937 (Matsumoto-Nishimura reference generator) + polar-free Box-Muller
+// pairing of ps_randn_one_bm (soint3d_cfuns.c:2303-2402).
+void sh_mt_randn(unsigned seed, size_t n, float *out) {
+    static const unsigned mag[2] = {0u, 0x9908b0dfu};
+    unsigned mt[624];
+    int mti;
+    mt[0] = seed;
+    for (mti = 1; mti < 624; mti++) mt[mti] = 1812433253u * (mt[mti - 1] ^ (mt[mti - 1] >> 30)) + (unsigned)mti;
+    auto next = [&]() {
+        if (mti >= 624) {
+            int k;
+            for (k = 0; k < 624 - 397; k++) {
+                unsigned y = (mt[k] & 0x80000000u) | (mt[k + 1] & 0x7fffffffu);
+                mt[k] = mt[k + 397] ^ (y >> 1) ^ mag[y & 1u];
+            }
+            for (; k < 623; k++) {
+                unsigned y = (mt[k] & 0x80000000u) | (mt[k + 1] & 0x7fffffffu);
+                mt[k] = mt[k + 397 - 624] ^ (y >> 1) ^ mag[y & 1u];
+            }
+            unsigned y = (mt[623] & 0x80000000u) | (mt[0] & 0x7fffffffu);
+            mt[623] = mt[396] ^ (y >> 1) ^ mag[y & 1u];
+            mti = 0;
+        }
+        unsigned y = mt[mti++];
+        y ^= (y >> 11);
+        y ^= (y << 7) & 0x9d2c5680u;
+        y ^= (y << 15) & 0xefc60000u;
+        y ^= (y >> 18);
+        return y;
+    };
+    bool iset = true;
+    float vset = 0.0f;
+    for (size_t i = 0; i < n; i++) {
+        if (iset) {
+            double x1, x2;
+            do { x1 = next() * (1.0 / 4294967295.0); } while (x1 == 0.0);
+            x2 = next() * (1.0 / 4294967295.0);
+            double z1 = std::sqrt(-2.0 * std::log(x1)), z2 = 2.0 * 3.14159265358979323846264338328 * x2;
+            iset = false;
+            vset = (float)(z1 * std::cos(z2));
+            out[i] = (float)(z1 * std::sin(z2));
+        } else {
+            iset = true;
+            out[i] = vset;
+        }
+    }
+}

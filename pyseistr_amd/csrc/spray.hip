@@ -1,0 +1,972 @@
+// Plane-wave spray and structure-oriented mean / median on gfx950.
+//
+// Algorithm: predict_step / predict1_step / predict2_step, pwspray_lop,
+// pwsmooth_lop (pyseistr/src/sof_cfuns.c:405-1135), the distance-ordered 3-D
+// spray of csomean3d / csomf3d (pyseistr/src/sof3d_cfuns.c:983-1054,
+// 1355-1753) and the median of sf_quantile / mf (sof_cfuns.c:1137-1252).
+//
+// Design:
+//  * traces are re-laid out "trace-fast" ([sample][trace]) so that the lanes
+//    of a wave, each walking its own trace, touch consecutive addresses;
+//  * one thread runs one prediction: the banded LDL^T define, the PWD
+//    normal-matrix build and the forward substitution stream through the
+//    trace in one pass with register windows; the factor and forward result
+//    go to a coalesced per-thread scratch that the backward substitution
+//    reads in reverse.  The float operations and their order are those of
+//    the reference (contraction off), so spray values are bit-exact;
+//  * the 3-D spray runs by dependency level: slot (a,b) of the 9x9 grid
+//    depends on its neighbours toward the centre, so all slots with equal
+//    |a|+|b| are independent (9 launches for radius 4);
+//  * the median is an exact order statistic by rank counting in registers.
+#include "common.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+namespace {
+
+constexpr int TB = 256;
+
+// ---------------------------------------------------------------------------
+// Layout transposes: F-order [trace][n1] <-> trace-fast [n1][trace].
+// ---------------------------------------------------------------------------
+__global__ void to_trace_fast(const float *__restrict__ src, float *__restrict__ dst, int n1, long ntr) {
+    __shared__ float tile[32][33];
+    long t0 = (long)blockIdx.x * 32;
+    int k0 = blockIdx.y * 32;
+    int tx = threadIdx.x, ty = threadIdx.y;  // 32 x 8
+    for (int r = ty; r < 32; r += 8) {
+        long tr = t0 + r;
+        int k = k0 + tx;
+        if (tr < ntr && k < n1) tile[r][tx] = src[tr * n1 + k];
+    }
+    __syncthreads();
+    for (int r = ty; r < 32; r += 8) {
+        int k = k0 + r;
+        long tr = t0 + tx;
+        if (tr < ntr && k < n1) dst[(long)k * ntr + tr] = tile[tx][r];
+    }
+}
+
+__global__ void from_trace_fast(const float *__restrict__ src, float *__restrict__ dst, int n1, long ntr) {
+    __shared__ float tile[32][33];
+    long t0 = (long)blockIdx.x * 32;
+    int k0 = blockIdx.y * 32;
+    int tx = threadIdx.x, ty = threadIdx.y;
+    for (int r = ty; r < 32; r += 8) {
+        int k = k0 + r;
+        long tr = t0 + tx;
+        if (tr < ntr && k < n1) tile[r][tx] = src[(long)k * ntr + tr];
+    }
+    __syncthreads();
+    for (int r = ty; r < 32; r += 8) {
+        long tr = t0 + r;
+        int k = k0 + tx;
+        if (tr < ntr && k < n1) dst[tr * n1 + k] = tile[tx][r];
+    }
+}
+
+int transpose(sh_ctx *ctx, const float *src, float *dst, int n1, long ntr, bool to_fast) {
+    dim3 grid((unsigned)((ntr + 31) / 32), (unsigned)((n1 + 31) / 32));
+    if (to_fast)
+        hipLaunchKernelGGL(to_trace_fast, grid, dim3(32, 8), 0, ctx->stream, src, dst, n1, ntr);
+    else
+        hipLaunchKernelGGL(from_trace_fast, grid, dim3(32, 8), 0, ctx->stream, src, dst, n1, ntr);
+    return sh_launch_check(ctx, "transpose");
+}
+
+__global__ void zero_kernel(float *__restrict__ y, long n) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = 0.0f;
+}
+
+// ---------------------------------------------------------------------------
+// The predictor.  Strided accessor: element k of a per-thread array is
+// p[k * s].  Forward prediction from NF traces (predict1_step for NF = 1,
+// predict2_step for NF = 2, and the forward predict_step with the input as
+// its own source):
+//   (e R + sum_f A_f^T A_f) out = sum_f A_f^T A_f in_f + e * boundary(in)
+// Adjoint predict_step (ADJ): out = A^T A S^{-1} in + e * boundary(S^{-1} in).
+// ---------------------------------------------------------------------------
+struct Acc {
+    const float *p;
+    long s;
+    __device__ float operator[](long k) const { return p[k * s]; }
+};
+
+struct Sink {
+    float *p;
+    long s;
+    __device__ float &operator[](long k) const { return p[k * s]; }
+};
+
+// Taps of the filter at sample kk (zero outside the trace), reversed when rev.
+template <int NW>
+__device__ inline void taps_at(const Acc &q, int kk, int n1, bool rev, float *a) {
+    constexpr int NA = 2 * NW + 1;
+    if (kk < 0 || kk >= n1) {
+#pragma unroll
+        for (int j = 0; j < NA; j++) a[j] = 0.0f;
+        return;
+    }
+    float b[NA];
+    apf_pass<NW>(q[kk], b);
+#pragma unroll
+    for (int j = 0; j < NA; j++) a[j] = rev ? b[NA - 1 - j] : b[j];
+}
+
+// Regulariser row k (regularization, sof_cfuns.c:548-565).
+template <int NB>
+__device__ inline void reg_row(int k, int n1, float e, float &diag, float *offd) {
+    diag = (float)(6. * e);
+    offd[0] = (float)(-4. * e);
+    offd[1] = e;
+#pragma unroll
+    for (int m = 2; m < NB; m++) offd[m] = 0.0f;
+    if (k == 0 || k == n1 - 1) diag = e + e;
+    if (k == 1 || k == n1 - 2) diag = (float)(e + 5. * e);
+    if (k == 0 || k == n1 - 2) offd[0] = (float)(-2. * e);
+}
+
+// Forward sweep: factor (sf_banded_define, sof_cfuns.c:161-185), right-hand
+// side, forward substitution (287-297).  Stores y[k] = b[k]/d[k] and o[m][k]
+// into scr (NB+1 arrays) for the backward sweep.
+template <int NW, int NF, bool ADJ>
+__device__ void pred_forward(const Acc *in, const Acc *q, const bool *rev, Sink scr, long scr_plane, int n1,
+                             float e, const float *tb) {
+    constexpr int NA = 2 * NW + 1, NB = 2 * NW;
+    float A[NF][NA][NA];  // taps at samples k-NW+w
+    float I[NF][NA];      // inputs at samples k..k+2NW
+    float T[NF][NA];      // A^T-side products at samples k-NW..k+NW
+    float Ld[NB], Lo[NB][NB];  // factor rows k-1-r
+    float Bw[NB];              // forward-substituted rows k-1-r
+#pragma unroll
+    for (int f = 0; f < NF; f++) {
+#pragma unroll
+        for (int w = 0; w < NA; w++) taps_at<NW>(q[f], w - NW, n1, rev[f], A[f][w]);
+        if (!ADJ) {
+#pragma unroll
+            for (int w = 0; w < NA; w++) I[f][w] = (w < n1) ? in[f][w] : 0.0f;
+            // T at samples -NW..NW: only sample NW.. can be nonzero
+#pragma unroll
+            for (int w = 0; w < NA; w++) {
+                int kk = w - NW;
+                float t = 0.0f;
+                if (kk >= NW && kk < n1 - NW) {
+                    // needs inputs kk-NW..kk+NW; kk <= NW here -> indices 0..2NW
+#pragma unroll
+                    for (int j = 0; j < NA; j++) t += A[f][w][j] * ((kk + j - NW) < n1 ? in[f][kk + j - NW] : 0.0f);
+                }
+                T[f][w] = t;
+            }
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < NB; r++) {
+        Ld[r] = 0.0f;
+        Bw[r] = 0.0f;
+#pragma unroll
+        for (int m = 0; m < NB; m++) Lo[r][m] = 0.0f;
+    }
+    for (int k = 0; k < n1; k++) {
+        // normal matrix row k (pwd_define, sof_cfuns.c:481-496)
+        float diag, offd[NB];
+        reg_row<NB>(k, n1, e, diag, offd);
+#pragma unroll
+        for (int f = 0; f < NF; f++) {
+#pragma unroll
+            for (int j = 0; j < NA; j++) {
+                int kk = k + j - NW;
+                if (kk >= NW && kk < n1 - NW) {
+                    float aj = A[f][j][j];
+                    diag += aj * aj;
+                }
+            }
+#pragma unroll
+            for (int m = 0; m < NB; m++) {
+#pragma unroll
+                for (int j = m + 1; j < NA; j++) {
+                    int kk = k + j - NW;
+                    if (kk >= NW && kk < n1 - NW) {
+                        float aj = A[f][j][j], am = A[f][j][j - m - 1];
+                        offd[m] += am * aj;
+                    }
+                }
+            }
+        }
+        // LDL^T row k
+        float t = diag;
+        int m1 = k < NB ? k : NB;
+#pragma unroll
+        for (int m = 0; m < NB; m++)
+            if (m < m1) t -= (Lo[m][m]) * (Lo[m][m]) * (Ld[m]);
+        float dk = t;
+        float ok[NB];
+        int nn = (n1 - k - 1) < NB ? (n1 - k - 1) : NB;
+#pragma unroll
+        for (int nidx = 0; nidx < NB; nidx++) {
+            ok[nidx] = 0.0f;
+            if (nidx < nn) {
+                float s = offd[nidx];
+                int mm = k < (NB - nidx - 1) ? k : (NB - nidx - 1);
+#pragma unroll
+                for (int m = 0; m < NB; m++)
+                    if (m < mm) s -= (Lo[m][m]) * (Lo[m][nidx + m + 1]) * (Ld[m]);
+                ok[nidx] = s / dk;
+            }
+        }
+        // right-hand side row k
+        float rhs;
+        if (ADJ) {
+            rhs = in[0][k];
+        } else {
+            float o0 = 0.0f;
+#pragma unroll
+            for (int f = 0; f < NF; f++) {
+                float of = 0.0f;
+#pragma unroll
+                for (int j = 0; j < NA; j++) {
+                    int kk = k + j - NW;
+                    if (kk >= NW && kk < n1 - NW) of += A[f][j][j] * T[f][j];
+                }
+                o0 = (f == 0) ? of : (o0 + of);
+            }
+            rhs = o0;
+            if (k == 0) rhs += e * tb[0];
+            if (k == 1) rhs += e * tb[1];
+            if (k == n1 - 2) rhs += e * tb[2];
+            if (k == n1 - 1) rhs += e * tb[3];
+        }
+        // forward substitution
+        float b = rhs;
+#pragma unroll
+        for (int m = 0; m < NB; m++)
+            if (m < m1) b -= (Lo[m][m]) * Bw[m];
+        Sink y{scr.p + 0 * scr_plane, scr.s};
+        y[k] = b / dk;
+#pragma unroll
+        for (int m = 0; m < NB; m++) {
+            Sink om{scr.p + (long)(1 + m) * scr_plane, scr.s};
+            om[k] = ok[m];
+        }
+        // shift windows
+#pragma unroll
+        for (int r = NB - 1; r > 0; r--) {
+            Ld[r] = Ld[r - 1];
+            Bw[r] = Bw[r - 1];
+#pragma unroll
+            for (int m = 0; m < NB; m++) Lo[r][m] = Lo[r - 1][m];
+        }
+        Ld[0] = dk;
+        Bw[0] = b;
+#pragma unroll
+        for (int m = 0; m < NB; m++) Lo[0][m] = ok[m];
+#pragma unroll
+        for (int f = 0; f < NF; f++) {
+#pragma unroll
+            for (int w = 0; w < NA - 1; w++)
+#pragma unroll
+                for (int j = 0; j < NA; j++) A[f][w][j] = A[f][w + 1][j];
+            taps_at<NW>(q[f], k + 1 + NW, n1, rev[f], A[f][NA - 1]);
+            if (!ADJ) {
+#pragma unroll
+                for (int w = 0; w < NA - 1; w++) {
+                    I[f][w] = I[f][w + 1];
+                    T[f][w] = T[f][w + 1];
+                }
+                int kn = k + 1 + 2 * NW;
+                I[f][NA - 1] = (kn < n1) ? in[f][kn] : 0.0f;
+                int kk = k + 1 + NW;
+                float tn = 0.0f;
+                if (kk >= NW && kk < n1 - NW) {
+#pragma unroll
+                    for (int j = 0; j < NA; j++) tn += A[f][NA - 1][j] * I[f][j];
+                }
+                T[f][NA - 1] = tn;
+            }
+        }
+    }
+}
+
+// Backward substitution (sof_cfuns.c:298-306); writes out[k] (descending)
+// and, when keep != nullptr, the solved values into scr plane 0.
+template <int NW>
+__device__ void pred_backward(Sink scr, long scr_plane, int n1, Sink out) {
+    constexpr int NB = 2 * NW;
+    float nx[NB];
+#pragma unroll
+    for (int m = 0; m < NB; m++) nx[m] = 0.0f;
+    Sink y{scr.p, scr.s};
+    for (int k = n1 - 1; k >= 0; k--) {
+        float t = y[k];
+        int m1 = (n1 - k - 1) < NB ? (n1 - k - 1) : NB;
+#pragma unroll
+        for (int m = 0; m < NB; m++) {
+            if (m < m1) {
+                Sink om{scr.p + (long)(1 + m) * scr_plane, scr.s};
+                t -= om[k] * nx[m];
+            }
+        }
+#pragma unroll
+        for (int m = NB - 1; m > 0; m--) nx[m] = nx[m - 1];
+        nx[0] = t;
+        out[k] = t;
+    }
+}
+
+// Adjoint PWD product inp = A^T A out (pwd_set adj, sof_cfuns.c:505-522)
+// on the solved trace s, plus the boundary terms (predict_step, 636-646).
+template <int NW>
+__device__ void pred_adj_apply(const Acc &s, const Acc &q, bool rev, int n1, float e, Sink out) {
+    constexpr int NA = 2 * NW + 1;
+    float A[NA][NA];   // taps at samples k-NW+w
+    float S[NA];       // solved at samples k+NW-NW.. : window for tmp at k+NW: s[k..k+2NW]
+    float T[NA];       // tmp at samples k-NW..k+NW
+#pragma unroll
+    for (int w = 0; w < NA; w++) taps_at<NW>(q, w - NW, n1, rev, A[w]);
+#pragma unroll
+    for (int w = 0; w < NA; w++) S[w] = (w < n1) ? s[w] : 0.0f;
+    // tmp at samples -NW..NW
+#pragma unroll
+    for (int w = 0; w < NA; w++) {
+        int kk = w - NW;
+        float t = 0.0f;
+        if (kk >= NW && kk < n1 - NW) {
+#pragma unroll
+            for (int j = NA - 1; j >= 0; j--) t += A[w][j] * s[kk - j + NW];
+        }
+        T[w] = t;
+    }
+    float s0 = s[0], s1 = s[1], s2 = s[n1 - 2], s3 = s[n1 - 1];
+    for (int k = 0; k < n1; k++) {
+        float v = 0.0f;
+#pragma unroll
+        for (int w = 0; w < NA; w++) {
+            int i = k - NW + w;
+            if (i >= NW && i < n1 - NW) v += A[w][2 * NW - w] * T[w];
+        }
+        if (k == 0) v += e * s0;
+        if (k == 1) v += e * s1;
+        if (k == n1 - 2) v += e * s2;
+        if (k == n1 - 1) v += e * s3;
+        out[k] = v;
+        // advance
+#pragma unroll
+        for (int w = 0; w < NA - 1; w++) {
+#pragma unroll
+            for (int j = 0; j < NA; j++) A[w][j] = A[w + 1][j];
+            T[w] = T[w + 1];
+            S[w] = S[w + 1];
+        }
+        taps_at<NW>(q, k + 1 + NW, n1, rev, A[NA - 1]);
+        int kn = k + 1 + 2 * NW;
+        S[NA - 1] = (kn < n1) ? s[kn] : 0.0f;
+        int kk = k + 1 + NW;
+        float tn = 0.0f;
+        if (kk >= NW && kk < n1 - NW) {
+            // s[kk - j + NW] = S[2NW - j] (S covers k+1..k+1+2NW)
+#pragma unroll
+            for (int j = NA - 1; j >= 0; j--) tn += A[NA - 1][j] * S[2 * NW - j];
+        }
+        T[NA - 1] = tn;
+    }
+}
+
+// Forward prediction of one trace: out = P(in_0[, in_1]).
+template <int NW, int NF>
+__device__ void predict(const Acc *in, const Acc *q, const bool *rev, Sink scr, long plane, int n1, float e,
+                        Sink out) {
+    float tb[4];
+    if (NF == 1) {
+        tb[0] = in[0][0]; tb[1] = in[0][1]; tb[2] = in[0][n1 - 2]; tb[3] = in[0][n1 - 1];
+    } else {
+        tb[0] = (float)(0.5 * (in[0][0] + in[1][0]));
+        tb[1] = (float)(0.5 * (in[0][1] + in[1][1]));
+        tb[2] = (float)(0.5 * (in[0][n1 - 2] + in[1][n1 - 2]));
+        tb[3] = (float)(0.5 * (in[0][n1 - 1] + in[1][n1 - 1]));
+    }
+    pred_forward<NW, NF, false>(in, q, rev, scr, plane, n1, e, tb);
+    pred_backward<NW>(scr, plane, n1, out);
+}
+
+// Adjoint prediction step (predict_step with adj, sof_cfuns.c:622-649).
+template <int NW>
+__device__ void predict_adj(const Acc &in, const Acc &q, bool rev, Sink scr, long plane, int n1, float e,
+                            Sink out) {
+    float tb[4] = {0, 0, 0, 0};
+    Acc ins[1] = {in};
+    Acc qs[1] = {q};
+    bool rv[1] = {rev};
+    pred_forward<NW, 1, true>(ins, qs, rv, scr, plane, n1, e, tb);
+    // solved values overwrite plane 0 (y) in place
+    Sink y{scr.p, scr.s};
+    pred_backward<NW>(scr, plane, n1, y);
+    Acc s{scr.p, scr.s};
+    pred_adj_apply<NW>(s, q, rev, n1, e, out);
+}
+
+// ---------------------------------------------------------------------------
+// 3-D spray by Manhattan level.  Node table entries: slot, k2-ns2, k3-ns3,
+// update, up2, up3.
+// ---------------------------------------------------------------------------
+struct Node3 {
+    int slot, o2, o3, update, up2, up3;
+};
+
+template <int NW>
+__global__ void __launch_bounds__(TB)
+spray3_level(const float *__restrict__ pi, const float *__restrict__ px, float *__restrict__ U,
+             const Node3 *__restrict__ nodes, int n1, int n2, int n3, int np2, float e, float *__restrict__ scr,
+             long nthr) {
+    long ntr = (long)n2 * n3;
+    long src = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (src >= ntr) return;
+    Node3 nd = nodes[blockIdx.y];
+    int i2 = (int)(src % n2), i3 = (int)(src / n2);
+    int j2 = i2 + nd.o2, j3 = i3 + nd.o3;
+    if (j2 < 0 || j2 >= n2 || j3 < 0 || j3 >= n3) return;
+    long j = j2 + (long)j3 * n2;
+    long plane_u = (long)n1 * ntr;  // one slot
+    Sink out{U + (long)nd.slot * plane_u + j, ntr};
+    long tid = (long)blockIdx.y * gridDim.x * blockDim.x + src;
+    constexpr int NB = 2 * NW;
+    Sink scr_s{scr + tid, nthr};
+    long plane = (long)n1 * nthr;
+    (void)NB;
+    Acc in[2], q[2];
+    bool rev[2];
+    int nf = 0;
+    if (nd.update & 1) {
+        long jj = nd.up2 ? j - 1 : j + 1;
+        int ks = nd.up2 ? nd.slot - 1 : nd.slot + 1;
+        in[nf] = Acc{U + (long)ks * plane_u + jj, ntr};
+        q[nf] = Acc{pi + (nd.up2 ? j - 1 : j), ntr};
+        rev[nf] = nd.up2 != 0;
+        nf++;
+    }
+    if (nd.update & 2) {
+        long jj = nd.up3 ? j - n2 : j + n2;
+        int ks = nd.up3 ? nd.slot - np2 : nd.slot + np2;
+        in[nf] = Acc{U + (long)ks * plane_u + jj, ntr};
+        q[nf] = Acc{px + (nd.up3 ? j - n2 : j), ntr};
+        rev[nf] = nd.up3 != 0;
+        nf++;
+    }
+    if (nf == 1) predict<NW, 1>(in, q, rev, scr_s, plane, n1, e, out);
+    else if (nf == 2) predict<NW, 2>(in, q, rev, scr_s, plane, n1, e, out);
+}
+
+// centre slot = data (trace-fast copy)
+__global__ void copy_slot(const float *__restrict__ d, float *__restrict__ U, long n) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) U[i] = d[i];
+}
+
+// Plain mean over all np slots in slot order (sof3d_cfuns.c:1525-1537).
+__global__ void slot_mean(const float *__restrict__ U, float *__restrict__ out, long n, int np) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float s = 0.0f;
+    for (int k = 0; k < np; k++) s = s + U[(long)k * n + i];
+    out[i] = s / (float)np;
+}
+
+// Exact q-th smallest of N values (rank counting); ties resolved to the
+// first value in window order.
+template <int N>
+__device__ inline float select_rank(const float *v, int q) {
+    float ans = v[0];
+    bool found = false;
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        int lt = 0, le = 0;
+#pragma unroll
+        for (int j = 0; j < N; j++) {
+            lt += (v[j] < v[i]) ? 1 : 0;
+            le += (v[j] <= v[i]) ? 1 : 0;
+        }
+        bool hit = !found && lt <= q && q < le;
+        ans = hit ? v[i] : ans;
+        found = found || hit;
+    }
+    return ans;
+}
+
+// Median of the nmf slots [c-m, c+m] of each sample (mf, axis 2, centre
+// column; sof3d_cfuns.c:1707-1736).  Window padded to N with +inf.
+template <int N>
+__global__ void __launch_bounds__(TB)
+slot_median(const float *__restrict__ U, float *__restrict__ out, long n, int first, int nmf) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float v[N];
+#pragma unroll
+    for (int k = 0; k < N; k++) v[k] = (k < nmf) ? U[(long)(first + k) * n + i] : INFINITY;
+    out[i] = select_rank<N>(v, (nmf - 1) / 2);
+}
+
+int launch_median(sh_ctx *ctx, const float *U, float *out, long n, int first, int nmf) {
+    int g = ceil_div(n, TB);
+#define MED(NN) \
+    if (nmf <= NN) { hipLaunchKernelGGL((slot_median<NN>), dim3(g), dim3(TB), 0, ctx->stream, U, out, n, first, nmf); return sh_launch_check(ctx, "slot_median"); }
+    MED(9) MED(17) MED(33) MED(49) MED(65) MED(81)
+#undef MED
+    return sh_fail(ctx, SH_EUNSUP, "median window %d exceeds 81", nmf);
+}
+
+// ---------------------------------------------------------------------------
+// 2-D spray chains.  Slice layout trace-fast per slice: [i3][k][i2].
+// Thread = (source i2, direction, slice i3).  Forward: the chain walks
+// predict_step(forw) to the ns neighbours (pwspray_lop, sof_cfuns.c:998-1028).
+// U2 layout: [slot][i3][k][i2].
+// ---------------------------------------------------------------------------
+template <int NW>
+__global__ void __launch_bounds__(TB)
+spray2_fwd(const float *__restrict__ x, const float *__restrict__ dip, float *__restrict__ U, int n1, int n2, int n3,
+           int ns, float e, float *__restrict__ scr, long nthr) {
+    long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;  // over n2 * 2
+    int i3 = blockIdx.y;
+    if (gid >= 2L * n2) return;
+    int dir = (int)(gid / n2);  // 0: toward lower i2 (forw=false), 1: toward higher (forw=true)
+    int i = (int)(gid % n2);
+    int ns2 = 2 * ns + 1;
+    long slice = (long)n1 * n2;
+    const float *xs = x + i3 * slice;
+    const float *ds = dip + i3 * slice;
+    long plane_slot = slice * n3;
+    long tid = (long)i3 * 2 * n2 + gid;
+    Sink scr_s{scr + tid, nthr};
+    long plane = (long)n1 * nthr;
+    if (dir == 0) {  // centre slot written once
+        Sink c{U + (long)ns * plane_slot + i3 * slice + i, n2};
+        for (int k = 0; k < n1; k++) c[k] = xs[(long)k * n2 + i];
+    }
+    Acc cur{xs + i, n2};
+    for (int is = 0; is < ns; is++) {
+        int ip = dir == 0 ? i - is - 1 : i + is + 1;
+        if (ip < 0 || ip >= n2) break;
+        int slot = dir == 0 ? ns - is - 1 : ns + is + 1;
+        Acc q{ds + (dir == 0 ? ip : ip - 1), n2};
+        Sink out{U + (long)slot * plane_slot + i3 * slice + ip, n2};
+        Acc in[1] = {cur};
+        Acc qs[1] = {q};
+        bool rv[1] = {dir == 1};
+        predict<NW, 1>(in, qs, rv, scr_s, plane, n1, e, out);
+        cur = Acc{out.p, n2};
+    }
+    (void)ns2;
+}
+
+// Adjoint chains (pwspray_lop adj, sof_cfuns.c:958-996): for output trace i,
+// chain dir 1 gathers slots ns+is+1 of traces i+is+1 (is = ns-1..0) through
+// adjoint predictions; dir 0 likewise from the left.  Results go to
+// C[dir][i3][k][i]; the centre slot is added by spray2_adj_combine.
+template <int NW>
+__global__ void __launch_bounds__(TB)
+spray2_adj(const float *__restrict__ U, const float *__restrict__ dip, float *__restrict__ C, int n1, int n2, int n3,
+           int ns, float e, float *__restrict__ scr, float *__restrict__ tr, long nthr) {
+    long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    int i3 = blockIdx.y;
+    if (gid >= 2L * n2) return;
+    int dir = (int)(gid / n2);
+    int i = (int)(gid % n2);
+    long slice = (long)n1 * n2;
+    long plane_slot = slice * n3;
+    const float *ds = dip + i3 * slice;
+    long tid = (long)i3 * 2 * n2 + gid;
+    Sink scr_s{scr + tid, nthr};
+    long plane = (long)n1 * nthr;
+    Sink t{tr + tid, nthr};   // running trace
+    for (int k = 0; k < n1; k++) t[k] = 0.0f;
+    for (int is = ns - 1; is >= 0; is--) {
+        int ip = dir == 1 ? i + is + 1 : i - is - 1;
+        if (ip < 0 || ip >= n2) continue;
+        int slot = dir == 1 ? ns + is + 1 : ns - is - 1;
+        Acc u{U + (long)slot * plane_slot + i3 * slice + ip, n2};
+        for (int k = 0; k < n1; k++) t[k] = t[k] + u[k];
+        Acc q{ds + (dir == 1 ? ip - 1 : ip), n2};
+        predict_adj<NW>(Acc{t.p, t.s}, q, dir == 1, scr_s, plane, n1, e, t);
+    }
+    Sink c{C + (long)dir * plane_slot + i3 * slice + i, n2};
+    for (int k = 0; k < n1; k++) c[k] = t[k];
+}
+
+// x = ((x or 0) + chain_right) + chain_left + centre: the accumulation
+// order of the adjoint spray (sof_cfuns.c:969-995).  C[0] holds the chains
+// from the left (dir 0), C[1] those from the right (dir 1).
+__global__ void spray2_adj_combine(const float *__restrict__ C, const float *__restrict__ U, float *__restrict__ x,
+                                   long n, int ns, int add) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float v = add ? x[i] : 0.0f;
+    v = v + C[n + i];
+    v = v + C[i];
+    v = v + U[(long)ns * n + i];
+    x[i] = v;
+}
+
+// pwsmooth forward weighting (sof_cfuns.c:1087-1098):
+// out = sum_is (u_is * w_is) * ww, ww = 1/t (t from the spray of ones).
+__global__ void pwsmooth_norm(const float *__restrict__ U1, float *__restrict__ ww, long n, int ns) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int ns2 = 2 * ns + 1;
+    float t = 0.0f;
+    for (int is = 0; is < ns2; is++) {
+        float w = (float)(ns + 1 - abs(is - ns));
+        t += U1[(long)is * n + i] * w * 1.0f;
+    }
+    ww[i] = (0.0f != t) ? (float)(1.0 / t) : 0.0f;
+}
+
+__global__ void pwsmooth_fwd(const float *__restrict__ U, const float *__restrict__ ww, float *__restrict__ out,
+                             long n, int ns, int add) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int ns2 = 2 * ns + 1;
+    float ws = ww[i];
+    float s = add ? out[i] : 0.0f;
+    for (int is = 0; is < ns2; is++) {
+        float w = (float)(ns + 1 - abs(is - ns));
+        s += U[(long)is * n + i] * w * ws;
+    }
+    out[i] = s;
+}
+
+// adjoint weighting: u_is = (smooth * w_is) * ww (sof_cfuns.c:1074-1082)
+__global__ void pwsmooth_adj_weight(const float *__restrict__ sm, const float *__restrict__ ww, float *__restrict__ U,
+                                    long n, int ns) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int ns2 = 2 * ns + 1;
+    float ws = ww[i], v = sm[i];
+    for (int is = 0; is < ns2; is++) {
+        float w = (float)(ns + 1 - abs(is - ns));
+        U[(long)is * n + i] = v * w * ws;
+    }
+}
+
+__global__ void fill_one(float *__restrict__ y, long n) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = 1.0f;
+}
+
+// 3-D visit plan by Manhattan level (dependencies are the axis neighbours
+// toward the centre, sof3d_cfuns.c:1022-1054).
+int plan3(int ns2, int ns3, std::vector<std::vector<Node3>> &levels) {
+    int np2 = 2 * ns2 + 1, np3 = 2 * ns3 + 1;
+    levels.assign(ns2 + ns3 + 1, {});
+    for (int k3 = 0; k3 < np3; k3++)
+        for (int k2 = 0; k2 < np2; k2++) {
+            int a = k2 - ns2, b = k3 - ns3;
+            int L = std::abs(a) + std::abs(b);
+            Node3 nd{};
+            nd.slot = k2 + np2 * k3;
+            nd.o2 = a;
+            nd.o3 = b;
+            nd.update = (a != 0 ? 1 : 0) | (b != 0 ? 2 : 0);
+            nd.up2 = a > 0;  // predecessor toward the centre along axis 2
+            nd.up3 = b > 0;
+            if (L > 0) levels[L].push_back(nd);
+        }
+    return SH_OK;
+}
+
+struct SprayBufs {
+    float *d, *pi, *px, *U, *scr;
+};
+
+// 3-D spray into U (trace-fast, [slot][k][trace]) from trace-fast inputs.
+int spray3_run(sh_ctx *ctx, int order, int n1, int n2, int n3, int ns2, int ns3, float e, const float *dT,
+               const float *piT, const float *pxT, float *U, float **scr_io) {
+    long ntr = (long)n2 * n3, n = (long)n1 * ntr;
+    int np2 = 2 * ns2 + 1, np = np2 * (2 * ns3 + 1);
+    std::vector<std::vector<Node3>> levels;
+    plan3(ns2, ns3, levels);
+    size_t maxn = 0;
+    for (auto &l : levels) maxn = std::max(maxn, l.size());
+    Node3 *dnodes = nullptr;
+    SH_TRY(sh_slot(ctx, SLOT_SPRAY0 + 5, sizeof(Node3) * np, (void **)&dnodes));
+    std::vector<Node3> flat;
+    std::vector<int> off;
+    for (auto &l : levels) {
+        off.push_back((int)flat.size());
+        flat.insert(flat.end(), l.begin(), l.end());
+    }
+    if (!flat.empty())
+        SH_HIP(ctx, hipMemcpyAsync(dnodes, flat.data(), sizeof(Node3) * flat.size(), hipMemcpyHostToDevice, ctx->stream));
+    int g = ceil_div(n, TB);
+    hipLaunchKernelGGL(zero_kernel, dim3(g), dim3(TB), 0, ctx->stream, U, n * np);
+    SH_TRY(sh_launch_check(ctx, "zero"));
+    int ctr = ns3 * np2 + ns2;
+    hipLaunchKernelGGL(copy_slot, dim3(g), dim3(TB), 0, ctx->stream, dT, U + (long)ctr * n, n);
+    SH_TRY(sh_launch_check(ctx, "copy_slot"));
+    int gx = ceil_div(ntr, TB);
+    long nthr = (long)gx * TB * (long)maxn;
+    int NB1 = 2 * order + 1;
+    float *scr = nullptr;
+    SH_TRY(sh_slot(ctx, SLOT_SPRAY0 + 4, (size_t)nthr * n1 * NB1 * sizeof(float), (void **)&scr));
+    for (size_t L = 1; L < levels.size(); L++) {
+        if (levels[L].empty()) continue;
+        dim3 grid(gx, (unsigned)levels[L].size());
+        SH_DISPATCH_NW(ctx, order, NW,
+                       hipLaunchKernelGGL((spray3_level<NW>), grid, dim3(TB), 0, ctx->stream, piT, pxT, U,
+                                          dnodes + off[L], n1, n2, n3, np2, e, scr, nthr));
+        SH_TRY(sh_launch_check(ctx, "spray3_level"));
+    }
+    if (scr_io) *scr_io = scr;
+    return SH_OK;
+}
+
+int check3(sh_ctx *ctx, const float *a, const float *b, const float *c, int n1, int n2, int n3, int r1, int r2,
+           const char *who) {
+    if (!a || !b || !c) return sh_fail(ctx, SH_ENULL, "%s: NULL input", who);
+    if (n1 < 8 || n2 < 1 || n3 < 1) return sh_fail(ctx, SH_EINVAL, "%s: bad shape %d x %d x %d (n1 >= 8)", who, n1, n2, n3);
+    if (r1 < 0 || r2 < 0 || r1 > 16 || r2 > 16) return sh_fail(ctx, SH_EINVAL, "%s: spray radius out of range", who);
+    return SH_OK;
+}
+
+// inputs F-order -> trace-fast buffers in one workspace slot
+int stage3(sh_ctx *ctx, const float *din, const float *dipi, const float *dipx, int n1, long ntr, float **dT,
+           float **piT, float **pxT) {
+    long n = (long)n1 * ntr;
+    float *b = nullptr;
+    SH_TRY(sh_slot(ctx, SLOT_SPRAY0 + 1, (size_t)n * 3 * sizeof(float), (void **)&b));
+    *dT = b; *piT = b + n; *pxT = b + 2 * n;
+    SH_TRY(transpose(ctx, din, *dT, n1, ntr, true));
+    SH_TRY(transpose(ctx, dipi, *piT, n1, ntr, true));
+    if (dipx) SH_TRY(transpose(ctx, dipx, *pxT, n1, ntr, true));
+    return SH_OK;
+}
+
+}  // namespace
+
+// Internal API (declared in common.hpp): 2-D structure-oriented smoothing.
+
+int pws2_spray(sh_ctx *ctx, Pwsmooth2 &P, const float *x) {
+    dim3 grid(ceil_div(2L * P.n2, TB), P.n3);
+    SH_DISPATCH_NW(ctx, P.order, NW,
+                   hipLaunchKernelGGL((spray2_fwd<NW>), grid, dim3(TB), 0, ctx->stream, x, P.dip, P.U, P.n1, P.n2,
+                                      P.n3, P.ns, P.e, P.scr, P.nthr));
+    return sh_launch_check(ctx, "spray2_fwd");
+}
+
+int pws2_setup(sh_ctx *ctx, Pwsmooth2 &P, int slot_base) {
+    long n = (long)P.n1 * P.n2 * P.n3;
+    int ns2 = 2 * P.ns + 1;
+    P.nthr = (long)ceil_div(2L * P.n2, TB) * TB * P.n3;
+    float *b = nullptr;
+    SH_TRY(sh_slot(ctx, slot_base, (size_t)n * (ns2 + 3) * sizeof(float), (void **)&b));
+    P.U = b;
+    P.ww = b + (long)ns2 * n;
+    P.C = b + (long)(ns2 + 1) * n;
+    SH_TRY(sh_slot(ctx, slot_base + 1, (size_t)P.nthr * P.n1 * (2 * P.order + 2) * sizeof(float), (void **)&P.scr));
+    P.tr = P.scr + (long)P.nthr * P.n1 * (2 * P.order + 1);
+    // normalisation: spray of ones (pwsmooth_set, sof_cfuns.c:1102-1122)
+    float *ones = P.C;  // C is free at this point (2n floats)
+    int g = ceil_div(n, TB);
+    hipLaunchKernelGGL(fill_one, dim3(g), dim3(TB), 0, ctx->stream, ones, n);
+    SH_TRY(sh_launch_check(ctx, "fill_one"));
+    hipLaunchKernelGGL(zero_kernel, dim3(ceil_div(n * ns2, TB)), dim3(TB), 0, ctx->stream, P.U, n * ns2);
+    SH_TRY(pws2_spray(ctx, P, ones));
+    hipLaunchKernelGGL(pwsmooth_norm, dim3(g), dim3(TB), 0, ctx->stream, P.U, P.ww, n, P.ns);
+    return sh_launch_check(ctx, "pwsmooth_norm");
+}
+
+// smooth (+)= S trace  or  trace (+)= S^T smooth   (pwsmooth_lop)
+int pws2_apply(sh_ctx *ctx, Pwsmooth2 &P, int adj, int add, float *trace, float *smooth) {
+    long n = (long)P.n1 * P.n2 * P.n3;
+    int ns2 = 2 * P.ns + 1;
+    int g = ceil_div(n, TB);
+    if (!adj) {
+        hipLaunchKernelGGL(zero_kernel, dim3(ceil_div(n * ns2, TB)), dim3(TB), 0, ctx->stream, P.U, n * ns2);
+        SH_TRY(pws2_spray(ctx, P, trace));
+        hipLaunchKernelGGL(pwsmooth_fwd, dim3(g), dim3(TB), 0, ctx->stream, P.U, P.ww, smooth, n, P.ns, add);
+        return sh_launch_check(ctx, "pwsmooth_fwd");
+    }
+    hipLaunchKernelGGL(pwsmooth_adj_weight, dim3(g), dim3(TB), 0, ctx->stream, smooth, P.ww, P.U, n, P.ns);
+    SH_TRY(sh_launch_check(ctx, "pwsmooth_adj_weight"));
+    dim3 grid(ceil_div(2L * P.n2, TB), P.n3);
+    SH_DISPATCH_NW(ctx, P.order, NW,
+                   hipLaunchKernelGGL((spray2_adj<NW>), grid, dim3(TB), 0, ctx->stream, P.U, P.dip, P.C, P.n1, P.n2,
+                                      P.n3, P.ns, P.e, P.scr, P.tr, P.nthr));
+    SH_TRY(sh_launch_check(ctx, "spray2_adj"));
+    hipLaunchKernelGGL(spray2_adj_combine, dim3(g), dim3(TB), 0, ctx->stream, P.C, P.U, trace, n, P.ns, add);
+    return sh_launch_check(ctx, "spray2_adj_combine");
+}
+
+int sh_transpose_internal(sh_ctx *ctx, const float *src, float *dst, int n1, long ntr, bool to_fast) {
+    return transpose(ctx, src, dst, n1, ntr, to_fast);
+}
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int sh_pwspray3d(sh_ctx *ctx, const float *din, const float *dipi, const float *dipx, int n1, int n2, int n3,
+                 int ns2, int ns3, int order, float eps, float *u) {
+    if (!ctx) return SH_ENULL;
+    SH_TRY(check3(ctx, din, dipi, dipx, n1, n2, n3, ns2, ns3, "pwspray3d"));
+    if (!u) return sh_fail(ctx, SH_ENULL, "pwspray3d: NULL output");
+    long ntr = (long)n2 * n3, n = (long)n1 * ntr;
+    int np = (2 * ns2 + 1) * (2 * ns3 + 1);
+    float *dT, *piT, *pxT, *U;
+    SH_TRY(stage3(ctx, din, dipi, dipx, n1, ntr, &dT, &piT, &pxT));
+    SH_TRY(sh_slot(ctx, SLOT_SPRAY0 + 2, (size_t)n * np * sizeof(float), (void **)&U));
+    SH_TRY(spray3_run(ctx, order, n1, n2, n3, ns2, ns3, eps * eps, dT, piT, pxT, U, nullptr));
+    // [slot][k][trace] -> u[(trace*np + slot)*n1 + k]: per slot transpose into a
+    // [trace][k] plane, then interleave slots with a strided copy.
+    float *tmp = nullptr;
+    SH_TRY(sh_slot(ctx, SLOT_SPRAY0 + 3, (size_t)n * sizeof(float), (void **)&tmp));
+    for (int s = 0; s < np; s++) {
+        SH_TRY(transpose(ctx, U + (long)s * n, tmp, n1, ntr, false));
+        SH_HIP(ctx, hipMemcpy2DAsync(u + (long)s * n1, (size_t)np * n1 * sizeof(float), tmp, (size_t)n1 * sizeof(float),
+                                     (size_t)n1 * sizeof(float), (size_t)ntr, hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    return SH_OK;
+}
+
+int sh_somean3d(sh_ctx *ctx, const float *din, const float *dipi, const float *dipx, int n1, int n2, int n3, int r1,
+                int r2, float eps, int order, float *out) {
+    (void)eps;
+    if (!ctx) return SH_ENULL;
+    SH_TRY(check3(ctx, din, dipi, dipx, n1, n2, n3, r1, r2, "somean3d"));
+    if (!out) return sh_fail(ctx, SH_ENULL, "somean3d: NULL output");
+    long ntr = (long)n2 * n3, n = (long)n1 * ntr;
+    int np = (2 * r1 + 1) * (2 * r2 + 1);
+    float *dT, *piT, *pxT, *U;
+    SH_TRY(stage3(ctx, din, dipi, dipx, n1, ntr, &dT, &piT, &pxT));
+    SH_TRY(sh_slot(ctx, SLOT_SPRAY0 + 2, (size_t)n * np * sizeof(float), (void **)&U));
+    const float e = 0.01f * 0.01f;  // eps forced to 0.01 (sof3d_cfuns.c:1402)
+    SH_TRY(spray3_run(ctx, order, n1, n2, n3, r1, r2, e, dT, piT, pxT, U, nullptr));
+    hipLaunchKernelGGL(slot_mean, dim3(ceil_div(n, TB)), dim3(TB), 0, ctx->stream, U, dT, n, np);
+    SH_TRY(sh_launch_check(ctx, "slot_mean"));
+    return transpose(ctx, dT, out, n1, ntr, false);
+}
+
+int sh_somf3d(sh_ctx *ctx, const float *din, const float *dipi, const float *dipx, int n1, int n2, int n3, int r1,
+              int r2, int nmf, int option, int order, float eps, float *out) {
+    (void)eps;
+    if (!ctx) return SH_ENULL;
+    SH_TRY(check3(ctx, din, dipi, dipx, n1, n2, n3, r1, r2, "somf3d"));
+    if (!out) return sh_fail(ctx, SH_ENULL, "somf3d: NULL output");
+    int np = (2 * r1 + 1) * (2 * r2 + 1);
+    if (nmf < 1 || nmf % 2 == 0 || nmf > np)
+        return sh_fail(ctx, SH_EINVAL, "somf3d: median length %d must be odd and <= %d", nmf, np);
+    if (option != 1) return sh_fail(ctx, SH_EUNSUP, "somf3d: only option=1 (MF) is implemented on the GPU");
+    long ntr = (long)n2 * n3, n = (long)n1 * ntr;
+    float *dT, *piT, *pxT, *U;
+    SH_TRY(stage3(ctx, din, dipi, dipx, n1, ntr, &dT, &piT, &pxT));
+    SH_TRY(sh_slot(ctx, SLOT_SPRAY0 + 2, (size_t)n * np * sizeof(float), (void **)&U));
+    const float e = 0.01f * 0.01f;  // eps forced to 0.01 (sof3d_cfuns.c:1586)
+    SH_TRY(spray3_run(ctx, order, n1, n2, n3, r1, r2, e, dT, piT, pxT, U, nullptr));
+    int c = (np - 1) / 2, m = (nmf - 1) / 2;
+    SH_TRY(launch_median(ctx, U, dT, n, c - m, nmf));
+    return transpose(ctx, dT, out, n1, ntr, false);
+}
+
+int sh_pwspray2d(sh_ctx *ctx, int adj, const float *dip, int n1, int n2, int ns, int order, float eps, float *x,
+                 float *u) {
+    if (!ctx) return SH_ENULL;
+    if (!dip || !x || !u) return sh_fail(ctx, SH_ENULL, "pwspray2d: NULL pointer");
+    if (n1 < 8 || n2 < 1 || ns < 0 || ns > 64) return sh_fail(ctx, SH_EINVAL, "pwspray2d: bad arguments");
+    long n = (long)n1 * n2;
+    int ns2 = 2 * ns + 1;
+    float *b = nullptr;
+    SH_TRY(sh_slot(ctx, SLOT_SPRAY0 + 1, (size_t)n * 3 * sizeof(float), (void **)&b));
+    float *xT = b, *dT = b + n, *tmp = b + 2 * n;
+    SH_TRY(transpose(ctx, dip, dT, n1, n2, true));
+    Pwsmooth2 P{n1, n2, 1, ns, order, eps * eps, dT};
+    P.nthr = (long)ceil_div(2L * n2, TB) * TB;
+    float *wb = nullptr;
+    SH_TRY(sh_slot(ctx, SLOT_SPRAY0 + 2, (size_t)n * (ns2 + 2) * sizeof(float), (void **)&wb));
+    P.U = wb;
+    P.C = wb + (long)ns2 * n;
+    SH_TRY(sh_slot(ctx, SLOT_SPRAY0 + 4, (size_t)P.nthr * n1 * (2 * order + 2) * sizeof(float), (void **)&P.scr));
+    P.tr = P.scr + (long)P.nthr * n1 * (2 * order + 1);
+    if (!adj) {
+        SH_TRY(transpose(ctx, x, xT, n1, n2, true));
+        hipLaunchKernelGGL(zero_kernel, dim3(ceil_div(n * ns2, TB)), dim3(TB), 0, ctx->stream, P.U, n * ns2);
+        SH_TRY(pws2_spray(ctx, P, xT));
+        for (int s = 0; s < ns2; s++) {
+            SH_TRY(transpose(ctx, P.U + (long)s * n, tmp, n1, n2, false));
+            SH_HIP(ctx, hipMemcpy2DAsync(u + (long)s * n1, (size_t)ns2 * n1 * sizeof(float), tmp,
+                                         (size_t)n1 * sizeof(float), (size_t)n1 * sizeof(float), (size_t)n2,
+                                         hipMemcpyDeviceToDevice, ctx->stream));
+        }
+        return SH_OK;
+    }
+    // adjoint: u (reference layout) -> slots trace-fast -> chains -> x
+    for (int s = 0; s < ns2; s++) {
+        SH_HIP(ctx, hipMemcpy2DAsync(tmp, (size_t)n1 * sizeof(float), u + (long)s * n1, (size_t)ns2 * n1 * sizeof(float),
+                                     (size_t)n1 * sizeof(float), (size_t)n2, hipMemcpyDeviceToDevice, ctx->stream));
+        SH_TRY(transpose(ctx, tmp, P.U + (long)s * n, n1, n2, true));
+    }
+    dim3 grid(ceil_div(2L * n2, TB), 1);
+    SH_DISPATCH_NW(ctx, order, NW,
+                   hipLaunchKernelGGL((spray2_adj<NW>), grid, dim3(TB), 0, ctx->stream, P.U, P.dip, P.C, n1, n2, 1, ns,
+                                      P.e, P.scr, P.tr, P.nthr));
+    SH_TRY(sh_launch_check(ctx, "spray2_adj"));
+    hipLaunchKernelGGL(spray2_adj_combine, dim3(ceil_div(n, TB)), dim3(TB), 0, ctx->stream, P.C, P.U, xT, n, ns, 0);
+    SH_TRY(sh_launch_check(ctx, "spray2_adj_combine"));
+    return transpose(ctx, xT, x, n1, n2, false);
+}
+
+int sh_somean2d(sh_ctx *ctx, const float *dn, const float *dip, int n1, int n2, int n3, int ns, int order, int adj,
+                float eps, float *out) {
+    if (!ctx) return SH_ENULL;
+    if (!dn || !dip || !out) return sh_fail(ctx, SH_ENULL, "somean2d: NULL pointer");
+    if (n1 < 8 || n2 < 1 || n3 < 1 || ns < 0 || ns > 64) return sh_fail(ctx, SH_EINVAL, "somean2d: bad arguments");
+    long n = (long)n1 * n2 * n3;
+    float *b = nullptr;
+    SH_TRY(sh_slot(ctx, SLOT_SPRAY0 + 1, (size_t)n * 3 * sizeof(float), (void **)&b));
+    float *xT = b, *dT = b + n, *yT = b + 2 * n;
+    // slices [i3][n1][n2] are per-slice transposes of [i3][n2][n1]
+    for (int i3 = 0; i3 < n3; i3++) {
+        SH_TRY(transpose(ctx, dip + (long)i3 * n1 * n2, dT + (long)i3 * n1 * n2, n1, n2, true));
+        SH_TRY(transpose(ctx, dn + (long)i3 * n1 * n2, xT + (long)i3 * n1 * n2, n1, n2, true));
+    }
+    Pwsmooth2 P{n1, n2, n3, ns, order, eps * eps, dT};
+    SH_TRY(pws2_setup(ctx, P, SLOT_SPRAY0 + 6));
+    if (!adj) {
+        SH_TRY(pws2_apply(ctx, P, 0, 0, xT, yT));
+    } else {
+        SH_TRY(pws2_apply(ctx, P, 1, 0, yT, xT));
+    }
+    for (int i3 = 0; i3 < n3; i3++)
+        SH_TRY(transpose(ctx, yT + (long)i3 * n1 * n2, out + (long)i3 * n1 * n2, n1, n2, false));
+    return SH_OK;
+}
+
+int sh_somf2d(sh_ctx *ctx, const float *dn, const float *dip, int n1, int n2, int n3, int ns, int nmf, int option,
+              int order, float eps, float *out) {
+    if (!ctx) return SH_ENULL;
+    if (!dn || !dip || !out) return sh_fail(ctx, SH_ENULL, "somf2d: NULL pointer");
+    if (n1 < 8 || n2 < 1 || n3 < 1 || ns < 0 || ns > 40) return sh_fail(ctx, SH_EINVAL, "somf2d: bad arguments");
+    int np = 2 * ns + 1;
+    if (nmf < 1 || nmf % 2 == 0 || nmf > np)
+        return sh_fail(ctx, SH_EINVAL, "somf2d: median length %d must be odd and <= %d", nmf, np);
+    if (option != 1) return sh_fail(ctx, SH_EUNSUP, "somf2d: only option=1 (MF) is implemented on the GPU");
+    long n = (long)n1 * n2 * n3;
+    float *b = nullptr;
+    SH_TRY(sh_slot(ctx, SLOT_SPRAY0 + 1, (size_t)n * 3 * sizeof(float), (void **)&b));
+    float *xT = b, *dT = b + n, *yT = b + 2 * n;
+    for (int i3 = 0; i3 < n3; i3++) {
+        SH_TRY(transpose(ctx, dip + (long)i3 * n1 * n2, dT + (long)i3 * n1 * n2, n1, n2, true));
+        SH_TRY(transpose(ctx, dn + (long)i3 * n1 * n2, xT + (long)i3 * n1 * n2, n1, n2, true));
+    }
+    Pwsmooth2 P{n1, n2, n3, ns, order, eps * eps, dT};
+    P.nthr = (long)ceil_div(2L * n2, TB) * TB * n3;
+    SH_TRY(sh_slot(ctx, SLOT_SPRAY0 + 6, (size_t)n * np * sizeof(float), (void **)&P.U));
+    SH_TRY(sh_slot(ctx, SLOT_SPRAY0 + 7, (size_t)P.nthr * n1 * (2 * order + 1) * sizeof(float), (void **)&P.scr));
+    hipLaunchKernelGGL(zero_kernel, dim3(ceil_div(n * np, TB)), dim3(TB), 0, ctx->stream, P.U, n * np);
+    SH_TRY(pws2_spray(ctx, P, xT));
+    int m = (nmf - 1) / 2;
+    SH_TRY(launch_median(ctx, P.U, yT, n, ns - m, nmf));
+    for (int i3 = 0; i3 < n3; i3++)
+        SH_TRY(transpose(ctx, yT + (long)i3 * n1 * n2, out + (long)i3 * n1 * n2, n1, n2, false));
+    return SH_OK;
+}
+
+}  // extern "C"

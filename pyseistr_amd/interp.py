@@ -1,0 +1,89 @@
+"""Drop-in structure-oriented interpolation.
+
+soint3dc (pyseistr/soint3d.py:65), soint2dc (pyseistr/soint2d.py:92),
+sint3dc / sint2dc (pyseistr/sint.py:97, 61): same signatures, defaults and
+output shapes; the arithmetic is csoint3d / csoint2d / csint3d / csint2d of
+the C extension (soint3d_cfuns.c, soint2d_cfuns.c), run by the gfx950
+library.  Extensions: ``report=True`` also returns the solver iteration count.
+Not implemented on the GPU (raise NotImplementedError): drift=1, and
+soint2dc(prec=1) (the reference's preconditioned predictive solver).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+
+
+def _iters():
+    return ctypes.c_int(0)
+
+
+def soint3dc(din, mask, dipi, dipx, order=1, niter=100, njs=[1, 1], drift=0, seed=202223, hasmask=1, var=0,
+             verb=1, report=False):
+    din = np.asarray(din)
+    n1, n2, n3 = din.shape
+    ctx = _lib.context()
+    d, m = _lib.to_device(din), _lib.to_device(mask)
+    p, q = _lib.to_device(dipi), _lib.to_device(dipx)
+    out = _lib.empty(n1 * n2 * n3)
+    it = _iters()
+    rc = _lib.load().sh_soint3d(ctx, _lib.ptr(d), _lib.ptr(m), _lib.ptr(p), _lib.ptr(q), n1, n2, n3, int(order),
+                                int(njs[0]), int(njs[1]), int(niter), int(drift), int(seed), int(hasmask),
+                                float(var), _lib.ptr(out), ctypes.byref(it))
+    _lib.check(ctx, rc, "soint3dc")
+    res = _lib.to_host(out).reshape(n1, n2, n3, order="F")
+    return (res, it.value) if report else res
+
+
+def soint2dc(din, mask, dip, order=1, niter=100, njs=[1, 1], drift=0, hasmask=1, twoplane=0, prec=0, verb=1,
+             report=False):
+    din = np.asarray(din)
+    n1, n2 = din.shape
+    if prec:
+        raise NotImplementedError("soint2dc(prec=1) is not implemented on the GPU")
+    if twoplane:
+        dip = np.asarray(dip)[:, :, 0]
+    ctx = _lib.context()
+    d, m, p = _lib.to_device(din), _lib.to_device(mask), _lib.to_device(dip)
+    out = _lib.empty(n1 * n2)
+    it = _iters()
+    rc = _lib.load().sh_soint2d(ctx, _lib.ptr(d), _lib.ptr(m), _lib.ptr(p), n1, n2, int(order), int(njs[0]),
+                                int(niter), int(drift), int(hasmask), int(twoplane), _lib.ptr(out), ctypes.byref(it))
+    _lib.check(ctx, rc, "soint2dc")
+    res = _lib.to_host(out).reshape(n1, n2, order="F")
+    return (res, it.value) if report else res
+
+
+def sint3dc(din, mask, dipi, dipx, niter=100, eps=0.01, ns1=1, ns2=1, order1=1, order2=1, verb=1, report=False):
+    din = np.asarray(din)
+    n1, n2, n3 = din.shape
+    ctx = _lib.context()
+    d, m = _lib.to_device(din), _lib.to_device(mask)
+    p, q = _lib.to_device(dipi), _lib.to_device(dipx)
+    out = _lib.empty(n1 * n2 * n3)
+    it = _iters()
+    rc = _lib.load().sh_sint3d(ctx, _lib.ptr(d), _lib.ptr(p), _lib.ptr(q), _lib.ptr(m), n1, n2, n3, int(niter),
+                               int(ns1), int(ns2), int(order1), int(order2), float(eps), _lib.ptr(out),
+                               ctypes.byref(it))
+    _lib.check(ctx, rc, "sint3dc")
+    res = _lib.to_host(out).reshape(n1, n2, n3, order="F")
+    return (res, it.value) if report else res
+
+
+def sint2dc(din, mask, dip, niter=100, eps=0.01, ns=1, order=1, verb=1, report=False):
+    din = np.asarray(din)
+    n1, n2 = din.shape
+    ctx = _lib.context()
+    d, m, p = _lib.to_device(din), _lib.to_device(mask), _lib.to_device(dip)
+    out = _lib.empty(n1 * n2)
+    it = _iters()
+    rc = _lib.load().sh_sint2d(ctx, _lib.ptr(d), _lib.ptr(p), _lib.ptr(m), n1, n2, int(niter), int(ns),
+                               int(order), float(eps), _lib.ptr(out), ctypes.byref(it))
+    _lib.check(ctx, rc, "sint2dc")
+    res = _lib.to_host(out).reshape(n1, n2, order="F")
+    return (res, it.value) if report else res
+
+
+soint3d = soint3dc
+soint2d = soint2dc

@@ -1,0 +1,105 @@
+"""The FP32 oracle reproduces the reference C extension bit for bit on the
+golden vectors that tests/golden/make_golden.py wrote from the compiled
+reference (dipc, csomean2d, csomf2d, csomean3d, csomf3d, csoint3d,
+csint3d, csoint2d, csint2d)."""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+G = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def load(name):
+    return np.load(os.path.join(G, name + ".npz"))
+
+
+def same(a, b):
+    a = np.asarray(a, np.float32).ravel(order="F")
+    b = np.asarray(b, np.float32).ravel(order="F")
+    np.testing.assert_array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+@pytest.mark.parametrize("name", ["dip3d", "dip2d", "dip2d_o1"])
+def test_dip(name):
+    z = load(name)
+    out = O.dip(z["data"], tuple(z["shape"]), niter=int(z["niter"]), liter=int(z["liter"]),
+                order=int(z["order"]), rect=tuple(z["rect"]))
+    same(out, z["dip"])
+
+
+def test_dip_mask():
+    z = load("dip3d_mask")
+    out = O.dip(z["data"], tuple(z["shape"]), niter=int(z["niter"]), liter=int(z["liter"]),
+                order=int(z["order"]), rect=tuple(z["rect"]), mask=z["mask"])
+    same(out, z["dip"])
+
+
+def test_somean3d():
+    z = load("somean3d")
+    s = tuple(z["shape"])
+    same(O.somean3d(z["data"], z["dipi"], z["dipx"], s, 2, 2, order=2), z["out"])
+
+
+def test_somf3d():
+    z = load("somf3d")
+    s = tuple(z["shape"])
+    same(O.somf3d(z["data"], z["dipi"], z["dipx"], s, 2, 2, order=2), z["out"])
+    same(O.somf3d(z["data"], z["dipi"], z["dipx"], s, 3, 3, order=2), z["out_r3"])
+    same(O.somf3d(z["data"], z["dipi"], z["dipx"], s, 2, 1, order=1), z["out_o1"])
+
+
+def test_somf3d_svmf_mostly_equal():
+    # The reference's space-varying median reads one row past its extended
+    # panel (uninitialised heap memory); the oracle replicates the edge row
+    # instead, so only the samples whose window length depends on that value
+    # may differ.
+    z = load("somf3d")
+    s = tuple(z["shape"])
+    a = O.somf3d(z["data"], z["dipi"], z["dipx"], s, 2, 2, order=2, option=2)
+    assert np.mean(a != z["out_sv"]) < 0.02
+
+
+def test_somean2d():
+    z = load("somean2d")
+    s = tuple(z["shape"])
+    same(O.somean2d(z["data"], z["dip"], s, 3, order=2), z["out"])
+    same(O.somean2d(z["data"], z["dip"], s, 3, order=2, adj=1), z["out_adj"])
+
+
+def test_somf2d():
+    z = load("somf2d")
+    s = tuple(z["shape"])
+    same(O.somf2d(z["data"], z["dip"], s, 3, order=2), z["out"])
+    assert np.mean(O.somf2d(z["data"], z["dip"], s, 3, order=2, option=2) != z["out_sv"]) < 0.02
+
+
+def test_soint3d():
+    z = load("soint3d")
+    s = tuple(z["shape"])
+    args = (z["data"], z["mask"], z["dipi"], z["dipx"], s)
+    same(O.soint3d(*args, order=2, niter=15), z["out"])
+    same(O.soint3d(*args, order=2, niter=4, var=0.05), z["out_var"])
+    same(O.soint3d(*args, order=1, niter=10, hasmask=0), z["out_nomask"])
+
+
+def test_sint3d():
+    z = load("sint3d")
+    s = tuple(z["shape"])
+    same(O.sint3d(z["data"], z["mask"], z["dipi"], z["dipx"], s, niter=8, eps=0.01, ns1=2, ns2=2, order1=2,
+                  order2=2), z["out"])
+
+
+def test_soint2d():
+    z = load("soint2d")
+    s = tuple(z["shape"])
+    same(O.soint2d(z["data"], z["mask"], z["dip"], s, order=2, niter=15), z["out"])
+    same(O.soint2d(z["data"], z["mask"], z["dip"], s, order=1, niter=10, hasmask=0), z["out_nomask"])
+
+
+def test_sint2d():
+    z = load("sint2d")
+    s = tuple(z["shape"])
+    same(O.sint2d(z["data"], z["mask"], z["dip"], s, niter=15, eps=0.01, ns=3, order=2), z["out"])
