@@ -39,6 +39,13 @@ int sh_version(void);
 /* Block until all work queued on the context's stream has finished. */
 int sh_sync(sh_ctx *ctx);
 
+/* Launch timing (no counterpart in the reference; instrumentation for the
+ * benchmark): when enabled, every kernel launch of the context is bracketed
+ * by HIP events on its stream.  sh_prof_report synchronises and writes
+ * {"kernel": [launches, total_ms, total_algorithmic_bytes], ...} to buf. */
+int sh_prof_enable(sh_ctx *ctx, int on);
+int sh_prof_report(sh_ctx *ctx, char *buf, int len);
+
 /* Per-direction iteration record of a dip call: for GN iteration k,
  * cg_iters[d][k] = shaping-CG iterations run (-1: zero denominator),
  * ls_step[d][k] = accepted line-search halving (8 = none accepted). */

@@ -37,6 +37,8 @@ _SIGS = {
     "sh_last_error": (ctypes.c_char_p, [c_void_p]),
     "sh_version": (c_int, []),
     "sh_sync": (c_int, [c_void_p]),
+    "sh_prof_enable": (c_int, [c_void_p, c_int]),
+    "sh_prof_report": (c_int, [c_void_p, ctypes.c_char_p, c_int]),
     "sh_dip": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_int, c_int,
                        c_float, c_int, c_int, c_int, c_void_p, ctypes.POINTER(DipReport)]),
     "sh_pwspray3d": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_int,
@@ -122,6 +124,20 @@ def check(ctx, rc, what):
         if rc == -5:
             raise NotImplementedError(f"{what}: {msg}")
         raise SeistrError(f"{what}: {msg} (status {rc})")
+
+
+def profile_enable(on=True):
+    ctx = context()
+    check(ctx, load().sh_prof_enable(ctx, 1 if on else 0), "profile")
+
+
+def profile_report():
+    """{kernel: (launches, total_ms, total_algorithmic_bytes)} since enable."""
+    import json
+    ctx = context()
+    buf = ctypes.create_string_buffer(1 << 16)
+    check(ctx, load().sh_prof_report(ctx, buf, len(buf)), "profile")
+    return {k: tuple(v) for k, v in json.loads(buf.value.decode()).items()}
 
 
 def ptr(t):

@@ -61,7 +61,54 @@ int reduce_partials(sh_ctx *ctx, const double *part, int nblk, int K, double *ou
     return sh_launch_check(ctx, "reduce_partials");
 }
 
+hipEvent_t sh_prof_event(sh_ctx *ctx) {
+    if (ctx->pool_used == ctx->pool.size()) {
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        ctx->pool.push_back(e);
+    }
+    return ctx->pool[ctx->pool_used++];
+}
+
 extern "C" {
+
+int sh_prof_enable(sh_ctx *ctx, int on) {
+    if (!ctx) return SH_ENULL;
+    SH_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->prof = on;
+    ctx->recs.clear();
+    ctx->pool_used = 0;
+    return SH_OK;
+}
+
+// JSON object {"name": [launches, total_ms, total_algorithmic_bytes], ...}
+int sh_prof_report(sh_ctx *ctx, char *buf, int len) {
+    if (!ctx || !buf || len < 3) return SH_ENULL;
+    SH_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    std::vector<std::string> names;
+    std::vector<double> cnt, ms, by;
+    for (auto &r : ctx->recs) {
+        float t = 0.0f;
+        if (hipEventElapsedTime(&t, r.a, r.b) != hipSuccess) t = 0.0f;
+        size_t k = 0;
+        while (k < names.size() && names[k] != r.name) k++;
+        if (k == names.size()) {
+            names.push_back(r.name);
+            cnt.push_back(0); ms.push_back(0); by.push_back(0);
+        }
+        cnt[k] += 1; ms[k] += t; by[k] += r.bytes;
+    }
+    std::string out = "{";
+    char tmp[256];
+    for (size_t k = 0; k < names.size(); k++) {
+        snprintf(tmp, sizeof tmp, "%s\"%s\": [%.0f, %.6f, %.6e]", k ? ", " : "", names[k].c_str(), cnt[k], ms[k], by[k]);
+        out += tmp;
+    }
+    out += "}";
+    if ((int)out.size() + 1 > len) return sh_fail(ctx, SH_EINVAL, "profile report needs %zu bytes", out.size() + 1);
+    memcpy(buf, out.c_str(), out.size() + 1);
+    return SH_OK;
+}
 
 sh_ctx *sh_ctx_create(void *hip_stream) {
     sh_ctx *ctx = new (std::nothrow) sh_ctx();
@@ -87,6 +134,7 @@ int sh_ctx_trim(sh_ctx *ctx) {
 void sh_ctx_destroy(sh_ctx *ctx) {
     if (!ctx) return;
     sh_ctx_trim(ctx);
+    for (auto e : ctx->pool) (void)hipEventDestroy(e);
     delete ctx;
 }
 

@@ -19,13 +19,31 @@
 // Context: stream, last error, and a set of cached device buffers ("slots")
 // that grow on demand and are reused across calls on the same context.
 // ---------------------------------------------------------------------------
+struct sh_prof_rec {
+    const char *name;
+    double bytes;  // algorithmic bytes of the launch
+    hipEvent_t a, b;
+};
+
 struct sh_ctx {
     hipStream_t stream = nullptr;
     std::string err;
     std::vector<void *> slot_ptr;
     std::vector<size_t> slot_size;
     int device = 0;
+    // optional per-launch timing (HIP events on the context stream)
+    int prof = 0;
+    std::vector<sh_prof_rec> recs;
+    std::vector<hipEvent_t> pool;
+    size_t pool_used = 0;
 };
+
+hipEvent_t sh_prof_event(sh_ctx *ctx);
+// Bracket a launch: PROF_A before, PROF_B(name, bytes) after.
+#define PROF_A(ctx) hipEvent_t prof_a_ = (ctx)->prof ? sh_prof_event(ctx) : nullptr; \
+    if (prof_a_) (void)hipEventRecord(prof_a_, (ctx)->stream)
+#define PROF_B(ctx, name, bytes) do { if (prof_a_) { hipEvent_t b_ = sh_prof_event(ctx); \
+    (void)hipEventRecord(b_, (ctx)->stream); (ctx)->recs.push_back({(name), (double)(bytes), prof_a_, b_}); } } while (0)
 
 int sh_fail(sh_ctx *ctx, int code, const char *fmt, ...);
 

@@ -425,6 +425,29 @@ struct DipGeom {
     long n;
 };
 
+// Algorithmic bytes per sample of one smoothing pass: each array the pass
+// reads or writes counted once (halo re-reads excluded).  After the first
+// CG iteration the model update is pending (x, r, p advanced on the fly).
+double pass_bytes(int P, int E, bool first, bool single) {
+    int rd = 0, wr = 0;
+    if (P == PRO_CG) {  // x, r, w (+ sx, sr; writes x', r')
+        rd += first ? 3 : 5;
+        wr += first ? 0 : 2;
+    } else {
+        rd += 1;  // the field being smoothed
+    }
+    if (E == EPI_PLAIN) {
+        wr += 1;
+    } else if (E == EPI_GP) {  // p (+ sp); writes gp (+ p)
+        rd += first ? 1 : 2;
+        wr += first ? 1 : 2;
+    } else {  // w, gp (unless gp is the input), sp, sx, sr; writes sp, sx, sr
+        rd += 1 + (single ? 0 : 1) + (first ? 0 : 3);
+        wr += 3;
+    }
+    return 4.0 * (rd + wr);
+}
+
 // One triangle-shaping application in -> out with the given prologue and
 // epilogue on the first / last active axis; returns number of blocks of
 // the final pass (for its partial sums).
@@ -438,6 +461,15 @@ int smooth_apply(sh_ctx *ctx, const DipGeom &g, const SmoothPlan &sp, const floa
         float *dst = is_last ? out : ((a % 2 == 0) ? W.t1 : W.t2);
         int ax = sp.axis[a], nb = sp.nb[a];
         int nblk;
+        // algorithmic bytes per sample of this pass (pending update assumed
+        // after the first CG iteration)
+        double bps = pass_bytes(P, E, first != 0, is_first && is_last);
+        static const char *names[2][2][3] = {
+            {{"smooth_contig<0,0>", "smooth_contig<0,1>", "smooth_contig<0,2>"},
+             {"smooth_contig<1,0>", "smooth_contig<1,1>", "smooth_contig<1,2>"}},
+            {{"smooth_strided<0,0>", "smooth_strided<0,1>", "smooth_strided<0,2>"},
+             {"smooth_strided<1,0>", "smooth_strided<1,1>", "smooth_strided<1,2>"}}};
+        PROF_A(ctx);
         if (ax == 0) {
             dim3 grid((unsigned)((long)ceil_div(g.n1, SM_CH1) * g.n2 * g.n3));
             nblk = grid.x;
@@ -462,6 +494,7 @@ int smooth_apply(sh_ctx *ctx, const DipGeom &g, const SmoothPlan &sp, const floa
             SM_S(0, 0) SM_S(0, 1) SM_S(0, 2) SM_S(1, 0) SM_S(1, 1) SM_S(1, 2)
 #undef SM_S
         }
+        PROF_B(ctx, names[ax != 0][P][E], bps * (double)g.n);
         SH_TRY(sh_launch_check(ctx, "smooth"));
         if (nblk > W.nblk_max) return sh_fail(ctx, SH_EINVAL, "internal: partial buffer too small");
         if (is_last) *nblk_out = nblk;
@@ -481,6 +514,9 @@ int pwd_launch(sh_ctx *ctx, int mode, const DipGeom &g, int axis, const float *x
                const float *dp, float lam, float *pout, DipWork &W, const unsigned char *mask) {
     int nblk = ceil_div(g.n, 256);
     const float pmax = 3.402823466e+38f, pmin = -3.402823466e+38f;
+    static const char *names[3] = {"pwd_kernel<apply>", "pwd_kernel<der_divne>", "pwd_kernel<trial>"};
+    static const double bps[3] = {12.0, 20.0, 20.0};
+    PROF_A(ctx);
     if (mode == 0)
         hipLaunchKernelGGL((pwd_kernel<NW, 0>), dim3(nblk), dim3(256), 0, ctx->stream, x, p, dp, lam, pmin, pmax,
                            pout, W.u1, W.u2, mask, g.n1, g.n2, g.n3, axis, W.part, W.sc);
@@ -490,6 +526,7 @@ int pwd_launch(sh_ctx *ctx, int mode, const DipGeom &g, int axis, const float *x
     else
         hipLaunchKernelGGL((pwd_kernel<NW, 2>), dim3(nblk), dim3(256), 0, ctx->stream, x, p, dp, lam, pmin, pmax,
                            pout, W.u1, W.u2, mask, g.n1, g.n2, g.n3, axis, W.part, W.sc);
+    PROF_B(ctx, names[mode], bps[mode] * (double)g.n);
     return sh_launch_check(ctx, "pwd_kernel");
 }
 

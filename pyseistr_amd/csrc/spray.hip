@@ -80,330 +80,7 @@ __global__ void zero_kernel(float *__restrict__ y, long n) {
     if (i < n) y[i] = 0.0f;
 }
 
-// ---------------------------------------------------------------------------
-// The predictor.  Strided accessor: element k of a per-thread array is
-// p[k * s].  Forward prediction from NF traces (predict1_step for NF = 1,
-// predict2_step for NF = 2, and the forward predict_step with the input as
-// its own source):
-//   (e R + sum_f A_f^T A_f) out = sum_f A_f^T A_f in_f + e * boundary(in)
-// Adjoint predict_step (ADJ): out = A^T A S^{-1} in + e * boundary(S^{-1} in).
-// ---------------------------------------------------------------------------
-struct Acc {
-    const float *p;
-    long s;
-    __device__ float operator[](long k) const { return p[k * s]; }
-};
-
-struct Sink {
-    float *p;
-    long s;
-    __device__ float &operator[](long k) const { return p[k * s]; }
-};
-
-// Taps of the filter at sample kk (zero outside the trace), reversed when rev.
-template <int NW>
-__device__ inline void taps_at(const Acc &q, int kk, int n1, bool rev, float *a) {
-    constexpr int NA = 2 * NW + 1;
-    if (kk < 0 || kk >= n1) {
-#pragma unroll
-        for (int j = 0; j < NA; j++) a[j] = 0.0f;
-        return;
-    }
-    float b[NA];
-    apf_pass<NW>(q[kk], b);
-#pragma unroll
-    for (int j = 0; j < NA; j++) a[j] = rev ? b[NA - 1 - j] : b[j];
-}
-
-// Regulariser row k (regularization, sof_cfuns.c:548-565).
-template <int NB>
-__device__ inline void reg_row(int k, int n1, float e, float &diag, float *offd) {
-    diag = (float)(6. * e);
-    offd[0] = (float)(-4. * e);
-    offd[1] = e;
-#pragma unroll
-    for (int m = 2; m < NB; m++) offd[m] = 0.0f;
-    if (k == 0 || k == n1 - 1) diag = e + e;
-    if (k == 1 || k == n1 - 2) diag = (float)(e + 5. * e);
-    if (k == 0 || k == n1 - 2) offd[0] = (float)(-2. * e);
-}
-
-// Forward sweep: factor (sf_banded_define, sof_cfuns.c:161-185), right-hand
-// side, forward substitution (287-297).  Stores y[k] = b[k]/d[k] and o[m][k]
-// into scr (NB+1 arrays) for the backward sweep.
-template <int NW, int NF, bool ADJ>
-__device__ void pred_forward(const Acc *in, const Acc *q, const bool *rev, Sink scr, long scr_plane, int n1,
-                             float e, const float *tb) {
-    constexpr int NA = 2 * NW + 1, NB = 2 * NW;
-    float A[NF][NA][NA];  // taps at samples k-NW+w
-    float I[NF][NA];      // inputs at samples k..k+2NW
-    float T[NF][NA];      // A^T-side products at samples k-NW..k+NW
-    float Ld[NB], Lo[NB][NB];  // factor rows k-1-r
-    float Bw[NB];              // forward-substituted rows k-1-r
-#pragma unroll
-    for (int f = 0; f < NF; f++) {
-#pragma unroll
-        for (int w = 0; w < NA; w++) taps_at<NW>(q[f], w - NW, n1, rev[f], A[f][w]);
-        if (!ADJ) {
-#pragma unroll
-            for (int w = 0; w < NA; w++) I[f][w] = (w < n1) ? in[f][w] : 0.0f;
-            // T at samples -NW..NW: only sample NW.. can be nonzero
-#pragma unroll
-            for (int w = 0; w < NA; w++) {
-                int kk = w - NW;
-                float t = 0.0f;
-                if (kk >= NW && kk < n1 - NW) {
-                    // needs inputs kk-NW..kk+NW; kk <= NW here -> indices 0..2NW
-#pragma unroll
-                    for (int j = 0; j < NA; j++) t += A[f][w][j] * ((kk + j - NW) < n1 ? in[f][kk + j - NW] : 0.0f);
-                }
-                T[f][w] = t;
-            }
-        }
-    }
-#pragma unroll
-    for (int r = 0; r < NB; r++) {
-        Ld[r] = 0.0f;
-        Bw[r] = 0.0f;
-#pragma unroll
-        for (int m = 0; m < NB; m++) Lo[r][m] = 0.0f;
-    }
-    for (int k = 0; k < n1; k++) {
-        // normal matrix row k (pwd_define, sof_cfuns.c:481-496)
-        float diag, offd[NB];
-        reg_row<NB>(k, n1, e, diag, offd);
-#pragma unroll
-        for (int f = 0; f < NF; f++) {
-#pragma unroll
-            for (int j = 0; j < NA; j++) {
-                int kk = k + j - NW;
-                if (kk >= NW && kk < n1 - NW) {
-                    float aj = A[f][j][j];
-                    diag += aj * aj;
-                }
-            }
-#pragma unroll
-            for (int m = 0; m < NB; m++) {
-#pragma unroll
-                for (int j = m + 1; j < NA; j++) {
-                    int kk = k + j - NW;
-                    if (kk >= NW && kk < n1 - NW) {
-                        float aj = A[f][j][j], am = A[f][j][j - m - 1];
-                        offd[m] += am * aj;
-                    }
-                }
-            }
-        }
-        // LDL^T row k
-        float t = diag;
-        int m1 = k < NB ? k : NB;
-#pragma unroll
-        for (int m = 0; m < NB; m++)
-            if (m < m1) t -= (Lo[m][m]) * (Lo[m][m]) * (Ld[m]);
-        float dk = t;
-        float ok[NB];
-        int nn = (n1 - k - 1) < NB ? (n1 - k - 1) : NB;
-#pragma unroll
-        for (int nidx = 0; nidx < NB; nidx++) {
-            ok[nidx] = 0.0f;
-            if (nidx < nn) {
-                float s = offd[nidx];
-                int mm = k < (NB - nidx - 1) ? k : (NB - nidx - 1);
-#pragma unroll
-                for (int m = 0; m < NB; m++)
-                    if (m < mm) s -= (Lo[m][m]) * (Lo[m][nidx + m + 1]) * (Ld[m]);
-                ok[nidx] = s / dk;
-            }
-        }
-        // right-hand side row k
-        float rhs;
-        if (ADJ) {
-            rhs = in[0][k];
-        } else {
-            float o0 = 0.0f;
-#pragma unroll
-            for (int f = 0; f < NF; f++) {
-                float of = 0.0f;
-#pragma unroll
-                for (int j = 0; j < NA; j++) {
-                    int kk = k + j - NW;
-                    if (kk >= NW && kk < n1 - NW) of += A[f][j][j] * T[f][j];
-                }
-                o0 = (f == 0) ? of : (o0 + of);
-            }
-            rhs = o0;
-            if (k == 0) rhs += e * tb[0];
-            if (k == 1) rhs += e * tb[1];
-            if (k == n1 - 2) rhs += e * tb[2];
-            if (k == n1 - 1) rhs += e * tb[3];
-        }
-        // forward substitution
-        float b = rhs;
-#pragma unroll
-        for (int m = 0; m < NB; m++)
-            if (m < m1) b -= (Lo[m][m]) * Bw[m];
-        Sink y{scr.p + 0 * scr_plane, scr.s};
-        y[k] = b / dk;
-#pragma unroll
-        for (int m = 0; m < NB; m++) {
-            Sink om{scr.p + (long)(1 + m) * scr_plane, scr.s};
-            om[k] = ok[m];
-        }
-        // shift windows
-#pragma unroll
-        for (int r = NB - 1; r > 0; r--) {
-            Ld[r] = Ld[r - 1];
-            Bw[r] = Bw[r - 1];
-#pragma unroll
-            for (int m = 0; m < NB; m++) Lo[r][m] = Lo[r - 1][m];
-        }
-        Ld[0] = dk;
-        Bw[0] = b;
-#pragma unroll
-        for (int m = 0; m < NB; m++) Lo[0][m] = ok[m];
-#pragma unroll
-        for (int f = 0; f < NF; f++) {
-#pragma unroll
-            for (int w = 0; w < NA - 1; w++)
-#pragma unroll
-                for (int j = 0; j < NA; j++) A[f][w][j] = A[f][w + 1][j];
-            taps_at<NW>(q[f], k + 1 + NW, n1, rev[f], A[f][NA - 1]);
-            if (!ADJ) {
-#pragma unroll
-                for (int w = 0; w < NA - 1; w++) {
-                    I[f][w] = I[f][w + 1];
-                    T[f][w] = T[f][w + 1];
-                }
-                int kn = k + 1 + 2 * NW;
-                I[f][NA - 1] = (kn < n1) ? in[f][kn] : 0.0f;
-                int kk = k + 1 + NW;
-                float tn = 0.0f;
-                if (kk >= NW && kk < n1 - NW) {
-#pragma unroll
-                    for (int j = 0; j < NA; j++) tn += A[f][NA - 1][j] * I[f][j];
-                }
-                T[f][NA - 1] = tn;
-            }
-        }
-    }
-}
-
-// Backward substitution (sof_cfuns.c:298-306); writes out[k] (descending)
-// and, when keep != nullptr, the solved values into scr plane 0.
-template <int NW>
-__device__ void pred_backward(Sink scr, long scr_plane, int n1, Sink out) {
-    constexpr int NB = 2 * NW;
-    float nx[NB];
-#pragma unroll
-    for (int m = 0; m < NB; m++) nx[m] = 0.0f;
-    Sink y{scr.p, scr.s};
-    for (int k = n1 - 1; k >= 0; k--) {
-        float t = y[k];
-        int m1 = (n1 - k - 1) < NB ? (n1 - k - 1) : NB;
-#pragma unroll
-        for (int m = 0; m < NB; m++) {
-            if (m < m1) {
-                Sink om{scr.p + (long)(1 + m) * scr_plane, scr.s};
-                t -= om[k] * nx[m];
-            }
-        }
-#pragma unroll
-        for (int m = NB - 1; m > 0; m--) nx[m] = nx[m - 1];
-        nx[0] = t;
-        out[k] = t;
-    }
-}
-
-// Adjoint PWD product inp = A^T A out (pwd_set adj, sof_cfuns.c:505-522)
-// on the solved trace s, plus the boundary terms (predict_step, 636-646).
-template <int NW>
-__device__ void pred_adj_apply(const Acc &s, const Acc &q, bool rev, int n1, float e, Sink out) {
-    constexpr int NA = 2 * NW + 1;
-    float A[NA][NA];   // taps at samples k-NW+w
-    float S[NA];       // solved at samples k+NW-NW.. : window for tmp at k+NW: s[k..k+2NW]
-    float T[NA];       // tmp at samples k-NW..k+NW
-#pragma unroll
-    for (int w = 0; w < NA; w++) taps_at<NW>(q, w - NW, n1, rev, A[w]);
-#pragma unroll
-    for (int w = 0; w < NA; w++) S[w] = (w < n1) ? s[w] : 0.0f;
-    // tmp at samples -NW..NW
-#pragma unroll
-    for (int w = 0; w < NA; w++) {
-        int kk = w - NW;
-        float t = 0.0f;
-        if (kk >= NW && kk < n1 - NW) {
-#pragma unroll
-            for (int j = NA - 1; j >= 0; j--) t += A[w][j] * s[kk - j + NW];
-        }
-        T[w] = t;
-    }
-    float s0 = s[0], s1 = s[1], s2 = s[n1 - 2], s3 = s[n1 - 1];
-    for (int k = 0; k < n1; k++) {
-        float v = 0.0f;
-#pragma unroll
-        for (int w = 0; w < NA; w++) {
-            int i = k - NW + w;
-            if (i >= NW && i < n1 - NW) v += A[w][2 * NW - w] * T[w];
-        }
-        if (k == 0) v += e * s0;
-        if (k == 1) v += e * s1;
-        if (k == n1 - 2) v += e * s2;
-        if (k == n1 - 1) v += e * s3;
-        out[k] = v;
-        // advance
-#pragma unroll
-        for (int w = 0; w < NA - 1; w++) {
-#pragma unroll
-            for (int j = 0; j < NA; j++) A[w][j] = A[w + 1][j];
-            T[w] = T[w + 1];
-            S[w] = S[w + 1];
-        }
-        taps_at<NW>(q, k + 1 + NW, n1, rev, A[NA - 1]);
-        int kn = k + 1 + 2 * NW;
-        S[NA - 1] = (kn < n1) ? s[kn] : 0.0f;
-        int kk = k + 1 + NW;
-        float tn = 0.0f;
-        if (kk >= NW && kk < n1 - NW) {
-            // s[kk - j + NW] = S[2NW - j] (S covers k+1..k+1+2NW)
-#pragma unroll
-            for (int j = NA - 1; j >= 0; j--) tn += A[NA - 1][j] * S[2 * NW - j];
-        }
-        T[NA - 1] = tn;
-    }
-}
-
-// Forward prediction of one trace: out = P(in_0[, in_1]).
-template <int NW, int NF>
-__device__ void predict(const Acc *in, const Acc *q, const bool *rev, Sink scr, long plane, int n1, float e,
-                        Sink out) {
-    float tb[4];
-    if (NF == 1) {
-        tb[0] = in[0][0]; tb[1] = in[0][1]; tb[2] = in[0][n1 - 2]; tb[3] = in[0][n1 - 1];
-    } else {
-        tb[0] = (float)(0.5 * (in[0][0] + in[1][0]));
-        tb[1] = (float)(0.5 * (in[0][1] + in[1][1]));
-        tb[2] = (float)(0.5 * (in[0][n1 - 2] + in[1][n1 - 2]));
-        tb[3] = (float)(0.5 * (in[0][n1 - 1] + in[1][n1 - 1]));
-    }
-    pred_forward<NW, NF, false>(in, q, rev, scr, plane, n1, e, tb);
-    pred_backward<NW>(scr, plane, n1, out);
-}
-
-// Adjoint prediction step (predict_step with adj, sof_cfuns.c:622-649).
-template <int NW>
-__device__ void predict_adj(const Acc &in, const Acc &q, bool rev, Sink scr, long plane, int n1, float e,
-                            Sink out) {
-    float tb[4] = {0, 0, 0, 0};
-    Acc ins[1] = {in};
-    Acc qs[1] = {q};
-    bool rv[1] = {rev};
-    pred_forward<NW, 1, true>(ins, qs, rv, scr, plane, n1, e, tb);
-    // solved values overwrite plane 0 (y) in place
-    Sink y{scr.p, scr.s};
-    pred_backward<NW>(scr, plane, n1, y);
-    Acc s{scr.p, scr.s};
-    pred_adj_apply<NW>(s, q, rev, n1, e, out);
-}
+#include "predict.hpp"
 
 // ---------------------------------------------------------------------------
 // 3-D spray by Manhattan level.  Node table entries: slot, k2-ns2, k3-ns3,
@@ -413,15 +90,21 @@ struct Node3 {
     int slot, o2, o3, update, up2, up3;
 };
 
+// The nodes of one level travel as a kernel argument (no host-to-device copy).
+constexpr int MAX_LEVEL_NODES = 64;
+struct Level3 {
+    Node3 nd[MAX_LEVEL_NODES];
+};
+
 template <int NW>
 __global__ void __launch_bounds__(TB)
 spray3_level(const float *__restrict__ pi, const float *__restrict__ px, float *__restrict__ U,
-             const Node3 *__restrict__ nodes, int n1, int n2, int n3, int np2, float e, float *__restrict__ scr,
+             const Level3 nodes, int n1, int n2, int n3, int np2, float e, float *__restrict__ scr,
              long nthr) {
     long ntr = (long)n2 * n3;
     long src = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (src >= ntr) return;
-    Node3 nd = nodes[blockIdx.y];
+    Node3 nd = nodes.nd[blockIdx.y];
     int i2 = (int)(src % n2), i3 = (int)(src / n2);
     int j2 = i2 + nd.o2, j3 = i3 + nd.o3;
     if (j2 < 0 || j2 >= n2 || j3 < 0 || j3 >= n3) return;
@@ -508,7 +191,8 @@ slot_median(const float *__restrict__ U, float *__restrict__ out, long n, int fi
 int launch_median(sh_ctx *ctx, const float *U, float *out, long n, int first, int nmf) {
     int g = ceil_div(n, TB);
 #define MED(NN) \
-    if (nmf <= NN) { hipLaunchKernelGGL((slot_median<NN>), dim3(g), dim3(TB), 0, ctx->stream, U, out, n, first, nmf); return sh_launch_check(ctx, "slot_median"); }
+    if (nmf <= NN) { PROF_A(ctx); hipLaunchKernelGGL((slot_median<NN>), dim3(g), dim3(TB), 0, ctx->stream, U, out, n, first, nmf); \
+        PROF_B(ctx, "slot_median", (nmf + 1) * 4.0 * (double)n); return sh_launch_check(ctx, "slot_median"); }
     MED(9) MED(17) MED(33) MED(49) MED(65) MED(81)
 #undef MED
     return sh_fail(ctx, SH_EUNSUP, "median window %d exceeds 81", nmf);
@@ -685,16 +369,7 @@ int spray3_run(sh_ctx *ctx, int order, int n1, int n2, int n3, int ns2, int ns3,
     plan3(ns2, ns3, levels);
     size_t maxn = 0;
     for (auto &l : levels) maxn = std::max(maxn, l.size());
-    Node3 *dnodes = nullptr;
-    SH_TRY(sh_slot(ctx, SLOT_SPRAY0 + 5, sizeof(Node3) * np, (void **)&dnodes));
-    std::vector<Node3> flat;
-    std::vector<int> off;
-    for (auto &l : levels) {
-        off.push_back((int)flat.size());
-        flat.insert(flat.end(), l.begin(), l.end());
-    }
-    if (!flat.empty())
-        SH_HIP(ctx, hipMemcpyAsync(dnodes, flat.data(), sizeof(Node3) * flat.size(), hipMemcpyHostToDevice, ctx->stream));
+    if (maxn > (size_t)MAX_LEVEL_NODES) return sh_fail(ctx, SH_EUNSUP, "spray radius too large");
     int g = ceil_div(n, TB);
     hipLaunchKernelGGL(zero_kernel, dim3(g), dim3(TB), 0, ctx->stream, U, n * np);
     SH_TRY(sh_launch_check(ctx, "zero"));
@@ -709,9 +384,15 @@ int spray3_run(sh_ctx *ctx, int order, int n1, int n2, int n3, int ns2, int ns3,
     for (size_t L = 1; L < levels.size(); L++) {
         if (levels[L].empty()) continue;
         dim3 grid(gx, (unsigned)levels[L].size());
+        Level3 lv{};
+        for (size_t q = 0; q < levels[L].size(); q++) lv.nd[q] = levels[L][q];
+        double by = 0;
+        for (auto &nd : levels[L]) by += (nd.update == 3 ? 20.0 : 12.0) * (double)n;  // per slot-sample
+        PROF_A(ctx);
         SH_DISPATCH_NW(ctx, order, NW,
                        hipLaunchKernelGGL((spray3_level<NW>), grid, dim3(TB), 0, ctx->stream, piT, pxT, U,
-                                          dnodes + off[L], n1, n2, n3, np2, e, scr, nthr));
+                                          lv, n1, n2, n3, np2, e, scr, nthr));
+        PROF_B(ctx, "spray3_level", by);
         SH_TRY(sh_launch_check(ctx, "spray3_level"));
     }
     if (scr_io) *scr_io = scr;
