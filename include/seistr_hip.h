@@ -137,6 +137,28 @@ int sh_sint3d(sh_ctx *ctx, const float *din, const float *dipi, const float *dip
 int sh_sint2d(sh_ctx *ctx, const float *din, const float *dip, const float *mask,
               int n1, int n2, int niter, int ns, int order, float eps, float *out, int *iters);
 
+/* ---- Learned warm start (no counterpart in the reference) -------------
+ * A 3-D CNN (six 3x3x3 convolutions, 1->32->32->32->32->32->2 channels,
+ * ReLU, zero padding; BF16 activations/weights on the matrix cores, FP32
+ * accumulation, FP32 master parameters) that predicts the initial dip
+ * passed to sh_dip's init -- the reference's commented idip/xdip hook
+ * (pyseistr/src/dip_cfuns.c:1861-1875).  Parameter vector (FP32):
+ *   W1[27][32] b1[32] {W[27][32 in][32 out] b[32]} x4 W6[27][32][2] b6[2],
+ * tap t = offset (t%3-1, (t/3)%3-1, t/9-1) along (i1, i2, i3). */
+long sh_cnn_nparams(void);
+/* dip[2N] = inline || crossline prediction for the volume x[N] * xscale. */
+int sh_cnn_forward(sh_ctx *ctx, const float *params, const float *x, float xscale,
+                   int n1, int n2, int n3, float *dip);
+/* loss = sum wt (y - target)^2 / (2N) over both components (target[2N],
+ * wt nullable [N]); grad[nparams] overwritten.  loss (nullable, host):
+ * filling it synchronises. */
+int sh_cnn_grad(sh_ctx *ctx, const float *params, const float *x, float xscale,
+                const float *target, const float *wt, int n1, int n2, int n3,
+                float *grad, double *loss);
+/* One Adam step (step >= 1; L2 term wd * params added to the gradient). */
+int sh_cnn_adam(sh_ctx *ctx, float *params, const float *grad, float *m, float *v,
+                float lr, float b1, float b2, float eps, int step, float wd);
+
 #ifdef __cplusplus
 }
 #endif

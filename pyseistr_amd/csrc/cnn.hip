@@ -1,0 +1,528 @@
+// Learned warm start for dip estimation: a 6-layer 3-D CNN (3x3x3 kernels,
+// 32 channels, ReLU) that maps a normalised seismic volume to an initial
+// (inline, crossline) dip field.  No counterpart in the reference; the dips
+// it produces only seed the reference's Gauss-Newton loop (sh_dip init).
+//
+// Kernels: BF16 activations in channels-last layout [voxel][32] (voxel =
+// F-order index), FP32 accumulation on the gfx950 matrix cores
+// (v_mfma_f32_32x32x16_bf16) for the 32-channel layers, implicit GEMM with
+// zero padding at the volume faces.  Backward: input gradients reuse the
+// forward kernel with flipped/transposed weights and the ReLU mask of the
+// layer below; weight gradients are per-block MFMA partials reduced in a
+// fixed order.  Master weights and Adam state are FP32.
+#include "common.hpp"
+
+#include <algorithm>
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+namespace {
+
+constexpr int C = 32, NT = 27, TB = 256;
+
+// Parameter layout (FP32, flat): W1[27][32] b1[32] {Wl[27][32ci][32co] bl[32]} x4  W6[27][32][2] b6[2]
+constexpr long OFF_W1 = 0, OFF_B1 = OFF_W1 + NT * C;
+constexpr long L_SZ = (long)NT * C * C + C;
+constexpr long OFF_L(int l) { return OFF_B1 + C + (long)(l - 2) * L_SZ; }  // l = 2..5
+constexpr long OFF_W6 = OFF_B1 + C + 4 * L_SZ, OFF_B6 = OFF_W6 + NT * C * 2;
+constexpr long NPARAM = OFF_B6 + 2;
+
+__device__ inline void tap_offset(int t, int &d1, int &d2, int &d3) {
+    d1 = t % 3 - 1;
+    d2 = (t / 3) % 3 - 1;
+    d3 = t / 9 - 1;
+}
+
+// ---------------------------------------------------------------------------
+// Weight packing: master FP32 -> BF16 [27][32 co][32 ci] for the forward of
+// layers 2..6 (layer 6 padded to 32 outputs with zeros) and the flipped,
+// transposed [27][32 ci][32 co] for the input-gradient pass.
+// ---------------------------------------------------------------------------
+__global__ void pack_weights(const float *__restrict__ P, __bf16 *__restrict__ Wf, __bf16 *__restrict__ Wb) {
+    // Wf, Wb: [5 layers][27][32][32]
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long per = (long)NT * C * C;
+    if (i >= 5 * per) return;
+    int l = (int)(i / per);  // 0..4 -> layers 2..6
+    long r = i % per;
+    int t = (int)(r / (C * C)), a = (int)((r / C) % C), b = (int)(r % C);
+    // forward: Wf[l][t][co=a][ci=b] = W[t][ci=b][co=a]
+    float wf, wb;
+    if (l < 4) {
+        const float *W = P + OFF_L(l + 2);
+        wf = W[((long)t * C + b) * C + a];
+        // backward: Wb[l][t'][co'=a (= ci)][ci'=b (= co)] = W[flip t'][ci=a][co=b]
+        wb = W[((long)(NT - 1 - t) * C + a) * C + b];
+    } else {
+        const float *W = P + OFF_W6;  // [27][32 ci][2 co]
+        wf = (a < 2) ? W[((long)t * C + b) * 2 + a] : 0.0f;
+        wb = (b < 2) ? W[((long)(NT - 1 - t) * C + a) * 2 + b] : 0.0f;
+    }
+    Wf[i] = (__bf16)wf;
+    Wb[i] = (__bf16)wb;
+}
+
+// ---------------------------------------------------------------------------
+// Layer 1 (1 -> 32), FP32 input, ReLU, BF16 output.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(TB) conv1_fwd(const float *__restrict__ x, float xscale, const float *__restrict__ P,
+                                                __bf16 *__restrict__ out, int n1, int n2, int n3) {
+    __shared__ float w[NT * C], b[C];
+    for (int i = threadIdx.x; i < NT * C; i += blockDim.x) w[i] = P[OFF_W1 + i];
+    for (int i = threadIdx.x; i < C; i += blockDim.x) b[i] = P[OFF_B1 + i];
+    __syncthreads();
+    long n = (long)n1 * n2 * n3;
+    long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    int i1 = (int)(v % n1);
+    long r = v / n1;
+    int i2 = (int)(r % n2), i3 = (int)(r / n2);
+    float xv[NT];
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+        int d1 = t % 3 - 1, d2 = (t / 3) % 3 - 1, d3 = t / 9 - 1;
+        int j1 = i1 + d1, j2 = i2 + d2, j3 = i3 + d3;
+        bool in = j1 >= 0 && j1 < n1 && j2 >= 0 && j2 < n2 && j3 >= 0 && j3 < n3;
+        xv[t] = in ? x[v + d1 + (long)n1 * (d2 + (long)n2 * d3)] * xscale : 0.0f;
+    }
+    __bf16 o[C];
+#pragma unroll
+    for (int co = 0; co < C; co++) {
+        float s = b[co];
+#pragma unroll
+        for (int t = 0; t < NT; t++) s += w[t * C + co] * xv[t];
+        o[co] = (__bf16)fmaxf(s, 0.0f);
+    }
+    bf16x8 *dst = (bf16x8 *)(out + v * C);
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        bf16x8 pk;
+#pragma unroll
+        for (int i = 0; i < 8; i++) pk[i] = o[q * 8 + i];
+        dst[q] = pk;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 32-channel 3x3x3 convolution on the matrix cores.
+// Block = 4 waves, tile 64 (i1) x 4 (i2) voxels at one i3; wave w owns row
+// i2 = y0 + w as two 32-voxel M-tiles.  Per tap and 16-channel half:
+// A = input [32 voxels][16 ci] from global (zero outside the volume),
+// B = W [16 ci][32 co] from LDS, D += A B in FP32.
+// MODE 0: out = relu(D + b) BF16;  MODE 1: out[v][0..1] = D + b FP32;
+// MODE 2: out = D * (aux > 0) BF16 (input gradient through the ReLU).
+// ---------------------------------------------------------------------------
+template <int MODE>
+__global__ void __launch_bounds__(TB)
+conv32(const __bf16 *__restrict__ in, const __bf16 *__restrict__ Wg, const float *__restrict__ bias,
+       void *__restrict__ outp, const __bf16 *__restrict__ aux, int n1, int n2, int n3) {
+    __shared__ bf16x8 wl[NT * C * C / 8];  // [27][32 co][32 ci]
+    for (int i = threadIdx.x; i < NT * C * C / 8; i += blockDim.x) wl[i] = ((const bf16x8 *)Wg)[i];
+    __syncthreads();
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int ntx = (n1 + 63) / 64, nty = (n2 + 3) / 4;
+    long bid = blockIdx.x;
+    const int x0 = (int)(bid % ntx) * 64;
+    bid /= ntx;
+    const int y = (int)(bid % nty) * 4 + w;
+    const int z = (int)(bid / nty);
+    const long n12 = (long)n1 * n2;
+    f32x16 acc[2];
+#pragma unroll
+    for (int m = 0; m < 2; m++)
+#pragma unroll
+        for (int r = 0; r < 16; r++) acc[m][r] = 0.0f;
+    const int kq = (lane >> 5) * 8;  // channel offset of this lane inside a 16-channel half
+    const bool rowok = y < n2;
+    for (int t = 0; t < NT; t++) {
+        int d1 = t % 3 - 1, d2 = (t / 3) % 3 - 1, d3 = t / 9 - 1;
+        int j2 = y + d2, j3 = z + d3;
+        bool plane = rowok && j2 >= 0 && j2 < n2 && j3 >= 0 && j3 < n3;
+        long rowbase = (long)j2 * n1 + (long)j3 * n12;
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            bf16x8 b = wl[(t * C + (lane & 31)) * (C / 8) + h * 2 + (lane >> 5)];
+#pragma unroll
+            for (int m = 0; m < 2; m++) {
+                int j1 = x0 + m * 32 + (lane & 31) + d1;
+                bf16x8 a;
+                if (plane && j1 >= 0 && j1 < n1) {
+                    a = *(const bf16x8 *)(in + (rowbase + j1) * C + h * 16 + kq);
+                } else {
+#pragma unroll
+                    for (int i = 0; i < 8; i++) a[i] = (__bf16)0.0f;
+                }
+                acc[m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc[m], 0, 0, 0);
+            }
+        }
+    }
+    if (!rowok) return;
+    const int co = lane & 31;
+    const float bco = (MODE == 2) ? 0.0f : bias[co];
+#pragma unroll
+    for (int m = 0; m < 2; m++) {
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            int row = (r / 4) * 8 + (lane >> 5) * 4 + (r % 4);
+            int i1 = x0 + m * 32 + row;
+            if (i1 >= n1) continue;
+            long v = (long)i1 + (long)y * n1 + (long)z * n12;
+            float s = acc[m][r] + bco;
+            if (MODE == 0) {
+                ((__bf16 *)outp)[v * C + co] = (__bf16)fmaxf(s, 0.0f);
+            } else if (MODE == 1) {
+                if (co < 2) ((float *)outp)[v * 2 + co] = s;
+            } else {
+                float gate = (float)aux[v * C + co] > 0.0f ? 1.0f : 0.0f;
+                ((__bf16 *)outp)[v * C + co] = (__bf16)(s * gate);
+            }
+        }
+    }
+}
+
+// Input gradient of layer 1's output when layer 2 is the consumer is given
+// by conv32<2>; layer 1 needs no input gradient.
+
+// ---------------------------------------------------------------------------
+// Weight gradient of a 32-channel layer: dW[t][ci][co] = sum_v in[v+t][ci]
+// g[v][co] over the voxels of each block (64 (i1) x 4 (i2) at one i3), on
+// the matrix cores with K = voxels.  The block stages its input tile with
+// halo ([ci][voxel] transposed) and the output gradient ([co][voxel]) in
+// LDS; wave w accumulates taps t = w, w+4, ... and writes its partial
+// [27][32][32] sums to part[block]; db partials go to pb[block][32].
+// ---------------------------------------------------------------------------
+constexpr int WG_X = 64, WG_Y = 4;
+constexpr int WG_HX = WG_X + 2, WG_HY = WG_Y + 2;
+
+__global__ void __launch_bounds__(TB)
+conv32_wgrad(const __bf16 *__restrict__ in, const __bf16 *__restrict__ g, float *__restrict__ part,
+             float *__restrict__ pb, int n1, int n2, int n3) {
+    // input tile [ci][z 3][y WG_HY][x WG_HX], gradient tile [co][y][x]
+    __shared__ __bf16 ti[C * 3 * WG_HY * WG_HX];
+    __shared__ __bf16 tg[C * WG_Y * WG_X];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int ntx = (n1 + WG_X - 1) / WG_X, nty = (n2 + WG_Y - 1) / WG_Y;
+    long bid = blockIdx.x;
+    const int x0 = (int)(bid % ntx) * WG_X;
+    bid /= ntx;
+    const int y0 = (int)(bid % nty) * WG_Y;
+    const int z = (int)(bid / nty);
+    const long n12 = (long)n1 * n2;
+    // stage input with halo (zero outside)
+    for (int i = threadIdx.x; i < 3 * WG_HY * WG_HX; i += blockDim.x) {
+        int xx = i % WG_HX, yy = (i / WG_HX) % WG_HY, zz = i / (WG_HX * WG_HY);
+        int j1 = x0 - 1 + xx, j2 = y0 - 1 + yy, j3 = z - 1 + zz;
+        bool ok = j1 >= 0 && j1 < n1 && j2 >= 0 && j2 < n2 && j3 >= 0 && j3 < n3;
+        const __bf16 *src = in + ((long)j1 + (long)j2 * n1 + (long)j3 * n12) * C;
+        for (int c = 0; c < C; c++) ti[c * (3 * WG_HY * WG_HX) + i] = ok ? src[c] : (__bf16)0.0f;
+    }
+    for (int i = threadIdx.x; i < WG_Y * WG_X; i += blockDim.x) {
+        int xx = i % WG_X, yy = i / WG_X;
+        int j1 = x0 + xx, j2 = y0 + yy;
+        bool ok = j1 < n1 && j2 < n2;
+        const __bf16 *src = g + ((long)j1 + (long)j2 * n1 + (long)z * n12) * C;
+        for (int c = 0; c < C; c++) tg[c * (WG_Y * WG_X) + i] = ok ? src[c] : (__bf16)0.0f;
+    }
+    __syncthreads();
+    const int kq = (lane >> 5) * 8;
+    for (int t = w; t < NT; t += 4) {
+        int d1 = t % 3 - 1, d2 = (t / 3) % 3 - 1, d3 = t / 9 - 1;
+        f32x16 acc;
+#pragma unroll
+        for (int r = 0; r < 16; r++) acc[r] = 0.0f;
+        // K = voxels, 16 per MFMA: rows yy, x chunks of 16
+        for (int yy = 0; yy < WG_Y; yy++) {
+            for (int xc = 0; xc < WG_X; xc += 16) {
+                // A[row = ci][k = voxel]: lane -> ci = lane & 31, voxels xc + kq .. +7
+                bf16x8 a, b;
+                const __bf16 *ai = ti + (lane & 31) * (3 * WG_HY * WG_HX) + (d3 + 1) * (WG_HY * WG_HX) +
+                                   (yy + d2 + 1) * WG_HX + (xc + kq + d1 + 1);
+                const __bf16 *bg = tg + (lane & 31) * (WG_Y * WG_X) + yy * WG_X + xc + kq;
+#pragma unroll
+                for (int i = 0; i < 8; i++) {
+                    a[i] = ai[i];
+                    b[i] = bg[i];
+                }
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+            }
+        }
+        // D[row = ci][col = co]
+        float *dst = part + (size_t)blockIdx.x * NT * C * C + (size_t)t * C * C;
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            int ci = (r / 4) * 8 + (lane >> 5) * 4 + (r % 4);
+            dst[ci * C + (lane & 31)] = acc[r];
+        }
+    }
+    // bias partials: sum of g over the tile per channel
+    if (threadIdx.x < C) {
+        float s = 0.0f;
+        for (int i = 0; i < WG_Y * WG_X; i++) s += (float)tg[threadIdx.x * (WG_Y * WG_X) + i];
+        pb[(size_t)blockIdx.x * C + threadIdx.x] = s;
+    }
+}
+
+// Layer 1 weight gradient: dW1[t][co] = sum_v x[v+t] g1[v][co]; block
+// partials over 256 voxels; thread (t, co) pairs.
+__global__ void __launch_bounds__(1024)
+conv1_wgrad(const float *__restrict__ x, float xscale, const __bf16 *__restrict__ g, float *__restrict__ part, int n1,
+            int n2, int n3, int vox_per_block) {
+    long n = (long)n1 * n2 * n3;
+    long v0 = (long)blockIdx.x * vox_per_block;
+    int tid = threadIdx.x;
+    float s = 0.0f, sb = 0.0f;
+    int t = tid / C, co = tid % C;
+    if (tid < NT * C) {
+        int d1 = t % 3 - 1, d2 = (t / 3) % 3 - 1, d3 = t / 9 - 1;
+        for (long v = v0; v < std::min(n, v0 + vox_per_block); v++) {
+            int i1 = (int)(v % n1);
+            long r = v / n1;
+            int i2 = (int)(r % n2), i3 = (int)(r / n2);
+            int j1 = i1 + d1, j2 = i2 + d2, j3 = i3 + d3;
+            bool ok = j1 >= 0 && j1 < n1 && j2 >= 0 && j2 < n2 && j3 >= 0 && j3 < n3;
+            float gv = (float)g[v * C + co];
+            if (ok) s += (x[v + d1 + (long)n1 * (d2 + (long)n2 * d3)] * xscale) * gv;
+            if (t == 0) sb += gv;
+        }
+        part[(size_t)blockIdx.x * (NT * C + C) + tid] = s;
+        if (t == 0) part[(size_t)blockIdx.x * (NT * C + C) + NT * C + co] = sb;
+    }
+}
+
+// Sum block partials (fixed order) and scatter into the gradient vector.
+// mode 0: 32-channel layer: part [nblk][27][32 ci][32 co] (+ pb [nblk][32])
+//         -> G[off + (t*32+ci)*32+co], G[off + 27*1024 + co] (bias)
+// mode 1: final layer (co < 2): -> G[OFF_W6 + (t*32+ci)*2+co], G[OFF_B6+co]
+// mode 2: layer 1: part [nblk][27*32 + 32] -> G[OFF_W1..], G[OFF_B1..]
+__global__ void reduce_wgrad(const float *__restrict__ part, const float *__restrict__ pb, int nblk, int mode, long off,
+                             float *__restrict__ G) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (mode == 2) {
+        if (i >= NT * C + C) return;
+        float s = 0.0f;
+        for (int b = 0; b < nblk; b++) s += part[(size_t)b * (NT * C + C) + i];
+        G[(i < NT * C ? OFF_W1 + i : OFF_B1 + (i - NT * C))] = s;
+        return;
+    }
+    long per = (long)NT * C * C;
+    if (i < per) {
+        int co = (int)(i % C);
+        if (mode == 1 && co >= 2) return;
+        float s = 0.0f;
+        for (int b = 0; b < nblk; b++) s += part[(size_t)b * per + i];
+        if (mode == 0) G[off + i] = s;
+        else G[OFF_W6 + (i / C) * 2 + co] = s;
+    } else if (i < per + C) {
+        int co = (int)(i - per);
+        if (mode == 1 && co >= 2) return;
+        float s = 0.0f;
+        for (int b = 0; b < nblk; b++) s += pb[(size_t)b * C + co];
+        if (mode == 0) G[off + per + co] = s;
+        else G[OFF_B6 + co] = s;
+    }
+}
+
+// Loss gradient: gy = 2 (y - t) m / count (BF16, padded to 32 channels),
+// partial sums of (y - t)^2 m.
+__global__ void __launch_bounds__(TB)
+loss_grad(const float *__restrict__ y, const float *__restrict__ tgt, const float *__restrict__ wt,
+          __bf16 *__restrict__ gy, long n, float scale, double *part) {
+    __shared__ double sh[16];
+    long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double acc = 0.0;
+    if (v < n) {
+        float m = wt ? wt[v] : 1.0f;
+        __bf16 *dst = gy + v * C;
+        for (int c = 0; c < 2; c++) {
+            float d = y[v * 2 + c] - tgt[c * n + v];
+            dst[c] = (__bf16)(2.0f * d * m * scale);
+            acc += (double)d * d * m;
+        }
+        for (int c = 2; c < C; c++) dst[c] = (__bf16)0.0f;
+    }
+    double t = block_sum(acc, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+// Adam on the FP32 master parameters.
+__global__ void adam_kernel(float *__restrict__ P, const float *__restrict__ G, float *__restrict__ M,
+                            float *__restrict__ V, long n, float lr, float b1, float b2, float eps, float c1, float c2,
+                            float wd) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float g = G[i] + wd * P[i];
+    float m = b1 * M[i] + (1.0f - b1) * g;
+    float v = b2 * V[i] + (1.0f - b2) * g * g;
+    M[i] = m;
+    V[i] = v;
+    P[i] -= lr * (m / c1) / (sqrtf(v / c2) + eps);
+}
+
+// (y interleaved [v][2]) -> dips [2][v]
+__global__ void split_dips(const float *__restrict__ y, float *__restrict__ out, long n) {
+    long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    out[v] = y[v * 2];
+    out[n + v] = y[v * 2 + 1];
+}
+
+struct CnnWork {
+    __bf16 *Wf, *Wb;    // packed weights [5][27][32][32]
+    __bf16 *act[6];     // layer outputs 1..5 (act[1..5]); act[0] unused
+    float *y;           // [V][2]
+    __bf16 *gA, *gB;    // gradient ping-pong [V][32]
+    float *part, *pb;
+    double *red, *dsum;
+    long nblk_wg;
+};
+
+int cnn_work(sh_ctx *ctx, long n, bool train, CnnWork &W, int n1, int n2, int n3) {
+    size_t wsz = (size_t)5 * NT * C * C * sizeof(__bf16);
+    char *b = nullptr;
+    size_t actsz = (size_t)n * C * sizeof(__bf16);
+    size_t ysz = ((size_t)n * 2 * sizeof(float) + 255) / 256 * 256;  // keeps later buffers 16-byte aligned
+    int nact = train ? 5 : 2;  // inference ping-pongs two buffers
+    size_t need = 2 * wsz + (size_t)nact * actsz + ysz + (train ? 2 * actsz : 0);
+    SH_TRY(sh_slot(ctx, SLOT_INTERP0 + 12, need, (void **)&b));
+    W.Wf = (__bf16 *)b;
+    W.Wb = (__bf16 *)(b + wsz);
+    char *p = b + 2 * wsz;
+    for (int l = 1; l <= 5; l++) W.act[l] = nullptr;
+    if (train) {
+        for (int l = 1; l <= 5; l++) { W.act[l] = (__bf16 *)p; p += actsz; }
+    } else {
+        W.act[1] = (__bf16 *)p; p += actsz;
+        W.act[2] = (__bf16 *)p; p += actsz;
+    }
+    W.y = (float *)p; p += ysz;
+    if (train) {
+        W.gA = (__bf16 *)p; p += actsz;
+        W.gB = (__bf16 *)p; p += actsz;
+        long nb = (long)ceil_div(n1, WG_X) * ceil_div(n2, WG_Y) * n3;
+        W.nblk_wg = nb;
+        SH_TRY(sh_slot(ctx, SLOT_INTERP0 + 13, (size_t)nb * (NT * C * C + C) * sizeof(float), (void **)&W.part));
+        W.pb = W.part + (size_t)nb * NT * C * C;
+        SH_TRY(sh_slot(ctx, SLOT_INTERP0 + 14, (size_t)ceil_div(n, TB) * sizeof(double) + 64, (void **)&W.red));
+        W.dsum = W.red + ceil_div(n, TB);
+    }
+    return SH_OK;
+}
+
+int conv_launch(sh_ctx *ctx, int mode, const __bf16 *in, const __bf16 *Wg, const float *bias, void *out,
+                const __bf16 *aux, int n1, int n2, int n3, const char *name) {
+    dim3 grid((unsigned)((long)ceil_div(n1, 64) * ceil_div(n2, 4) * n3));
+    PROF_A(ctx);
+    if (mode == 0)
+        hipLaunchKernelGGL((conv32<0>), grid, dim3(TB), 0, ctx->stream, in, Wg, bias, out, aux, n1, n2, n3);
+    else if (mode == 1)
+        hipLaunchKernelGGL((conv32<1>), grid, dim3(TB), 0, ctx->stream, in, Wg, bias, out, aux, n1, n2, n3);
+    else
+        hipLaunchKernelGGL((conv32<2>), grid, dim3(TB), 0, ctx->stream, in, Wg, bias, out, aux, n1, n2, n3);
+    double v = (double)n1 * n2 * n3;
+    PROF_B(ctx, name, v * (64.0 + (mode == 1 ? 8.0 : 64.0) + (mode == 2 ? 64.0 : 0.0)));
+    return sh_launch_check(ctx, name);
+}
+
+int cnn_forward(sh_ctx *ctx, const float *P, const float *x, float xscale, int n1, int n2, int n3, CnnWork &W,
+                bool train) {
+    long n = (long)n1 * n2 * n3;
+    long per = (long)NT * C * C;
+    hipLaunchKernelGGL(pack_weights, dim3(ceil_div(5 * per, TB)), dim3(TB), 0, ctx->stream, P, W.Wf, W.Wb);
+    SH_TRY(sh_launch_check(ctx, "pack_weights"));
+    PROF_A(ctx);
+    hipLaunchKernelGGL(conv1_fwd, dim3(ceil_div(n, TB)), dim3(TB), 0, ctx->stream, x, xscale, P, W.act[1], n1, n2, n3);
+    PROF_B(ctx, "conv1_fwd", (double)n * (4.0 + 64.0));
+    SH_TRY(sh_launch_check(ctx, "conv1_fwd"));
+    __bf16 *cur = W.act[1];
+    for (int l = 2; l <= 5; l++) {
+        __bf16 *dst = train ? W.act[l] : (cur == W.act[1] ? W.act[2] : W.act[1]);
+        SH_TRY(conv_launch(ctx, 0, cur, W.Wf + (size_t)(l - 2) * per, P + OFF_L(l) + per, dst, nullptr, n1, n2, n3,
+                           "conv32<relu>"));
+        cur = dst;
+    }
+    return conv_launch(ctx, 1, cur, W.Wf + (size_t)4 * per, P + OFF_B6, W.y, nullptr, n1, n2, n3, "conv32<final>");
+}
+
+}  // namespace
+
+extern "C" {
+
+long sh_cnn_nparams(void) { return NPARAM; }
+
+// Predict (inline, crossline) dips dip[2N] from the normalised volume x[N].
+int sh_cnn_forward(sh_ctx *ctx, const float *params, const float *x, float xscale, int n1, int n2, int n3,
+                   float *dip) {
+    if (!ctx) return SH_ENULL;
+    if (!params || !x || !dip) return sh_fail(ctx, SH_ENULL, "cnn_forward: NULL pointer");
+    if (n1 < 1 || n2 < 1 || n3 < 1) return sh_fail(ctx, SH_EINVAL, "cnn_forward: bad shape");
+    long n = (long)n1 * n2 * n3;
+    CnnWork W{};
+    SH_TRY(cnn_work(ctx, n, false, W, n1, n2, n3));
+    SH_TRY(cnn_forward(ctx, params, x, xscale, n1, n2, n3, W, false));
+    hipLaunchKernelGGL(split_dips, dim3(ceil_div(n, TB)), dim3(TB), 0, ctx->stream, W.y, dip, n);
+    return sh_launch_check(ctx, "split_dips");
+}
+
+// Loss and gradient: loss = sum wt (y - target)^2 / count over both dip
+// components (target [2][N], wt nullable weights [N]); grad[NPARAM] FP32.
+int sh_cnn_grad(sh_ctx *ctx, const float *params, const float *x, float xscale, const float *target, const float *wt,
+                int n1, int n2, int n3, float *grad, double *loss) {
+    if (!ctx) return SH_ENULL;
+    if (!params || !x || !target || !grad) return sh_fail(ctx, SH_ENULL, "cnn_grad: NULL pointer");
+    if (n1 < 2 || n2 < 1 || n3 < 1) return sh_fail(ctx, SH_EINVAL, "cnn_grad: bad shape");
+    long n = (long)n1 * n2 * n3;
+    long per = (long)NT * C * C;
+    CnnWork W{};
+    SH_TRY(cnn_work(ctx, n, true, W, n1, n2, n3));
+    SH_TRY(cnn_forward(ctx, params, x, xscale, n1, n2, n3, W, true));
+    int nb = ceil_div(n, TB);
+    float scale = 1.0f / (float)(2 * n);
+    hipLaunchKernelGGL(loss_grad, dim3(nb), dim3(TB), 0, ctx->stream, W.y, target, wt, W.gA, n, scale, W.red);
+    SH_TRY(sh_launch_check(ctx, "loss_grad"));
+    if (loss) {
+        double h = 0;
+        SH_TRY(reduce_partials(ctx, W.red, nb, 1, W.dsum));
+        SH_HIP(ctx, hipMemcpyAsync(&h, W.dsum, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        SH_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        *loss = h * (double)scale;
+    }
+    unsigned gwg = (unsigned)W.nblk_wg;
+    int nred = ceil_div(per + C, TB);
+    // layer 6 (input act5, output gradient gA)
+    __bf16 *g = W.gA, *gn = W.gB;
+    for (int l = 6; l >= 2; l--) {
+        const __bf16 *ain = W.act[l - 1];
+        PROF_A(ctx);
+        hipLaunchKernelGGL(conv32_wgrad, dim3(gwg), dim3(TB), 0, ctx->stream, ain, g, W.part, W.pb, n1, n2, n3);
+        PROF_B(ctx, "conv32_wgrad", (double)n * 128.0);
+        SH_TRY(sh_launch_check(ctx, "conv32_wgrad"));
+        hipLaunchKernelGGL(reduce_wgrad, dim3(nred), dim3(TB), 0, ctx->stream, W.part, W.pb, (int)gwg, l == 6 ? 1 : 0,
+                           l == 6 ? 0L : OFF_L(l), grad);
+        SH_TRY(sh_launch_check(ctx, "reduce_wgrad"));
+        // input gradient through the ReLU of layer l-1
+        SH_TRY(conv_launch(ctx, 2, g, W.Wb + (size_t)(l - 2) * per, nullptr, gn, ain, n1, n2, n3, "conv32<bwd>"));
+        std::swap(g, gn);
+    }
+    // layer 1 weights
+    int vpb = 256;
+    int nb1 = ceil_div(n, vpb);
+    SH_TRY(sh_slot(ctx, SLOT_INTERP0 + 15, (size_t)nb1 * (NT * C + C) * sizeof(float), (void **)&W.part));
+    hipLaunchKernelGGL(conv1_wgrad, dim3(nb1), dim3(1024), 0, ctx->stream, x, xscale, g, W.part, n1, n2, n3, vpb);
+    SH_TRY(sh_launch_check(ctx, "conv1_wgrad"));
+    hipLaunchKernelGGL(reduce_wgrad, dim3(ceil_div(NT * C + C, TB)), dim3(TB), 0, ctx->stream, W.part, nullptr, nb1, 2,
+                       0L, grad);
+    return sh_launch_check(ctx, "reduce_wgrad");
+}
+
+int sh_cnn_adam(sh_ctx *ctx, float *params, const float *grad, float *m, float *v, float lr, float b1, float b2,
+                float eps, int step, float wd) {
+    if (!ctx) return SH_ENULL;
+    if (!params || !grad || !m || !v || step < 1) return sh_fail(ctx, SH_EINVAL, "cnn_adam: bad arguments");
+    float c1 = 1.0f - powf(b1, (float)step), c2 = 1.0f - powf(b2, (float)step);
+    hipLaunchKernelGGL(adam_kernel, dim3(ceil_div(NPARAM, TB)), dim3(TB), 0, ctx->stream, params, grad, m, v, NPARAM,
+                       lr, b1, b2, eps, c1, c2, wd);
+    return sh_launch_check(ctx, "adam");
+}
+
+}  // extern "C"

@@ -221,7 +221,9 @@ struct Pwsmooth2 {
     float *C;           // adjoint chains [2][...]
     float *scr, *tr;    // per-thread solver scratch and running trace
     long nthr;
+    float *F;           // LDL^T factors, two variants per trace
 };
 int pws2_setup(sh_ctx *ctx, Pwsmooth2 &P, int slot_base);
+int pws2_prepare(sh_ctx *ctx, Pwsmooth2 &P, int slot_scr, int slot_F);
 int pws2_apply(sh_ctx *ctx, Pwsmooth2 &P, int adj, int add, float *trace, float *smooth);
 int sh_transpose_internal(sh_ctx *ctx, const float *src, float *dst, int n1, long ntr, bool to_fast);

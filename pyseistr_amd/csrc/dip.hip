@@ -14,6 +14,7 @@
 #include "common.hpp"
 
 #include <algorithm>
+#include <cstdlib>
 
 namespace {
 
@@ -377,6 +378,97 @@ smooth_strided(const float *__restrict__ in, float *__restrict__ out, int n1, in
                      blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z), sh);
 }
 
+// All axes in one pass ("2.5-D"): a block owns a TI1 x TI2 tile of
+// (i1, i2) outputs and marches along i3 over a segment of planes.  Each
+// input plane (with halos, folded at the volume edges) is smoothed along i1
+// and i2 in LDS and pushed into a ring of 2*h3+1 planes; the i3 stencil is
+// applied from the ring.  Axes with radius 1 have h = 0 (identity).
+constexpr int F_TI1 = 64, F_TI2 = 8;
+
+template <int PRO, int EPI>
+__global__ void __launch_bounds__(256)
+smooth_fused(const float *__restrict__ in, float *__restrict__ out, int n1, int n2, int n3, int r1, int r2, int r3,
+             int seg, CgArgs A, const DipScal *sc, double *part, int first) {
+    if (sc->cg_done) return;
+    extern __shared__ float lds[];
+    __shared__ float w1[SM_HMAX + 1], w2[SM_HMAX + 1], w3[SM_HMAX + 1];
+    __shared__ double sh[16];
+    const int h1 = r1 - 1, h2 = r2 - 1, h3 = r3 - 1;
+    const int W1 = F_TI1 + 2 * h1, R2 = F_TI2 + 2 * h2, RING = 2 * h3 + 1;
+    float *tile = lds;                 // [R2][W1]
+    float *s1 = tile + R2 * W1;        // [R2][F_TI1]
+    float *ring = s1 + R2 * F_TI1;     // [RING][F_TI2][F_TI1]
+    const int tx = threadIdx.x, ty = threadIdx.y;  // 64 x 4
+    const int ntile1 = (n1 + F_TI1 - 1) / F_TI1;
+    const int x0 = (blockIdx.x % ntile1) * F_TI1, y0 = (blockIdx.x / ntile1) * F_TI2;
+    const int z0 = blockIdx.y * seg, z1 = min(n3, z0 + seg);
+    const long n12 = (long)n1 * n2;
+    bool pend = (PRO == PRO_CG || EPI == EPI_GP) ? (sc->pending != 0) : false;
+    float a2 = pend ? (float)sc->alpha2 : 0.0f;
+    float a1 = (EPI == EPI_S) ? (float)sc->alpha : 0.0f;
+    int cur = (PRO == PRO_CG) ? sc->cur : 0;
+    {
+        int t = tx + 64 * ty;
+        for (int k = t; k <= SM_HMAX; k += 256) {
+            w1[k] = (k < r1) ? (float)((double)(r1 - k) / ((double)r1 * r1)) : 0.0f;
+            w2[k] = (k < r2) ? (float)((double)(r2 - k) / ((double)r2 * r2)) : 0.0f;
+            w3[k] = (k < r3) ? (float)((double)(r3 - k) / ((double)r3 * r3)) : 0.0f;
+        }
+    }
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int q = z0 - h3; q < z1 + h3; q++) {
+        const int p = fold_idx(q, n3);
+        const bool ownz = q >= z0 && q < z1;
+        // 1. load the plane tile with halos
+        for (int r = ty; r < R2; r += 4) {
+            int yy = y0 - h2 + r;
+            int iy = fold_idx(yy, n2);
+            for (int c = tx; c < W1; c += 64) {
+                int xx = x0 - h1 + c;
+                int ix = fold_idx(xx, n1);
+                bool own = ownz && xx >= x0 && xx < x0 + F_TI1 && xx < n1 && yy >= y0 && yy < y0 + F_TI2 && yy < n2;
+                tile[r * W1 + c] = sm_load<PRO>(in, ix + (long)n1 * iy + n12 * p, own, A, pend, a2, cur);
+            }
+        }
+        __syncthreads();
+        // 2. along i1
+        for (int r = ty; r < R2; r += 4) {
+            const float *c = tile + r * W1 + tx + h1;
+            float s = w1[0] * c[0];
+            for (int k = 1; k <= h1; k++) s += w1[k] * (c[-k] + c[k]);
+            s1[r * F_TI1 + tx] = s;
+        }
+        __syncthreads();
+        // 3. along i2, into the ring
+        const int slot = (q - (z0 - h3)) % RING;
+        for (int r = ty; r < F_TI2; r += 4) {
+            const float *c = s1 + (r + h2) * F_TI1 + tx;
+            float s = w2[0] * c[0];
+            for (int k = 1; k <= h2; k++) s += w2[k] * (c[-k * F_TI1] + c[k * F_TI1]);
+            ring[(slot * F_TI2 + r) * F_TI1 + tx] = s;
+        }
+        __syncthreads();
+        // 4. along i3 for plane o = q - h3
+        const int o = q - h3;
+        if (o >= z0) {
+            const int so = (o - (z0 - h3)) % RING;
+            for (int r = ty; r < F_TI2; r += 4) {
+                int ix = x0 + tx, iy = y0 + r;
+                float s = w3[0] * ring[(so * F_TI2 + r) * F_TI1 + tx];
+                for (int k = 1; k <= h3; k++) {
+                    int sm = (so - k + RING) % RING, spl = (so + k) % RING;
+                    s += w3[k] * (ring[(sm * F_TI2 + r) * F_TI1 + tx] + ring[(spl * F_TI2 + r) * F_TI1 + tx]);
+                }
+                if (ix < n1 && iy < n2) sm_store<EPI>(s, out, ix + (long)n1 * iy + n12 * o, A, pend, a2, a1, first, acc);
+            }
+        }
+        // the next plane's loads may overwrite tile/s1 only after everyone
+        // has finished with them; the ring slot reused next is not read here
+        __syncthreads();
+    }
+    sm_partials<EPI>(acc, part, gridDim.x * gridDim.y, blockIdx.x + gridDim.x * blockIdx.y, sh);
+}
+
 // mask32 (dip_cfuns.c:914-966, both = false)
 __global__ void mask_kernel(const float *__restrict__ md, unsigned char *__restrict__ m0,
                             unsigned char *__restrict__ m1, int n1, int n2, int n3, int nw) {
@@ -418,6 +510,8 @@ struct SmoothPlan {
     int naxes;
     int axis[3];
     int nb[3];
+    int fused;  // all axes in one pass (smooth_fused)
+    int r[3];   // radius per axis (1 = inactive)
 };
 
 struct DipGeom {
@@ -451,8 +545,39 @@ double pass_bytes(int P, int E, bool first, bool single) {
 // One triangle-shaping application in -> out with the given prologue and
 // epilogue on the first / last active axis; returns number of blocks of
 // the final pass (for its partial sums).
+int fused_segments(const DipGeom &g, int h3) {
+    long tiles = (long)ceil_div(g.n1, F_TI1) * ceil_div(g.n2, F_TI2);
+    int nseg = (int)std::max<long>(1, (2048 + tiles - 1) / tiles);
+    int minseg = std::max(8, 4 * h3);  // keep the halo-plane overhead modest
+    nseg = std::min(nseg, std::max(1, g.n3 / minseg));
+    return ceil_div(g.n3, ceil_div(g.n3, nseg));  // segment length
+}
+
 int smooth_apply(sh_ctx *ctx, const DipGeom &g, const SmoothPlan &sp, const float *in, float *out, int pro,
                  int epi, const CgArgs &A, DipWork &W, int first, int *nblk_out) {
+    if (sp.fused) {
+        int seg = fused_segments(g, sp.r[2] - 1);
+        dim3 grid((unsigned)((long)ceil_div(g.n1, F_TI1) * ceil_div(g.n2, F_TI2)), ceil_div(g.n3, seg));
+        int nblk = grid.x * grid.y;
+        if (nblk > W.nblk_max) return sh_fail(ctx, SH_EINVAL, "internal: partial buffer too small");
+        int h1 = sp.r[0] - 1, h2 = sp.r[1] - 1, h3 = sp.r[2] - 1;
+        size_t lds = sizeof(float) * ((size_t)(F_TI2 + 2 * h2) * (F_TI1 + 2 * h1) + (size_t)(F_TI2 + 2 * h2) * F_TI1 +
+                                      (size_t)(2 * h3 + 1) * F_TI2 * F_TI1);
+        double bps = pass_bytes(pro, epi, first != 0, true);
+        static const char *names[2][3] = {{"smooth_fused<0,0>", "smooth_fused<0,1>", "smooth_fused<0,2>"},
+                                          {"smooth_fused<1,0>", "smooth_fused<1,1>", "smooth_fused<1,2>"}};
+        PROF_A(ctx);
+#define SM_F(PP, EE)                                                                                   \
+    if (pro == PP && epi == EE)                                                                        \
+        hipLaunchKernelGGL((smooth_fused<PP, EE>), grid, dim3(64, 4), lds, ctx->stream, in, out, g.n1, g.n2, g.n3, \
+                           sp.r[0], sp.r[1], sp.r[2], seg, A, W.sc, W.part, first);
+        SM_F(0, 0) SM_F(0, 1) SM_F(0, 2) SM_F(1, 0) SM_F(1, 1) SM_F(1, 2)
+#undef SM_F
+        PROF_B(ctx, names[pro][epi], bps * (double)g.n);
+        SH_TRY(sh_launch_check(ctx, "smooth_fused"));
+        *nblk_out = nblk;
+        return SH_OK;
+    }
     const float *src = in;
     for (int a = 0; a < sp.naxes; a++) {
         bool is_first = a == 0, is_last = a == sp.naxes - 1;
@@ -592,6 +717,18 @@ extern "C" int sh_dip(sh_ctx *ctx, const float *data, const float *mask, const f
             plan.nb[plan.naxes] = rect[a];
             plan.naxes++;
         }
+    for (int a = 0; a < 3; a++) plan.r[a] = rect[a] > 1 ? rect[a] : 1;
+    {
+        // fused single pass unless the LDS footprint is too large
+        int h1 = plan.r[0] - 1, h2 = plan.r[1] - 1, h3 = plan.r[2] - 1;
+        size_t lds = sizeof(float) * ((size_t)(F_TI2 + 2 * h2) * (F_TI1 + 2 * h1) + (size_t)(F_TI2 + 2 * h2) * F_TI1 +
+                                      (size_t)(2 * h3 + 1) * F_TI2 * F_TI1);
+        const char *env = getenv("SH_SMOOTH_SEPARABLE");
+        // halo re-read factor of the fused tile; wide radii along i2 favour
+        // the separable passes
+        double halo = (double)(F_TI2 + 2 * h2) / F_TI2 * (double)(F_TI1 + 2 * h1) / F_TI1;
+        plan.fused = (lds <= 60 * 1024) && halo <= 3.0 && !(env && env[0] == '1');
+    }
     if (plan.naxes == 0) {  // no smoothing: identity through a 1-tap pass
         plan.naxes = 1;
         plan.axis[0] = 0;
@@ -610,6 +747,7 @@ extern "C" int sh_dip(sh_ctx *ctx, const float *data, const float *mask, const f
     long nblk_max = std::max<long>(ceil_div(n, 256), (long)ceil_div(n1, SM_CH1) * n2 * n3);
     nblk_max = std::max<long>(nblk_max, (long)ceil_div(n1, 32) * ceil_div(n2, SM_CHS) * n3);
     nblk_max = std::max<long>(nblk_max, (long)ceil_div(n1, 32) * ceil_div(n3, SM_CHS) * n2);
+    nblk_max = std::max<long>(nblk_max, (long)ceil_div(n1, F_TI1) * ceil_div(n2, F_TI2) * n3);
     W.nblk_max = (int)nblk_max;
     SH_TRY(sh_slot(ctx, SLOT_RED, (size_t)nblk_max * 3 * sizeof(double), (void **)&W.part));
     SH_TRY(sh_slot(ctx, SLOT_SCAL, sizeof(DipScal), (void **)&W.sc));

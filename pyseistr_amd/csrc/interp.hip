@@ -560,7 +560,7 @@ static int sint_common(sh_ctx *ctx, const float *din, const float *dipi, const f
     SH_TRY(pws2_setup(ctx, Sh.pin, SLOT_INTERP0 + 4));
     if (three) {
         Sh.px = Pwsmooth2{n1, n3, n2, ns2, o2, eps * eps, dxX};
-        SH_TRY(pws2_setup(ctx, Sh.px, SLOT_INTERP0 + 6));
+        SH_TRY(pws2_setup(ctx, Sh.px, SLOT_INTERP0 + 7));
     }
     SH_HIP(ctx, hipMemcpyAsync(pT, dT, sizeof(float) * n, hipMemcpyDeviceToDevice, ctx->stream));
     SH_TRY(sint_cg(ctx, Sh, known, pT, xT, dT, lam, (float)(10 * 1.19209290E-07F), niter, iters));

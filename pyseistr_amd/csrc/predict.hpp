@@ -329,3 +329,257 @@ SH_HD void predict_adj(const Acc &in, const Acc &q, bool rev, Sink scr, long pla
     pred_adj_apply<NW>(s, q, rev, n1, e, out);
 }
 
+
+// ---------------------------------------------------------------------------
+// Factored form.  The system matrix of a prediction depends only on the
+// slope traces and the filter directions, so the LDL^T factor is computed
+// once (factor_sweep) and shared by every prediction that uses the same
+// matrix; solve_factored then builds the right-hand side, substitutes
+// forward and backward with the stored factor.  The arithmetic is the same
+// as the fused path, operation for operation.
+// Factor layout: F plane 0 = d[k], plane 1+m = o[m][k] (stride fs).
+// ---------------------------------------------------------------------------
+template <int NW, int NF>
+SH_HD void factor_sweep(const Acc *q, const bool *rev, int n1, float e, Sink F, long fplane) {
+    constexpr int NA = 2 * NW + 1, NB = 2 * NW;
+    float A[NF][NA][NA];
+    float Ld[NB], Lo[NB][NB];
+#pragma unroll
+    for (int f = 0; f < NF; f++)
+#pragma unroll
+        for (int w = 0; w < NA; w++) taps_at<NW>(q[f], w - NW, n1, rev[f], A[f][w]);
+#pragma unroll
+    for (int r = 0; r < NB; r++) {
+        Ld[r] = 0.0f;
+#pragma unroll
+        for (int m = 0; m < NB; m++) Lo[r][m] = 0.0f;
+    }
+    for (int k = 0; k < n1; k++) {
+        float diag, offd[NB];
+        reg_row<NB>(k, n1, e, diag, offd);
+#pragma unroll
+        for (int f = 0; f < NF; f++) {
+#pragma unroll
+            for (int j = 0; j < NA; j++) {
+                int kk = k + j - NW;
+                if (kk >= NW && kk < n1 - NW) {
+                    float aj = A[f][j][j];
+                    diag += aj * aj;
+                }
+            }
+#pragma unroll
+            for (int m = 0; m < NB; m++) {
+#pragma unroll
+                for (int j = m + 1; j < NA; j++) {
+                    int kk = k + j - NW;
+                    if (kk >= NW && kk < n1 - NW) {
+                        float aj = A[f][j][j], am = A[f][j][j - m - 1];
+                        offd[m] += am * aj;
+                    }
+                }
+            }
+        }
+        float t = diag;
+        int m1 = k < NB ? k : NB;
+#pragma unroll
+        for (int m = 0; m < NB; m++)
+            if (m < m1) t -= (Lo[m][m]) * (Lo[m][m]) * (Ld[m]);
+        float dk = t;
+        float ok[NB];
+        int nn = (n1 - k - 1) < NB ? (n1 - k - 1) : NB;
+#pragma unroll
+        for (int nidx = 0; nidx < NB; nidx++) {
+            ok[nidx] = 0.0f;
+            if (nidx < nn) {
+                float s = offd[nidx];
+                int mm = k < (NB - nidx - 1) ? k : (NB - nidx - 1);
+#pragma unroll
+                for (int m = 0; m < NB; m++)
+                    if (m < mm) s -= (Lo[m][m]) * (Lo[m][nidx + m + 1]) * (Ld[m]);
+                ok[nidx] = s / dk;
+            }
+        }
+        F.p[(long)k * F.s] = dk;
+#pragma unroll
+        for (int m = 0; m < NB; m++) F.p[(long)(1 + m) * fplane + (long)k * F.s] = ok[m];
+#pragma unroll
+        for (int r = NB - 1; r > 0; r--) {
+            Ld[r] = Ld[r - 1];
+#pragma unroll
+            for (int m = 0; m < NB; m++) Lo[r][m] = Lo[r - 1][m];
+        }
+        Ld[0] = dk;
+#pragma unroll
+        for (int m = 0; m < NB; m++) Lo[0][m] = ok[m];
+#pragma unroll
+        for (int f = 0; f < NF; f++) {
+#pragma unroll
+            for (int w = 0; w < NA - 1; w++)
+#pragma unroll
+                for (int j = 0; j < NA; j++) A[f][w][j] = A[f][w + 1][j];
+            taps_at<NW>(q[f], k + 1 + NW, n1, rev[f], A[f][NA - 1]);
+        }
+    }
+}
+
+// Forward prediction with a precomputed factor F; y is per-thread scratch.
+template <int NW, int NF>
+SH_HD void solve_factored(const Acc *in, const Acc *q, const bool *rev, const float *F, long fs, long fplane,
+                          Sink y, int n1, float e, Sink out) {
+    constexpr int NA = 2 * NW + 1, NB = 2 * NW;
+    float tb[4];
+    if (NF == 1) {
+        tb[0] = in[0][0]; tb[1] = in[0][1]; tb[2] = in[0][n1 - 2]; tb[3] = in[0][n1 - 1];
+    } else {
+        tb[0] = (float)(0.5 * (in[0][0] + in[1][0]));
+        tb[1] = (float)(0.5 * (in[0][1] + in[1][1]));
+        tb[2] = (float)(0.5 * (in[0][n1 - 2] + in[1][n1 - 2]));
+        tb[3] = (float)(0.5 * (in[0][n1 - 1] + in[1][n1 - 1]));
+    }
+    float A[NF][NA][NA], I[NF][NA], T[NF][NA];
+    float Lo[NB][NB], Bw[NB];
+#pragma unroll
+    for (int f = 0; f < NF; f++) {
+#pragma unroll
+        for (int w = 0; w < NA; w++) taps_at<NW>(q[f], w - NW, n1, rev[f], A[f][w]);
+#pragma unroll
+        for (int w = 0; w < NA; w++) I[f][w] = (w < n1) ? in[f][w] : 0.0f;
+#pragma unroll
+        for (int w = 0; w < NA; w++) {
+            int kk = w - NW;
+            float t = 0.0f;
+            if (kk >= NW && kk < n1 - NW) {
+#pragma unroll
+                for (int j = 0; j < NA; j++) t += A[f][w][j] * ((kk + j - NW) < n1 ? in[f][kk + j - NW] : 0.0f);
+            }
+            T[f][w] = t;
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < NB; r++) {
+        Bw[r] = 0.0f;
+#pragma unroll
+        for (int m = 0; m < NB; m++) Lo[r][m] = 0.0f;
+    }
+    for (int k = 0; k < n1; k++) {
+        float rhs = 0.0f;
+#pragma unroll
+        for (int f = 0; f < NF; f++) {
+            float of = 0.0f;
+#pragma unroll
+            for (int j = 0; j < NA; j++) {
+                int kk = k + j - NW;
+                if (kk >= NW && kk < n1 - NW) of += A[f][j][j] * T[f][j];
+            }
+            rhs = (f == 0) ? of : (rhs + of);
+        }
+        if (k == 0) rhs += e * tb[0];
+        if (k == 1) rhs += e * tb[1];
+        if (k == n1 - 2) rhs += e * tb[2];
+        if (k == n1 - 1) rhs += e * tb[3];
+        int m1 = k < NB ? k : NB;
+        float b = rhs;
+#pragma unroll
+        for (int m = 0; m < NB; m++)
+            if (m < m1) b -= (Lo[m][m]) * Bw[m];
+        float dk = F[(long)k * fs];
+        y[k] = b / dk;
+        float ok[NB];
+#pragma unroll
+        for (int m = 0; m < NB; m++) ok[m] = F[(long)(1 + m) * fplane + (long)k * fs];
+#pragma unroll
+        for (int r = NB - 1; r > 0; r--) {
+            Bw[r] = Bw[r - 1];
+#pragma unroll
+            for (int m = 0; m < NB; m++) Lo[r][m] = Lo[r - 1][m];
+        }
+        Bw[0] = b;
+#pragma unroll
+        for (int m = 0; m < NB; m++) Lo[0][m] = ok[m];
+#pragma unroll
+        for (int f = 0; f < NF; f++) {
+#pragma unroll
+            for (int w = 0; w < NA - 1; w++) {
+#pragma unroll
+                for (int j = 0; j < NA; j++) A[f][w][j] = A[f][w + 1][j];
+                I[f][w] = I[f][w + 1];
+                T[f][w] = T[f][w + 1];
+            }
+            taps_at<NW>(q[f], k + 1 + NW, n1, rev[f], A[f][NA - 1]);
+            int kn = k + 1 + 2 * NW;
+            I[f][NA - 1] = (kn < n1) ? in[f][kn] : 0.0f;
+            int kk = k + 1 + NW;
+            float tn = 0.0f;
+            if (kk >= NW && kk < n1 - NW) {
+#pragma unroll
+                for (int j = 0; j < NA; j++) tn += A[f][NA - 1][j] * I[f][j];
+            }
+            T[f][NA - 1] = tn;
+        }
+    }
+    float nx[NB];
+#pragma unroll
+    for (int m = 0; m < NB; m++) nx[m] = 0.0f;
+    for (int k = n1 - 1; k >= 0; k--) {
+        float t = y[k];
+        int m1 = (n1 - k - 1) < NB ? (n1 - k - 1) : NB;
+#pragma unroll
+        for (int m = 0; m < NB; m++)
+            if (m < m1) t -= F[(long)(1 + m) * fplane + (long)k * fs] * nx[m];
+#pragma unroll
+        for (int m = NB - 1; m > 0; m--) nx[m] = nx[m - 1];
+        nx[0] = t;
+        out[k] = t;
+    }
+}
+
+// Adjoint prediction with a precomputed factor: s = S^{-1} in (forward and
+// backward substitution), then out = A^T A s + e * boundary(s)
+// (predict_step with adj, sof_cfuns.c:622-649).  y holds s.
+template <int NW>
+SH_HD void predict_adj_factored(const Acc &in, const Acc &q, bool rev, const float *F, long fs, long fplane,
+                                Sink y, int n1, float e, Sink out) {
+    constexpr int NB = 2 * NW;
+    float Lo[NB][NB], Bw[NB];
+#pragma unroll
+    for (int r = 0; r < NB; r++) {
+        Bw[r] = 0.0f;
+#pragma unroll
+        for (int m = 0; m < NB; m++) Lo[r][m] = 0.0f;
+    }
+    for (int k = 0; k < n1; k++) {
+        int m1 = k < NB ? k : NB;
+        float b = in[k];
+#pragma unroll
+        for (int m = 0; m < NB; m++)
+            if (m < m1) b -= (Lo[m][m]) * Bw[m];
+        y[k] = b / F[(long)k * fs];
+        float ok[NB];
+#pragma unroll
+        for (int m = 0; m < NB; m++) ok[m] = F[(long)(1 + m) * fplane + (long)k * fs];
+#pragma unroll
+        for (int r = NB - 1; r > 0; r--) {
+            Bw[r] = Bw[r - 1];
+#pragma unroll
+            for (int m = 0; m < NB; m++) Lo[r][m] = Lo[r - 1][m];
+        }
+        Bw[0] = b;
+#pragma unroll
+        for (int m = 0; m < NB; m++) Lo[0][m] = ok[m];
+    }
+    float nx[NB];
+#pragma unroll
+    for (int m = 0; m < NB; m++) nx[m] = 0.0f;
+    for (int k = n1 - 1; k >= 0; k--) {
+        float t = y[k];
+        int m1 = (n1 - k - 1) < NB ? (n1 - k - 1) : NB;
+#pragma unroll
+        for (int m = 0; m < NB; m++)
+            if (m < m1) t -= F[(long)(1 + m) * fplane + (long)k * fs] * nx[m];
+#pragma unroll
+        for (int m = NB - 1; m > 0; m--) nx[m] = nx[m - 1];
+        nx[0] = t;
+        y[k] = t;
+    }
+    pred_adj_apply<NW>(Acc{y.p, y.s}, q, rev, n1, e, out);
+}

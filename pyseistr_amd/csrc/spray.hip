@@ -96,12 +96,54 @@ struct Level3 {
     Node3 nd[MAX_LEVEL_NODES];
 };
 
+// Factor variants per target trace j (the matrix of a prediction depends
+// only on the target, the axes used and the directions toward the source):
+// v = 0..3: both axes, up2 = v & 1, up3 = v >> 1; v = 4, 5: axis 2 only,
+// up2 = v - 4; v = 6, 7: axis 3 only, up3 = v - 6.
+__host__ __device__ inline int node_variant(int update, int up2, int up3) {
+    return update == 3 ? (up2 | (up3 << 1)) : (update == 1 ? 4 + up2 : 6 + up3);
+}
+
 template <int NW>
 __global__ void __launch_bounds__(TB)
+factor3_kernel(const float *__restrict__ pi, const float *__restrict__ px, float *__restrict__ F, int n1, int n2,
+               int n3, float e) {
+    long ntr = (long)n2 * n3, n = (long)n1 * ntr;
+    long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= ntr) return;
+    int v = blockIdx.y;
+    int j2 = (int)(j % n2), j3 = (int)(j / n2);
+    int use2 = (v < 4 || v == 4 || v == 5), use3 = (v < 4 || v >= 6);
+    int up2 = v < 4 ? (v & 1) : (v - 4), up3 = v < 4 ? (v >> 1) : (v - 6);
+    Acc q[2];
+    bool rev[2];
+    int nf = 0;
+    if (use2) {
+        if (up2 ? j2 == 0 : j2 == n2 - 1) return;  // never used at this edge
+        q[nf] = Acc{pi + (up2 ? j - 1 : j), ntr};
+        rev[nf] = up2 != 0;
+        nf++;
+    }
+    if (use3) {
+        if (up3 ? j3 == 0 : j3 == n3 - 1) return;
+        q[nf] = Acc{px + (up3 ? j - n2 : j), ntr};
+        rev[nf] = up3 != 0;
+        nf++;
+    }
+    constexpr int NB = 2 * NW;
+    Sink Fs{F + (long)v * (NB + 1) * n + j, ntr};
+    if (nf == 1) factor_sweep<NW, 1>(q, rev, n1, e, Fs, n);
+    else factor_sweep<NW, 2>(q, rev, n1, e, Fs, n);
+}
+
+// One launch per (level, number of source traces NF): every thread predicts
+// one slot of one source trace with the shared factor of its target.
+template <int NW, int NF>
+__global__ void __launch_bounds__(TB)
 spray3_level(const float *__restrict__ pi, const float *__restrict__ px, float *__restrict__ U,
-             const Level3 nodes, int n1, int n2, int n3, int np2, float e, float *__restrict__ scr,
-             long nthr) {
-    long ntr = (long)n2 * n3;
+             const float *__restrict__ F, const Level3 nodes, int n1, int n2, int n3, int np2, float e,
+             float *__restrict__ scr, long nthr) {
+    long ntr = (long)n2 * n3, n = (long)n1 * ntr;
     long src = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (src >= ntr) return;
     Node3 nd = nodes.nd[blockIdx.y];
@@ -109,20 +151,16 @@ spray3_level(const float *__restrict__ pi, const float *__restrict__ px, float *
     int j2 = i2 + nd.o2, j3 = i3 + nd.o3;
     if (j2 < 0 || j2 >= n2 || j3 < 0 || j3 >= n3) return;
     long j = j2 + (long)j3 * n2;
-    long plane_u = (long)n1 * ntr;  // one slot
-    Sink out{U + (long)nd.slot * plane_u + j, ntr};
+    Sink out{U + (long)nd.slot * n + j, ntr};
     long tid = (long)blockIdx.y * gridDim.x * blockDim.x + src;
-    constexpr int NB = 2 * NW;
-    Sink scr_s{scr + tid, nthr};
-    long plane = (long)n1 * nthr;
-    (void)NB;
+    Sink y{scr + tid, nthr};
     Acc in[2], q[2];
     bool rev[2];
     int nf = 0;
     if (nd.update & 1) {
         long jj = nd.up2 ? j - 1 : j + 1;
         int ks = nd.up2 ? nd.slot - 1 : nd.slot + 1;
-        in[nf] = Acc{U + (long)ks * plane_u + jj, ntr};
+        in[nf] = Acc{U + (long)ks * n + jj, ntr};
         q[nf] = Acc{pi + (nd.up2 ? j - 1 : j), ntr};
         rev[nf] = nd.up2 != 0;
         nf++;
@@ -130,13 +168,14 @@ spray3_level(const float *__restrict__ pi, const float *__restrict__ px, float *
     if (nd.update & 2) {
         long jj = nd.up3 ? j - n2 : j + n2;
         int ks = nd.up3 ? nd.slot - np2 : nd.slot + np2;
-        in[nf] = Acc{U + (long)ks * plane_u + jj, ntr};
+        in[nf] = Acc{U + (long)ks * n + jj, ntr};
         q[nf] = Acc{px + (nd.up3 ? j - n2 : j), ntr};
         rev[nf] = nd.up3 != 0;
         nf++;
     }
-    if (nf == 1) predict<NW, 1>(in, q, rev, scr_s, plane, n1, e, out);
-    else if (nf == 2) predict<NW, 2>(in, q, rev, scr_s, plane, n1, e, out);
+    constexpr int NB = 2 * NW;
+    const float *Fv = F + (long)node_variant(nd.update, nd.up2, nd.up3) * (NB + 1) * n + j;
+    solve_factored<NW, NF>(in, q, rev, Fv, ntr, n, y, n1, e, out);
 }
 
 // centre slot = data (trace-fast copy)
@@ -202,25 +241,44 @@ int launch_median(sh_ctx *ctx, const float *U, float *out, long n, int first, in
 // 2-D spray chains.  Slice layout trace-fast per slice: [i3][k][i2].
 // Thread = (source i2, direction, slice i3).  Forward: the chain walks
 // predict_step(forw) to the ns neighbours (pwspray_lop, sof_cfuns.c:998-1028).
-// U2 layout: [slot][i3][k][i2].
+// U2 layout: [slot][i3][k][i2].  The system matrix of a step depends only on
+// the target trace and the direction, so both factors of every trace are
+// built once (factor2_kernel) and shared by the ns chains through it:
+// v = 0: forw = false with the target's slope; v = 1: forw = true with the
+// slope of the trace before it.  F2 layout [v][NB+1][i3][k][i2].
 // ---------------------------------------------------------------------------
 template <int NW>
 __global__ void __launch_bounds__(TB)
-spray2_fwd(const float *__restrict__ x, const float *__restrict__ dip, float *__restrict__ U, int n1, int n2, int n3,
-           int ns, float e, float *__restrict__ scr, long nthr) {
+factor2_kernel(const float *__restrict__ dip, float *__restrict__ F, int n1, int n2, int n3, float e) {
+    long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    int i3 = blockIdx.y;
+    if (gid >= 2L * n2) return;
+    int v = (int)(gid / n2), i = (int)(gid % n2);
+    if (v == 0 ? i > n2 - 2 : i < 1) return;  // never a target in that direction
+    long slice = (long)n1 * n2, n = slice * n3;
+    constexpr int NB = 2 * NW;
+    Acc q[1] = {Acc{dip + i3 * slice + (v == 0 ? i : i - 1), n2}};
+    bool rev[1] = {v == 1};
+    Sink Fs{F + (long)v * (NB + 1) * n + i3 * slice + i, n2};
+    factor_sweep<NW, 1>(q, rev, n1, e, Fs, n);
+}
+
+template <int NW>
+__global__ void __launch_bounds__(TB)
+spray2_fwd(const float *__restrict__ x, const float *__restrict__ dip, const float *__restrict__ F,
+           float *__restrict__ U, int n1, int n2, int n3, int ns, float e, float *__restrict__ scr, long nthr) {
     long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;  // over n2 * 2
     int i3 = blockIdx.y;
     if (gid >= 2L * n2) return;
     int dir = (int)(gid / n2);  // 0: toward lower i2 (forw=false), 1: toward higher (forw=true)
     int i = (int)(gid % n2);
-    int ns2 = 2 * ns + 1;
     long slice = (long)n1 * n2;
     const float *xs = x + i3 * slice;
     const float *ds = dip + i3 * slice;
-    long plane_slot = slice * n3;
+    long plane_slot = slice * n3, n = plane_slot;
+    constexpr int NB = 2 * NW;
     long tid = (long)i3 * 2 * n2 + gid;
-    Sink scr_s{scr + tid, nthr};
-    long plane = (long)n1 * nthr;
+    Sink y{scr + tid, nthr};
     if (dir == 0) {  // centre slot written once
         Sink c{U + (long)ns * plane_slot + i3 * slice + i, n2};
         for (int k = 0; k < n1; k++) c[k] = xs[(long)k * n2 + i];
@@ -235,10 +293,10 @@ spray2_fwd(const float *__restrict__ x, const float *__restrict__ dip, float *__
         Acc in[1] = {cur};
         Acc qs[1] = {q};
         bool rv[1] = {dir == 1};
-        predict<NW, 1>(in, qs, rv, scr_s, plane, n1, e, out);
+        const float *Fv = F + (long)dir * (NB + 1) * n + i3 * slice + ip;
+        solve_factored<NW, 1>(in, qs, rv, Fv, n2, n, y, n1, e, out);
         cur = Acc{out.p, n2};
     }
-    (void)ns2;
 }
 
 // Adjoint chains (pwspray_lop adj, sof_cfuns.c:958-996): for output trace i,
@@ -247,19 +305,20 @@ spray2_fwd(const float *__restrict__ x, const float *__restrict__ dip, float *__
 // C[dir][i3][k][i]; the centre slot is added by spray2_adj_combine.
 template <int NW>
 __global__ void __launch_bounds__(TB)
-spray2_adj(const float *__restrict__ U, const float *__restrict__ dip, float *__restrict__ C, int n1, int n2, int n3,
-           int ns, float e, float *__restrict__ scr, float *__restrict__ tr, long nthr) {
+spray2_adj(const float *__restrict__ U, const float *__restrict__ dip, const float *__restrict__ F,
+           float *__restrict__ C, int n1, int n2, int n3, int ns, float e, float *__restrict__ scr,
+           float *__restrict__ tr, long nthr) {
     long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
     int i3 = blockIdx.y;
     if (gid >= 2L * n2) return;
     int dir = (int)(gid / n2);
     int i = (int)(gid % n2);
     long slice = (long)n1 * n2;
-    long plane_slot = slice * n3;
+    long plane_slot = slice * n3, n = plane_slot;
     const float *ds = dip + i3 * slice;
+    constexpr int NB = 2 * NW;
     long tid = (long)i3 * 2 * n2 + gid;
-    Sink scr_s{scr + tid, nthr};
-    long plane = (long)n1 * nthr;
+    Sink y{scr + tid, nthr};
     Sink t{tr + tid, nthr};   // running trace
     for (int k = 0; k < n1; k++) t[k] = 0.0f;
     for (int is = ns - 1; is >= 0; is--) {
@@ -269,7 +328,8 @@ spray2_adj(const float *__restrict__ U, const float *__restrict__ dip, float *__
         Acc u{U + (long)slot * plane_slot + i3 * slice + ip, n2};
         for (int k = 0; k < n1; k++) t[k] = t[k] + u[k];
         Acc q{ds + (dir == 1 ? ip - 1 : ip), n2};
-        predict_adj<NW>(Acc{t.p, t.s}, q, dir == 1, scr_s, plane, n1, e, t);
+        const float *Fv = F + (long)(dir == 1 ? 1 : 0) * (NB + 1) * n + i3 * slice + ip;
+        predict_adj_factored<NW>(Acc{t.p, t.s}, q, dir == 1, Fv, n2, n, y, n1, e, t);
     }
     Sink c{C + (long)dir * plane_slot + i3 * slice + i, n2};
     for (int k = 0; k < n1; k++) c[k] = t[k];
@@ -379,21 +439,43 @@ int spray3_run(sh_ctx *ctx, int order, int n1, int n2, int n3, int ns2, int ns3,
     int gx = ceil_div(ntr, TB);
     long nthr = (long)gx * TB * (long)maxn;
     int NB1 = 2 * order + 1;
-    float *scr = nullptr;
-    SH_TRY(sh_slot(ctx, SLOT_SPRAY0 + 4, (size_t)nthr * n1 * NB1 * sizeof(float), (void **)&scr));
-    for (size_t L = 1; L < levels.size(); L++) {
-        if (levels[L].empty()) continue;
-        dim3 grid(gx, (unsigned)levels[L].size());
-        Level3 lv{};
-        for (size_t q = 0; q < levels[L].size(); q++) lv.nd[q] = levels[L][q];
-        double by = 0;
-        for (auto &nd : levels[L]) by += (nd.update == 3 ? 20.0 : 12.0) * (double)n;  // per slot-sample
+    float *scr = nullptr, *F = nullptr;
+    SH_TRY(sh_slot(ctx, SLOT_SPRAY0 + 4, (size_t)nthr * n1 * sizeof(float), (void **)&scr));
+    SH_TRY(sh_slot(ctx, SLOT_SPRAY0 + 8, (size_t)8 * NB1 * n * sizeof(float), (void **)&F));
+    {
         PROF_A(ctx);
         SH_DISPATCH_NW(ctx, order, NW,
-                       hipLaunchKernelGGL((spray3_level<NW>), grid, dim3(TB), 0, ctx->stream, piT, pxT, U,
-                                          lv, n1, n2, n3, np2, e, scr, nthr));
-        PROF_B(ctx, "spray3_level", by);
-        SH_TRY(sh_launch_check(ctx, "spray3_level"));
+                       hipLaunchKernelGGL((factor3_kernel<NW>), dim3(gx, 8), dim3(TB), 0, ctx->stream, piT, pxT, F,
+                                          n1, n2, n3, e));
+        PROF_B(ctx, "factor3_kernel", 2.0 * 4.0 * (double)n);
+        SH_TRY(sh_launch_check(ctx, "factor3_kernel"));
+    }
+    for (size_t L = 1; L < levels.size(); L++) {
+        for (int nf = 1; nf <= 2; nf++) {
+            Level3 lv{};
+            int cnt = 0;
+            double by = 0;
+            for (auto &nd : levels[L])
+                if ((nd.update == 3 ? 2 : 1) == nf) {
+                    lv.nd[cnt++] = nd;
+                    by += (nf == 2 ? 20.0 : 12.0) * (double)n;  // per slot-sample: inputs, slopes, output
+                }
+            if (!cnt) continue;
+            dim3 grid(gx, cnt);
+            PROF_A(ctx);
+            if (nf == 1) {
+                SH_DISPATCH_NW(ctx, order, NW,
+                               hipLaunchKernelGGL((spray3_level<NW, 1>), grid, dim3(TB), 0, ctx->stream, piT, pxT, U,
+                                                  F, lv, n1, n2, n3, np2, e, scr, nthr));
+                PROF_B(ctx, "spray3_level<1>", by);
+            } else {
+                SH_DISPATCH_NW(ctx, order, NW,
+                               hipLaunchKernelGGL((spray3_level<NW, 2>), grid, dim3(TB), 0, ctx->stream, piT, pxT, U,
+                                                  F, lv, n1, n2, n3, np2, e, scr, nthr));
+                PROF_B(ctx, "spray3_level<2>", by);
+            }
+            SH_TRY(sh_launch_check(ctx, "spray3_level"));
+        }
     }
     if (scr_io) *scr_io = scr;
     return SH_OK;
@@ -424,25 +506,42 @@ int stage3(sh_ctx *ctx, const float *din, const float *dipi, const float *dipx, 
 
 // Internal API (declared in common.hpp): 2-D structure-oriented smoothing.
 
+// Scratch (solver + running trace planes) and the factor cache of P.
+int pws2_prepare(sh_ctx *ctx, Pwsmooth2 &P, int slot_scr, int slot_F) {
+    long n = (long)P.n1 * P.n2 * P.n3;
+    int NB1 = 2 * P.order + 1;
+    P.nthr = (long)ceil_div(2L * P.n2, TB) * TB * P.n3;
+    SH_TRY(sh_slot(ctx, slot_scr, (size_t)P.nthr * P.n1 * 2 * sizeof(float), (void **)&P.scr));
+    P.tr = P.scr + (long)P.nthr * P.n1;
+    SH_TRY(sh_slot(ctx, slot_F, (size_t)2 * NB1 * n * sizeof(float), (void **)&P.F));
+    dim3 grid(ceil_div(2L * P.n2, TB), P.n3);
+    PROF_A(ctx);
+    SH_DISPATCH_NW(ctx, P.order, NW,
+                   hipLaunchKernelGGL((factor2_kernel<NW>), grid, dim3(TB), 0, ctx->stream, P.dip, P.F, P.n1, P.n2,
+                                      P.n3, P.e));
+    PROF_B(ctx, "factor2_kernel", 4.0 * (double)n);
+    return sh_launch_check(ctx, "factor2_kernel");
+}
+
 int pws2_spray(sh_ctx *ctx, Pwsmooth2 &P, const float *x) {
     dim3 grid(ceil_div(2L * P.n2, TB), P.n3);
+    PROF_A(ctx);
     SH_DISPATCH_NW(ctx, P.order, NW,
-                   hipLaunchKernelGGL((spray2_fwd<NW>), grid, dim3(TB), 0, ctx->stream, x, P.dip, P.U, P.n1, P.n2,
-                                      P.n3, P.ns, P.e, P.scr, P.nthr));
+                   hipLaunchKernelGGL((spray2_fwd<NW>), grid, dim3(TB), 0, ctx->stream, x, P.dip, P.F, P.U, P.n1,
+                                      P.n2, P.n3, P.ns, P.e, P.scr, P.nthr));
+    PROF_B(ctx, "spray2_fwd", 12.0 * 2 * P.ns * (double)P.n1 * P.n2 * P.n3);
     return sh_launch_check(ctx, "spray2_fwd");
 }
 
 int pws2_setup(sh_ctx *ctx, Pwsmooth2 &P, int slot_base) {
     long n = (long)P.n1 * P.n2 * P.n3;
     int ns2 = 2 * P.ns + 1;
-    P.nthr = (long)ceil_div(2L * P.n2, TB) * TB * P.n3;
     float *b = nullptr;
     SH_TRY(sh_slot(ctx, slot_base, (size_t)n * (ns2 + 3) * sizeof(float), (void **)&b));
     P.U = b;
     P.ww = b + (long)ns2 * n;
     P.C = b + (long)(ns2 + 1) * n;
-    SH_TRY(sh_slot(ctx, slot_base + 1, (size_t)P.nthr * P.n1 * (2 * P.order + 2) * sizeof(float), (void **)&P.scr));
-    P.tr = P.scr + (long)P.nthr * P.n1 * (2 * P.order + 1);
+    SH_TRY(pws2_prepare(ctx, P, slot_base + 1, slot_base + 2));
     // normalisation: spray of ones (pwsmooth_set, sof_cfuns.c:1102-1122)
     float *ones = P.C;  // C is free at this point (2n floats)
     int g = ceil_div(n, TB);
@@ -469,8 +568,8 @@ int pws2_apply(sh_ctx *ctx, Pwsmooth2 &P, int adj, int add, float *trace, float 
     SH_TRY(sh_launch_check(ctx, "pwsmooth_adj_weight"));
     dim3 grid(ceil_div(2L * P.n2, TB), P.n3);
     SH_DISPATCH_NW(ctx, P.order, NW,
-                   hipLaunchKernelGGL((spray2_adj<NW>), grid, dim3(TB), 0, ctx->stream, P.U, P.dip, P.C, P.n1, P.n2,
-                                      P.n3, P.ns, P.e, P.scr, P.tr, P.nthr));
+                   hipLaunchKernelGGL((spray2_adj<NW>), grid, dim3(TB), 0, ctx->stream, P.U, P.dip, P.F, P.C, P.n1,
+                                      P.n2, P.n3, P.ns, P.e, P.scr, P.tr, P.nthr));
     SH_TRY(sh_launch_check(ctx, "spray2_adj"));
     hipLaunchKernelGGL(spray2_adj_combine, dim3(g), dim3(TB), 0, ctx->stream, P.C, P.U, trace, n, P.ns, add);
     return sh_launch_check(ctx, "spray2_adj_combine");
@@ -559,13 +658,11 @@ int sh_pwspray2d(sh_ctx *ctx, int adj, const float *dip, int n1, int n2, int ns,
     float *xT = b, *dT = b + n, *tmp = b + 2 * n;
     SH_TRY(transpose(ctx, dip, dT, n1, n2, true));
     Pwsmooth2 P{n1, n2, 1, ns, order, eps * eps, dT};
-    P.nthr = (long)ceil_div(2L * n2, TB) * TB;
     float *wb = nullptr;
     SH_TRY(sh_slot(ctx, SLOT_SPRAY0 + 2, (size_t)n * (ns2 + 2) * sizeof(float), (void **)&wb));
     P.U = wb;
     P.C = wb + (long)ns2 * n;
-    SH_TRY(sh_slot(ctx, SLOT_SPRAY0 + 4, (size_t)P.nthr * n1 * (2 * order + 2) * sizeof(float), (void **)&P.scr));
-    P.tr = P.scr + (long)P.nthr * n1 * (2 * order + 1);
+    SH_TRY(pws2_prepare(ctx, P, SLOT_SPRAY0 + 4, SLOT_SPRAY0 + 9));
     if (!adj) {
         SH_TRY(transpose(ctx, x, xT, n1, n2, true));
         hipLaunchKernelGGL(zero_kernel, dim3(ceil_div(n * ns2, TB)), dim3(TB), 0, ctx->stream, P.U, n * ns2);
@@ -586,8 +683,8 @@ int sh_pwspray2d(sh_ctx *ctx, int adj, const float *dip, int n1, int n2, int ns,
     }
     dim3 grid(ceil_div(2L * n2, TB), 1);
     SH_DISPATCH_NW(ctx, order, NW,
-                   hipLaunchKernelGGL((spray2_adj<NW>), grid, dim3(TB), 0, ctx->stream, P.U, P.dip, P.C, n1, n2, 1, ns,
-                                      P.e, P.scr, P.tr, P.nthr));
+                   hipLaunchKernelGGL((spray2_adj<NW>), grid, dim3(TB), 0, ctx->stream, P.U, P.dip, P.F, P.C, n1, n2, 1,
+                                      ns, P.e, P.scr, P.tr, P.nthr));
     SH_TRY(sh_launch_check(ctx, "spray2_adj"));
     hipLaunchKernelGGL(spray2_adj_combine, dim3(ceil_div(n, TB)), dim3(TB), 0, ctx->stream, P.C, P.U, xT, n, ns, 0);
     SH_TRY(sh_launch_check(ctx, "spray2_adj_combine"));
@@ -638,9 +735,8 @@ int sh_somf2d(sh_ctx *ctx, const float *dn, const float *dip, int n1, int n2, in
         SH_TRY(transpose(ctx, dn + (long)i3 * n1 * n2, xT + (long)i3 * n1 * n2, n1, n2, true));
     }
     Pwsmooth2 P{n1, n2, n3, ns, order, eps * eps, dT};
-    P.nthr = (long)ceil_div(2L * n2, TB) * TB * n3;
     SH_TRY(sh_slot(ctx, SLOT_SPRAY0 + 6, (size_t)n * np * sizeof(float), (void **)&P.U));
-    SH_TRY(sh_slot(ctx, SLOT_SPRAY0 + 7, (size_t)P.nthr * n1 * (2 * order + 1) * sizeof(float), (void **)&P.scr));
+    SH_TRY(pws2_prepare(ctx, P, SLOT_SPRAY0 + 7, SLOT_SPRAY0 + 9));
     hipLaunchKernelGGL(zero_kernel, dim3(ceil_div(n * np, TB)), dim3(TB), 0, ctx->stream, P.U, n * np);
     SH_TRY(pws2_spray(ctx, P, xT));
     int m = (nmf - 1) / 2;

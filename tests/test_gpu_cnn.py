@@ -1,0 +1,167 @@
+"""GPU parity of the warm-start network kernels against the CPU oracle.
+
+The oracle (oracle/cnn_oracle.py) rounds activations, weights and
+back-propagated gradients to bfloat16 at the same points as the kernels and
+sums in float64; the kernels sum in float32 on the matrix cores.  Bars
+(stated per test): forward relative L2 error <= 5e-3, per-layer gradient
+relative L2 error <= 2e-2, loss relative error <= 1e-3; Adam (FP32)
+relative error <= 1e-5 against the float64 formula.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import cnn_oracle as K
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(40, 12, 9), (70, 6, 5)]
+
+
+@pytest.fixture(scope="module")
+def ws():
+    import torch
+    from pyseistr_amd import _lib, warmstart
+    assert torch.cuda.is_available()
+    assert not _lib.missing_symbols()
+    return warmstart
+
+
+def _case(seed, shape):
+    rng = np.random.default_rng(seed)
+    p = K.init_params(seed)
+    p = (p + 0.02 * rng.standard_normal(p.size)).astype(np.float32)
+    n = int(np.prod(shape))
+    x = (rng.standard_normal(n) * 3.0).astype(np.float32)
+    t = (rng.standard_normal(2 * n) * 0.5).astype(np.float32)
+    wt = (rng.random(n) > 0.2).astype(np.float32)
+    return p, x, t, wt
+
+
+def _dev(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a, np.float32)).cuda()
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / max(np.linalg.norm(b), 1e-30))
+
+
+def _blocks():
+    out = {"W1": slice(K.OFF_W1, K.OFF_B1), "b1": slice(K.OFF_B1, K.OFF_B1 + K.C)}
+    for l in (2, 3, 4, 5):
+        o = K.OFF_L[l]
+        out[f"W{l}"] = slice(o, o + K.NT * K.C * K.C)
+        out[f"b{l}"] = slice(o + K.NT * K.C * K.C, o + K._L)
+    out["W6"] = slice(K.OFF_W6, K.OFF_B6)
+    out["b6"] = slice(K.OFF_B6, K.OFF_B6 + 2)
+    return out
+
+
+def test_nparams(ws):
+    from pyseistr_amd import _lib
+    assert _lib.load().sh_cnn_nparams() == K.NPARAM == ws.NPARAM
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_forward_matches_oracle(ws, shape):
+    p, x, _, _ = _case(1, shape)
+    xs = 0.37
+    got = ws.cnn_forward(_dev(p), _dev(x), shape, xs).cpu().numpy()
+    ref = K.predict(p, x, shape, xscale=xs, quant=True)
+    print("forward rel err", _rel(got, ref))
+    assert _rel(got, ref) <= 5e-3
+    exact = K.predict(p, x, shape, xscale=xs, quant=False)
+    print("bf16 network vs exact float64 network", _rel(got, exact))
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_gradient_matches_oracle(ws, shape):
+    p, x, t, wt = _case(2, shape)
+    xs = 0.5
+    loss, g = ws.cnn_grad(_dev(p), _dev(x), shape, xs, _dev(t), _dev(wt))
+    g = g.cpu().numpy()
+    lref, gref = K.loss_and_grad(p, x, t, shape, wt=wt, xscale=xs, quant=True)
+    assert abs(loss - lref) <= 1e-3 * abs(lref)
+    for name, sl in _blocks().items():
+        e = _rel(g[sl], gref[sl])
+        print(f"grad {name}: rel err {e:.2e}")
+        assert e <= 2e-2, name
+    # without weights (wt = NULL) the loss is the plain mean square
+    loss2, _ = ws.cnn_grad(_dev(p), _dev(x), shape, xs, _dev(t), None)
+    l2, _ = K.loss_and_grad(p, x, t, shape, wt=None, xscale=xs, quant=True)
+    assert abs(loss2 - l2) <= 1e-3 * abs(l2)
+
+
+def test_adam_matches_formula(ws):
+    rng = np.random.default_rng(3)
+    p = rng.standard_normal(K.NPARAM).astype(np.float32)
+    pd, md, vd = _dev(p), _dev(np.zeros_like(p)), _dev(np.zeros_like(p))
+    ph, mh, vh = p.astype(np.float64), np.zeros(K.NPARAM), np.zeros(K.NPARAM)
+    for step in range(1, 4):
+        g = rng.standard_normal(K.NPARAM).astype(np.float32)
+        ws.cnn_adam(pd, _dev(g), md, vd, 1e-3, step, wd=1e-4)
+        ph, mh, vh = K.adam(ph, g, mh, vh, 1e-3, 0.9, 0.999, 1e-8, step, wd=1e-4)
+    got = pd.cpu().numpy()
+    assert np.abs(got - ph).max() <= 1e-5 * max(1.0, np.abs(ph).max())
+
+
+def test_training_reduces_loss(ws):
+    """A few hundred Adam steps on one window fit its dip3d dips."""
+    import torch
+    from pyseistr_amd import dip, synth
+    shape = (64, 32, 32)
+    v, _ = synth.volume3d(shape, seed=21, n_events=3, smax=3.0, sigma=0.1, spike_frac=0.0)
+    d = torch.from_numpy(v.ravel(order="F")).cuda()
+    tgt = dip.dip_device(d, shape, niter=10, liter=10, order=2, rect=(5, 5, 5))
+    xs = ws.input_scale(d)
+    p = _dev(ws.init_params(0))
+    m, vv = torch.zeros_like(p), torch.zeros_like(p)
+    l0, g = ws.cnn_grad(p, d, shape, xs, tgt)
+    for s in range(1, 301):
+        ws.cnn_grad(p, d, shape, xs, tgt, grad=g, want_loss=False)
+        ws.cnn_adam(p, g, m, vv, 1e-3, s)
+    l1, _ = ws.cnn_grad(p, d, shape, xs, tgt)
+    print("loss", l0, "->", l1)
+    assert np.isfinite(l1) and l1 < 0.3 * l0
+
+
+def test_errors(ws):
+    import torch
+    from pyseistr_amd import _lib
+    ctx = _lib.context()
+    lib = _lib.load()
+    p = torch.zeros(K.NPARAM, device="cuda")
+    assert lib.sh_cnn_forward(ctx, None, _lib.ptr(p), 1.0, 4, 4, 4, _lib.ptr(p)) == -2
+    assert lib.sh_cnn_forward(ctx, _lib.ptr(p), _lib.ptr(p), 1.0, 0, 4, 4, _lib.ptr(p)) == -1
+    assert lib.sh_cnn_adam(ctx, _lib.ptr(p), _lib.ptr(p), _lib.ptr(p), _lib.ptr(p), 1e-3, 0.9, 0.999, 1e-8, 0,
+                           0.0) == -1
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(os.path.dirname(__file__), "..", "pyseistr_amd", "weights",
+                                                    "warmstart.npz")), reason="no trained weights committed")
+def test_warm_start_gain(ws):
+    """The trained network's warm start reaches a smaller dip error after 2
+    Gauss-Newton iterations than the cold start after 2 (and after 5), on a
+    held-out volume (seed 5, not in the training seeds)."""
+    import torch
+    from pyseistr_amd import dip, synth
+    shape = (256, 128, 128)
+    v, _ = synth.volume3d(shape, seed=5, n_events=4, smax=3.0, sigma=0.3, spike_frac=0.01)
+    d = torch.from_numpy(v.ravel(order="F")).cuda()
+    kw = dict(liter=10, order=2, rect=(5, 5, 5))
+    conv = dip.dip_device(d, shape, niter=30, **kw)
+    net = ws.WarmStart.load()
+
+    def rmse(a):
+        return float(torch.sqrt(torch.mean((a - conv) ** 2)).item())
+
+    init = net.predict_device(d, shape)
+    warm2 = net.dip_device(d, shape, niter=2, init=init, **kw)
+    cold2 = dip.dip_device(d, shape, niter=2, **kw)
+    cold5 = dip.dip_device(d, shape, niter=5, **kw)
+    r = {"init": rmse(init), "warm2": rmse(warm2), "cold2": rmse(cold2), "cold5": rmse(cold5)}
+    print(r)
+    assert r["warm2"] < 0.5 * r["cold2"]
+    assert r["warm2"] < r["cold5"]
