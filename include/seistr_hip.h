@@ -46,7 +46,7 @@ int sh_sync(sh_ctx *ctx);
 int sh_prof_enable(sh_ctx *ctx, int on);
 int sh_prof_report(sh_ctx *ctx, char *buf, int len);
 
-/* Per-direction iteration record of a dip call: for GN iteration k,
+/* Per-direction iteration record of a dip call: for GN iteration k < 64,
  * cg_iters[d][k] = shaping-CG iterations run (-1: zero denominator),
  * ls_step[d][k] = accepted line-search halving (8 = none accepted). */
 typedef struct {
@@ -136,6 +136,29 @@ int sh_sint3d(sh_ctx *ctx, const float *din, const float *dipi, const float *dip
  * soint2d_cfuns.c:2421-2538) behind sint2dc (pyseistr/sint.py:61). */
 int sh_sint2d(sh_ctx *ctx, const float *din, const float *dip, const float *mask,
               int n1, int n2, int niter, int ns, int order, float eps, float *out, int *iters);
+
+/* ---- Batched 2-D path (no counterpart in the reference) ---------------
+ * npatch patches stored back to back in one device array: patch p is an
+ * n1s[p] x n2s[p] F-order block at element offset offs[p] (offsets
+ * ascending, non-overlapping; the three tables are HOST arrays).  Each
+ * patch gets exactly the per-patch algorithm; every pipeline stage is one
+ * launch over the whole batch. */
+int sh_batch_tile_count(void);
+int sh_batch_tile_shape(int cfg, int *t1, int *t2);
+/* dip2dc per patch: dipc with n3 = 1 (pyseistr/src/dip_cfuns.c:1694-1989,
+ * behind dip2dc, pyseistr/dip2d.py:115).  tile: smoothing tile shape
+ * index (-1 = default; see sh_batch_tile_shape).  init (nullable, same
+ * layout as data): initial dips.  rec (nullable, host, npatch*niter*2):
+ * per patch and GN iteration the CG iteration count and accepted
+ * line-search halving, as sh_dip_report; filling it synchronises. */
+int sh_dip2d_batched(sh_ctx *ctx, const float *data, const long *offs, const int *n1s,
+                     const int *n2s, int npatch, int niter, int liter, int order, int r1,
+                     int r2, int tile, const float *init, float *dip, int *rec);
+/* somf2dc per patch: csomf2d (pyseistr/src/sof_cfuns.c:1534-1672, behind
+ * somf2dc, pyseistr/somf2d.py:60) with option 1 (n3 = 1 per patch). */
+int sh_somf2d_batched(sh_ctx *ctx, const float *data, const float *dip, const long *offs,
+                      const int *n1s, const int *n2s, int npatch, int ns, int nmf, int option,
+                      int order, float eps, float *out);
 
 /* ---- Learned warm start (no counterpart in the reference) -------------
  * A 3-D CNN (six 3x3x3 convolutions, 1->32->32->32->32->32->2 channels,

@@ -61,6 +61,12 @@ _SIGS = {
                           c_int, c_int, c_int, c_int, c_float, c_void_p, c_int_p]),
     "sh_sint2d": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_int,
                           c_float, c_void_p, c_int_p]),
+    "sh_batch_tile_count": (c_int, []),
+    "sh_batch_tile_shape": (c_int, [c_int, c_int_p, c_int_p]),
+    "sh_dip2d_batched": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_int,
+                                 c_int, c_int, c_int, c_void_p, c_void_p, c_int_p]),
+    "sh_somf2d_batched": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int,
+                                  c_int, c_int, c_float, c_void_p]),
     "sh_cnn_nparams": (ctypes.c_long, []),
     "sh_cnn_forward": (c_int, [c_void_p, c_void_p, c_void_p, c_float, c_int, c_int, c_int, c_void_p]),
     "sh_cnn_grad": (c_int, [c_void_p, c_void_p, c_void_p, c_float, c_void_p, c_void_p, c_int, c_int, c_int,

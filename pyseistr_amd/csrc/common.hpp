@@ -74,8 +74,9 @@ enum {
     SLOT_SCAL = 1,       // device scalars
     SLOT_DIP0 = 2,       // dip module: 2..19
     SLOT_SPRAY0 = 20,    // spray module: 20..39
-    SLOT_INTERP0 = 40,   // interpolation module: 40..59
-    SLOT_COUNT = 64
+    SLOT_INTERP0 = 40,   // interpolation module: 40..51, warm-start network: 52..59
+    SLOT_BATCH0 = 64,    // batched 2-D module: 64..79
+    SLOT_COUNT = 80
 };
 
 // ---------------------------------------------------------------------------
@@ -227,3 +228,5 @@ int pws2_setup(sh_ctx *ctx, Pwsmooth2 &P, int slot_base);
 int pws2_prepare(sh_ctx *ctx, Pwsmooth2 &P, int slot_scr, int slot_F);
 int pws2_apply(sh_ctx *ctx, Pwsmooth2 &P, int adj, int add, float *trace, float *smooth);
 int sh_transpose_internal(sh_ctx *ctx, const float *src, float *dst, int n1, long ntr, bool to_fast);
+// Median of slots [first, first+nmf) of each of n samples (slot-major U).
+int sh_median_internal(sh_ctx *ctx, const float *U, float *out, long n, int first, int nmf);

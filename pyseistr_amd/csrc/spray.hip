@@ -431,7 +431,8 @@ int spray3_run(sh_ctx *ctx, int order, int n1, int n2, int n3, int ns2, int ns3,
     for (auto &l : levels) maxn = std::max(maxn, l.size());
     if (maxn > (size_t)MAX_LEVEL_NODES) return sh_fail(ctx, SH_EUNSUP, "spray radius too large");
     int g = ceil_div(n, TB);
-    hipLaunchKernelGGL(zero_kernel, dim3(g), dim3(TB), 0, ctx->stream, U, n * np);
+    // slots whose target lies outside the volume stay zero (sof3d_cfuns.c:1478-1503)
+    hipLaunchKernelGGL(zero_kernel, dim3(ceil_div(n * np, TB)), dim3(TB), 0, ctx->stream, U, n * np);
     SH_TRY(sh_launch_check(ctx, "zero"));
     int ctr = ns3 * np2 + ns2;
     hipLaunchKernelGGL(copy_slot, dim3(g), dim3(TB), 0, ctx->stream, dT, U + (long)ctr * n, n);
@@ -577,6 +578,10 @@ int pws2_apply(sh_ctx *ctx, Pwsmooth2 &P, int adj, int add, float *trace, float 
 
 int sh_transpose_internal(sh_ctx *ctx, const float *src, float *dst, int n1, long ntr, bool to_fast) {
     return transpose(ctx, src, dst, n1, ntr, to_fast);
+}
+
+int sh_median_internal(sh_ctx *ctx, const float *U, float *out, long n, int first, int nmf) {
+    return launch_median(ctx, U, out, n, first, nmf);
 }
 
 // ---------------------------------------------------------------------------

@@ -154,36 +154,38 @@ def _gen(args):
     return training_volume(seed, shape)
 
 
-def make_dataset(seeds, vol_shape, win, per_vol, target_niter, liter=10, rect=(5, 5, 5), workers=8, log=print):
+def make_dataset(seeds, vol_shape, win, per_vol, target_niter, liter=10, rect=(5, 5, 5), pool=None, log=print):
     """Cut 64^3 windows (x, target dips) from seeded volumes; everything
-    stays on the device.  Returns list of (x[W], target[2W], xscale)."""
+    stays on the device.  Returns list of (x[W], target[2W], xscale).
+    pool: optional multiprocessing pool for the volume generator, created
+    before the process first touches the GPU."""
     import torch
-    from multiprocessing import Pool
     w1, w2, w3 = win
     wn = w1 * w2 * w3
     out = []
     rng = np.random.default_rng(12345)
     t0 = time.time()
-    with Pool(workers) as pool:
-        for k, v in enumerate(pool.imap(_gen, [(s, vol_shape) for s in seeds], chunksize=2)):
-            d = torch.from_numpy(v.ravel(order="F")).cuda()
-            n = d.numel()
-            dips = dip_device(d, vol_shape, niter=target_niter, liter=liter, order=2, rect=rect)
-            xs = input_scale(d)
-            vol = d.view(vol_shape[2], vol_shape[1], vol_shape[0])
-            di = dips[:n].view(vol_shape[2], vol_shape[1], vol_shape[0])
-            dx = dips[n:].view(vol_shape[2], vol_shape[1], vol_shape[0])
-            for _ in range(per_vol):
-                o1 = int(rng.integers(0, vol_shape[0] - w1 + 1))
-                o2 = int(rng.integers(0, vol_shape[1] - w2 + 1))
-                o3 = int(rng.integers(0, vol_shape[2] - w3 + 1))
-                sl = (slice(o3, o3 + w3), slice(o2, o2 + w2), slice(o1, o1 + w1))
-                x = vol[sl].contiguous().view(-1)
-                t = torch.cat([di[sl].contiguous().view(-1), dx[sl].contiguous().view(-1)])
-                assert x.numel() == wn
-                out.append((x, t, xs))
-            if (k + 1) % 25 == 0:
-                log(f"dataset: {k + 1}/{len(seeds)} volumes, {len(out)} windows, {time.time() - t0:.0f} s")
+    jobs = [(s, vol_shape) for s in seeds]
+    gen = pool.imap(_gen, jobs, chunksize=2) if pool is not None else map(_gen, jobs)
+    for k, v in enumerate(gen):
+        d = torch.from_numpy(v.ravel(order="F")).cuda()
+        n = d.numel()
+        dips = dip_device(d, vol_shape, niter=target_niter, liter=liter, order=2, rect=rect)
+        xs = input_scale(d)
+        vol = d.view(vol_shape[2], vol_shape[1], vol_shape[0])
+        di = dips[:n].view(vol_shape[2], vol_shape[1], vol_shape[0])
+        dx = dips[n:].view(vol_shape[2], vol_shape[1], vol_shape[0])
+        for _ in range(per_vol):
+            o1 = int(rng.integers(0, vol_shape[0] - w1 + 1))
+            o2 = int(rng.integers(0, vol_shape[1] - w2 + 1))
+            o3 = int(rng.integers(0, vol_shape[2] - w3 + 1))
+            sl = (slice(o3, o3 + w3), slice(o2, o2 + w2), slice(o1, o1 + w1))
+            x = vol[sl].contiguous().view(-1)
+            t = torch.cat([di[sl].contiguous().view(-1), dx[sl].contiguous().view(-1)])
+            assert x.numel() == wn
+            out.append((x, t, xs))
+        if (k + 1) % 25 == 0:
+            log(f"dataset: {k + 1}/{len(seeds)} volumes, {len(out)} windows, {time.time() - t0:.0f} s")
     return out
 
 
@@ -243,10 +245,13 @@ def main(argv=None):
     def log(msg):
         print(msg, flush=True)
 
+    from multiprocessing import Pool
+    pool = Pool(a.workers)  # forked before the GPU is touched
     vol_shape = tuple(int(s) for s in a.vol_shape.split(","))
     seeds = list(range(a.seed0, a.seed0 + a.volumes))
     t0 = time.time()
-    data = make_dataset(seeds, vol_shape, (64, 64, 64), a.per_vol, a.target_niter, workers=a.workers, log=log)
+    data = make_dataset(seeds, vol_shape, (64, 64, 64), a.per_vol, a.target_niter, pool=pool, log=log)
+    pool.close()
     log(f"dataset ready: {len(data)} windows in {time.time() - t0:.0f} s")
     params, hist = train(data, a.steps, lr=a.lr, log=log)
     meta = {"volumes": a.volumes, "vol_shape": vol_shape, "seeds": [a.seed0, a.seed0 + a.volumes - 1],
