@@ -122,60 +122,70 @@ conv32(const __bf16 *__restrict__ in, const __bf16 *__restrict__ Wg, const float
     __syncthreads();
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int ntx = (n1 + 63) / 64, nty = (n2 + 3) / 4;
-    long bid = blockIdx.x;
-    const int x0 = (int)(bid % ntx) * 64;
-    bid /= ntx;
-    const int y = (int)(bid % nty) * 4 + w;
-    const int z = (int)(bid / nty);
+    const long ntiles = (long)ntx * nty * n3;
     const long n12 = (long)n1 * n2;
-    f32x16 acc[2];
+    // K order of the two MFMAs of a tap: lane half hi holds input channels
+    // hi*16 + [0, 8) for the first and hi*16 + [8, 16) for the second, so
+    // each lane reads its 16 channels (32 bytes) of a voxel in one go.
+    const int co = lane & 31, hi = lane >> 5;
+    const float bco = (MODE == 2) ? 0.0f : bias[co];
+    // the weights stay in LDS while the block walks over tiles
+    for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        long bid = tile;
+        const int x0 = (int)(bid % ntx) * 64;
+        bid /= ntx;
+        const int y = (int)(bid % nty) * 4 + w;
+        const int z = (int)(bid / nty);
+        f32x16 acc[2];
 #pragma unroll
-    for (int m = 0; m < 2; m++)
+        for (int m = 0; m < 2; m++)
 #pragma unroll
-        for (int r = 0; r < 16; r++) acc[m][r] = 0.0f;
-    const int kq = (lane >> 5) * 8;  // channel offset of this lane inside a 16-channel half
-    const bool rowok = y < n2;
-    for (int t = 0; t < NT; t++) {
-        int d1 = t % 3 - 1, d2 = (t / 3) % 3 - 1, d3 = t / 9 - 1;
-        int j2 = y + d2, j3 = z + d3;
-        bool plane = rowok && j2 >= 0 && j2 < n2 && j3 >= 0 && j3 < n3;
-        long rowbase = (long)j2 * n1 + (long)j3 * n12;
-#pragma unroll
-        for (int h = 0; h < 2; h++) {
-            bf16x8 b = wl[(t * C + (lane & 31)) * (C / 8) + h * 2 + (lane >> 5)];
+            for (int r = 0; r < 16; r++) acc[m][r] = 0.0f;
+        const bool rowok = y < n2;
+        for (int t = 0; t < NT; t++) {
+            int d1 = t % 3 - 1, d2 = (t / 3) % 3 - 1, d3 = t / 9 - 1;
+            int j2 = y + d2, j3 = z + d3;
+            bool plane = rowok && j2 >= 0 && j2 < n2 && j3 >= 0 && j3 < n3;
+            long rowbase = (long)j2 * n1 + (long)j3 * n12;
+            const bf16x8 b0 = wl[((t * C + co) * C + hi * 16) / 8];
+            const bf16x8 b1 = wl[((t * C + co) * C + hi * 16 + 8) / 8];
 #pragma unroll
             for (int m = 0; m < 2; m++) {
                 int j1 = x0 + m * 32 + (lane & 31) + d1;
-                bf16x8 a;
+                bf16x8 a0, a1;
                 if (plane && j1 >= 0 && j1 < n1) {
-                    a = *(const bf16x8 *)(in + (rowbase + j1) * C + h * 16 + kq);
+                    const bf16x8 *src = (const bf16x8 *)(in + (rowbase + j1) * C + hi * 16);
+                    a0 = src[0];
+                    a1 = src[1];
                 } else {
 #pragma unroll
-                    for (int i = 0; i < 8; i++) a[i] = (__bf16)0.0f;
+                    for (int i = 0; i < 8; i++) {
+                        a0[i] = (__bf16)0.0f;
+                        a1[i] = (__bf16)0.0f;
+                    }
                 }
-                acc[m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc[m], 0, 0, 0);
+                acc[m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b0, acc[m], 0, 0, 0);
+                acc[m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b1, acc[m], 0, 0, 0);
             }
         }
-    }
-    if (!rowok) return;
-    const int co = lane & 31;
-    const float bco = (MODE == 2) ? 0.0f : bias[co];
+        if (!rowok) continue;
 #pragma unroll
-    for (int m = 0; m < 2; m++) {
+        for (int m = 0; m < 2; m++) {
 #pragma unroll
-        for (int r = 0; r < 16; r++) {
-            int row = (r / 4) * 8 + (lane >> 5) * 4 + (r % 4);
-            int i1 = x0 + m * 32 + row;
-            if (i1 >= n1) continue;
-            long v = (long)i1 + (long)y * n1 + (long)z * n12;
-            float s = acc[m][r] + bco;
-            if (MODE == 0) {
-                ((__bf16 *)outp)[v * C + co] = (__bf16)fmaxf(s, 0.0f);
-            } else if (MODE == 1) {
-                if (co < 2) ((float *)outp)[v * 2 + co] = s;
-            } else {
-                float gate = (float)aux[v * C + co] > 0.0f ? 1.0f : 0.0f;
-                ((__bf16 *)outp)[v * C + co] = (__bf16)(s * gate);
+            for (int r = 0; r < 16; r++) {
+                int row = (r / 4) * 8 + hi * 4 + (r % 4);
+                int i1 = x0 + m * 32 + row;
+                if (i1 >= n1) continue;
+                long v = (long)i1 + (long)y * n1 + (long)z * n12;
+                float s = acc[m][r] + bco;
+                if (MODE == 0) {
+                    ((__bf16 *)outp)[v * C + co] = (__bf16)fmaxf(s, 0.0f);
+                } else if (MODE == 1) {
+                    if (co < 2) ((float *)outp)[v * 2 + co] = s;
+                } else {
+                    float gate = (float)aux[v * C + co] > 0.0f ? 1.0f : 0.0f;
+                    ((__bf16 *)outp)[v * C + co] = (__bf16)(s * gate);
+                }
             }
         }
     }
@@ -411,7 +421,9 @@ int cnn_work(sh_ctx *ctx, long n, bool train, CnnWork &W, int n1, int n2, int n3
 
 int conv_launch(sh_ctx *ctx, int mode, const __bf16 *in, const __bf16 *Wg, const float *bias, void *out,
                 const __bf16 *aux, int n1, int n2, int n3, const char *name) {
-    dim3 grid((unsigned)((long)ceil_div(n1, 64) * ceil_div(n2, 4) * n3));
+    // persistent blocks: the 55 KB weight tile is staged once per block
+    long ntiles = (long)ceil_div(n1, 64) * ceil_div(n2, 4) * n3;
+    dim3 grid((unsigned)std::min<long>(ntiles, 2048));
     PROF_A(ctx);
     if (mode == 0)
         hipLaunchKernelGGL((conv32<0>), grid, dim3(TB), 0, ctx->stream, in, Wg, bias, out, aux, n1, n2, n3);

@@ -54,16 +54,19 @@ pwd_kernel(const float *__restrict__ x, const float *__restrict__ p, const float
            int axis, double *part, const DipScal *sc) {
     if (MODE == 2 && sc->accepted) return;
     __shared__ double sh[16];
-    long n = (long)n1 * n2 * n3;
-    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long n = (long)n1 * n2 * n3;
+    const long ip = (axis == 2) ? n1 : (long)n1 * n2;
     double acc = 0.0;
-    if (i < n) {
+    // each block owns a contiguous range of samples (launch-cheap when the
+    // device flags make the launch a no-op)
+    const long chunk = (n + gridDim.x - 1) / gridDim.x;
+    const long i0 = (long)blockIdx.x * chunk, i1e = min(n, i0 + chunk);
+    for (long i = i0 + threadIdx.x; i < i1e; i += blockDim.x) {
         int i1 = (int)(i % n1);
         long r = i / n1;
         int i2 = (int)(r % n2), i3 = (int)(r / n2);
-        long ip = (axis == 2) ? n1 : (long)n1 * n2;
         bool valid = i1 >= NW && i1 < n1 - NW && (axis == 2 ? i2 < n2 - 1 : i3 < n3 - 1);
-        acc = pwd_sample<NW, MODE>(x, p, dp, lam, pmin, pmax, pout, u1, u2, mask && mask[i], i, ip, valid);
+        acc += pwd_sample<NW, MODE>(x, p, dp, lam, pmin, pmax, pout, u1, u2, mask && mask[i], i, ip, valid);
     }
     double t = block_sum(acc, sh);
     if (threadIdx.x == 0) part[blockIdx.x] = t;
@@ -76,9 +79,10 @@ cg_start_kernel(const float *__restrict__ num, const float *__restrict__ den, fl
                 float *__restrict__ r, float *__restrict__ x, float *__restrict__ pc, long n,
                 double *part, const DipScal *sc) {
     __shared__ double sh[16];
-    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     double acc = 0.0;
-    if (i < n) acc = cg_start_sample(num, den, w, r, x, pc, i, sc);
+    const long chunk = (n + gridDim.x - 1) / gridDim.x;
+    const long i0 = (long)blockIdx.x * chunk, i1e = min(n, i0 + chunk);
+    for (long i = i0 + threadIdx.x; i < i1e; i += blockDim.x) acc += cg_start_sample(num, den, w, r, x, pc, i, sc);
     double t = block_sum(acc, sh);
     if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
@@ -87,9 +91,8 @@ cg_start_kernel(const float *__restrict__ num, const float *__restrict__ den, fl
 __global__ void __launch_bounds__(256)
 cg_result_kernel(const float *__restrict__ xa, const float *__restrict__ xb, const float *__restrict__ sx,
                  float *__restrict__ dp, long n, const DipScal *sc) {
-    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    cg_result_sample(xa, xb, sx, dp, i, sc);
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        cg_result_sample(xa, xb, sx, dp, i, sc);
 }
 
 // ---------------------------------------------------------------------------
@@ -98,10 +101,11 @@ cg_result_kernel(const float *__restrict__ xa, const float *__restrict__ xb, con
 
 
 
-// Axis 1 (contiguous): block = one chunk of one trace.
+// Axis 1 (contiguous): task = one chunk of one trace; a block runs a
+// contiguous range of tasks (ntask = chunks * traces).
 template <int PRO, int EPI>
 __global__ void __launch_bounds__(256)
-smooth_contig(const float *__restrict__ in, float *__restrict__ out, int n1, int nb, CgArgs A,
+smooth_contig(const float *__restrict__ in, float *__restrict__ out, int n1, long ntask, int nb, CgArgs A,
               const DipScal *sc, double *part, int first) {
     if (sc->cg_done) return;
     __shared__ float tile[SM_CH1 + 2 * SM_HMAX];
@@ -109,72 +113,87 @@ smooth_contig(const float *__restrict__ in, float *__restrict__ out, int n1, int
     __shared__ double sh[16];
     int h = nb - 1;
     int nch = (n1 + SM_CH1 - 1) / SM_CH1;
-    long base = (long)(blockIdx.x / nch) * n1;
-    int c0 = (blockIdx.x % nch) * SM_CH1;
-    int len = min(SM_CH1, n1 - c0);
     bool pend = (PRO == PRO_CG || EPI == EPI_GP) ? (sc->pending != 0) : false;
     float a2 = pend ? (float)sc->alpha2 : 0.0f;
     float a1 = (EPI == EPI_S) ? (float)sc->alpha : 0.0f;
     int cur = (PRO == PRO_CG) ? sc->cur : 0;
     sm_weights(wts, nb);
-    for (int t = threadIdx.x; t < len + 2 * h; t += blockDim.x) {
-        int pos = c0 - h + t;
-        int q = fold_idx(pos, n1);
-        bool own = pos >= c0 && pos < c0 + len;
-        tile[t] = sm_load<PRO>(in, base + q, own, A, pend, a2, cur);
-    }
-    __syncthreads();
     double acc[3] = {0.0, 0.0, 0.0};
-    for (int t = threadIdx.x; t < len; t += blockDim.x) {
-        const float *c = tile + t + h;
-        float s = wts[0] * c[0];
-        for (int k = 1; k <= h; k++) s += wts[k] * (c[-k] + c[k]);
-        sm_store<EPI>(s, out, base + c0 + t, A, pend, a2, a1, first, acc);
+    const long per = (ntask + gridDim.x - 1) / gridDim.x;
+    const long t0 = (long)blockIdx.x * per, t1 = min(ntask, t0 + per);
+    for (long task = t0; task < t1; task++) {
+        long base = (task / nch) * n1;
+        int c0 = (int)(task % nch) * SM_CH1;
+        int len = min(SM_CH1, n1 - c0);
+        __syncthreads();  // the previous task's readers are done with tile (and wts is ready)
+        for (int t = threadIdx.x; t < len + 2 * h; t += blockDim.x) {
+            int pos = c0 - h + t;
+            int q = fold_idx(pos, n1);
+            bool own = pos >= c0 && pos < c0 + len;
+            tile[t] = sm_load<PRO>(in, base + q, own, A, pend, a2, cur);
+        }
+        __syncthreads();
+        for (int t = threadIdx.x; t < len; t += blockDim.x) {
+            const float *c = tile + t + h;
+            float s = wts[0] * c[0];
+            for (int k = 1; k <= h; k++) s += wts[k] * (c[-k] + c[k]);
+            sm_store<EPI>(s, out, base + c0 + t, A, pend, a2, a1, first, acc);
+        }
     }
     sm_partials<EPI>(acc, part, gridDim.x, blockIdx.x, sh);
 }
 
-// Axes 2/3 (strided): block = 32 consecutive i1 x a chunk along the axis.
-// grid = (ceil(n1/32), ceil(naxis/SM_CHS), nother); index = i1 + o*ostride + q*stride.
+// Axes 2/3 (strided): task = 32 consecutive i1 x a chunk along the axis x
+// one position of the other strided axis (ntx x nty x nother tasks, i1
+// chunk fastest); index = i1 + o*ostride + q*stride.
 template <int PRO, int EPI>
 __global__ void __launch_bounds__(256)
 smooth_strided(const float *__restrict__ in, float *__restrict__ out, int n1, int naxis, long stride,
-               long ostride, int nb, CgArgs A, const DipScal *sc, double *part, int first) {
+               long ostride, int nother, int nb, CgArgs A, const DipScal *sc, double *part, int first) {
     if (sc->cg_done) return;
     extern __shared__ float tile[];  // [(SM_CHS + 2h) rows][32]
     __shared__ float wts[SM_HMAX + 1];
     __shared__ double sh[16];
     int h = nb - 1;
     int tx = threadIdx.x, ty = threadIdx.y;
-    int i1 = blockIdx.x * 32 + tx;
-    int c0 = blockIdx.y * SM_CHS;
-    int len = min(SM_CHS, naxis - c0);
-    long base = (long)blockIdx.z * ostride + i1;
     bool pend = (PRO == PRO_CG || EPI == EPI_GP) ? (sc->pending != 0) : false;
     float a2 = pend ? (float)sc->alpha2 : 0.0f;
     float a1 = (EPI == EPI_S) ? (float)sc->alpha : 0.0f;
     int cur = (PRO == PRO_CG) ? sc->cur : 0;
     sm_weights(wts, nb);
-    if (i1 < n1) {
-        for (int row = ty; row < len + 2 * h; row += blockDim.y) {
-            int pos = c0 - h + row;
-            int q = fold_idx(pos, naxis);
-            bool own = pos >= c0 && pos < c0 + len;
-            tile[row * 32 + tx] = sm_load<PRO>(in, base + (long)q * stride, own, A, pend, a2, cur);
-        }
-    }
-    __syncthreads();
+    const int ntx = (n1 + 31) / 32, nty = (naxis + SM_CHS - 1) / SM_CHS;
+    const long ntask = (long)ntx * nty * nother;
+    const long per = (ntask + gridDim.x - 1) / gridDim.x;
+    const long t0 = (long)blockIdx.x * per, t1 = min(ntask, t0 + per);
     double acc[3] = {0.0, 0.0, 0.0};
-    if (i1 < n1) {
-        for (int row = ty; row < len; row += blockDim.y) {
-            const float *c = tile + (row + h) * 32 + tx;
-            float s = wts[0] * c[0];
-            for (int k = 1; k <= h; k++) s += wts[k] * (c[-k * 32] + c[k * 32]);
-            sm_store<EPI>(s, out, base + (long)(c0 + row) * stride, A, pend, a2, a1, first, acc);
+    for (long task = t0; task < t1; task++) {
+        int bx = (int)(task % ntx);
+        long rest = task / ntx;
+        int by = (int)(rest % nty), bz = (int)(rest / nty);
+        int i1 = bx * 32 + tx;
+        int c0 = by * SM_CHS;
+        int len = min(SM_CHS, naxis - c0);
+        long base = (long)bz * ostride + i1;
+        __syncthreads();
+        if (i1 < n1) {
+            for (int row = ty; row < len + 2 * h; row += blockDim.y) {
+                int pos = c0 - h + row;
+                int q = fold_idx(pos, naxis);
+                bool own = pos >= c0 && pos < c0 + len;
+                tile[row * 32 + tx] = sm_load<PRO>(in, base + (long)q * stride, own, A, pend, a2, cur);
+            }
+        }
+        __syncthreads();
+        if (i1 < n1) {
+            for (int row = ty; row < len; row += blockDim.y) {
+                const float *c = tile + (row + h) * 32 + tx;
+                float s = wts[0] * c[0];
+                for (int k = 1; k <= h; k++) s += wts[k] * (c[-k * 32] + c[k * 32]);
+                sm_store<EPI>(s, out, base + (long)(c0 + row) * stride, A, pend, a2, a1, first, acc);
+            }
         }
     }
-    sm_partials<EPI>(acc, part, gridDim.x * gridDim.y * gridDim.z,
-                     blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z), sh);
+    sm_partials<EPI>(acc, part, gridDim.x, blockIdx.x, sh);
 }
 
 // All axes in one pass ("2.5-D"): a block owns a TI1 x TI2 tile of
@@ -297,6 +316,13 @@ __global__ void fill_kernel(float *__restrict__ y, float v, long n) {
 // ---------------------------------------------------------------------------
 // Host orchestration
 // ---------------------------------------------------------------------------
+// Grid cap of the streaming dip kernels: each block walks a contiguous
+// range of work, so a launch that the device flags turn into a no-op (CG
+// converged, line search accepted) costs a few microseconds instead of a
+// full-size dispatch, and the partial sums stay few.
+constexpr long DIP_GRID_CAP = 4096;
+inline int pw_blocks(long n) { return (int)std::min<long>(ceil_div(n, 256), DIP_GRID_CAP); }
+
 struct DipWork {
     float *p[2], *u1, *u2, *w, *xa, *xb, *ra, *rb, *sp, *sx, *sr, *gp, *pc, *t1, *t2, *dp;
     unsigned char *m0, *m1;
@@ -395,11 +421,12 @@ int smooth_apply(sh_ctx *ctx, const DipGeom &g, const SmoothPlan &sp, const floa
              {"smooth_strided<1,0>", "smooth_strided<1,1>", "smooth_strided<1,2>"}}};
         PROF_A(ctx);
         if (ax == 0) {
-            dim3 grid((unsigned)((long)ceil_div(g.n1, SM_CH1) * g.n2 * g.n3));
+            long ntask = (long)ceil_div(g.n1, SM_CH1) * g.n2 * g.n3;
+            dim3 grid((unsigned)std::min<long>(ntask, DIP_GRID_CAP));
             nblk = grid.x;
 #define SM_C(PP, EE)                                                                          \
     if (P == PP && E == EE)                                                                   \
-        hipLaunchKernelGGL((smooth_contig<PP, EE>), grid, dim3(256), 0, ctx->stream, src, dst, g.n1, nb, A, \
+        hipLaunchKernelGGL((smooth_contig<PP, EE>), grid, dim3(256), 0, ctx->stream, src, dst, g.n1, ntask, nb, A, \
                            W.sc, W.part, first);
             SM_C(0, 0) SM_C(0, 1) SM_C(0, 2) SM_C(1, 0) SM_C(1, 1) SM_C(1, 2)
 #undef SM_C
@@ -408,13 +435,14 @@ int smooth_apply(sh_ctx *ctx, const DipGeom &g, const SmoothPlan &sp, const floa
             long stride = ax == 1 ? g.n1 : (long)g.n1 * g.n2;
             int nother = ax == 1 ? g.n3 : g.n2;
             long ostride = ax == 1 ? (long)g.n1 * g.n2 : g.n1;
-            dim3 grid(ceil_div(g.n1, 32), ceil_div(naxis, SM_CHS), nother);
-            nblk = grid.x * grid.y * grid.z;
+            long ntask = (long)ceil_div(g.n1, 32) * ceil_div(naxis, SM_CHS) * nother;
+            dim3 grid((unsigned)std::min<long>(ntask, DIP_GRID_CAP));
+            nblk = grid.x;
             size_t lds = (size_t)(SM_CHS + 2 * (nb - 1)) * 32 * sizeof(float);
 #define SM_S(PP, EE)                                                                          \
     if (P == PP && E == EE)                                                                   \
         hipLaunchKernelGGL((smooth_strided<PP, EE>), grid, dim3(32, 8), lds, ctx->stream, src, dst, g.n1, \
-                           naxis, stride, ostride, nb, A, W.sc, W.part, first);
+                           naxis, stride, ostride, nother, nb, A, W.sc, W.part, first);
             SM_S(0, 0) SM_S(0, 1) SM_S(0, 2) SM_S(1, 0) SM_S(1, 1) SM_S(1, 2)
 #undef SM_S
         }
@@ -436,7 +464,7 @@ int finalize(sh_ctx *ctx, DipWork &W, int nblk, int K, int mode, int it, long n,
 template <int NW>
 int pwd_launch(sh_ctx *ctx, int mode, const DipGeom &g, int axis, const float *x, const float *p,
                const float *dp, float lam, float *pout, DipWork &W, const unsigned char *mask) {
-    int nblk = ceil_div(g.n, 256);
+    int nblk = pw_blocks(g.n);
     const float pmax = 3.402823466e+38f, pmin = -3.402823466e+38f;
     static const char *names[3] = {"pwd_kernel<apply>", "pwd_kernel<der_divne>", "pwd_kernel<trial>"};
     static const double bps[3] = {12.0, 20.0, 20.0};
@@ -460,7 +488,7 @@ template <int NW>
 int dip_direction(sh_ctx *ctx, const DipGeom &g, const SmoothPlan &plan, int axis, const float *x,
                   const unsigned char *mask, int niter, int liter, DipWork &W, int *final_idx) {
     long n = g.n;
-    int nb_pw = ceil_div(n, 256);
+    int nb_pw = pw_blocks(n);
     int cur = 0;  // index of the current dip buffer
     CgArgs A{W.w, W.xa, W.xb, W.ra, W.rb, W.sx, W.sr, W.sp, W.pc, W.gp};
     SH_TRY(pwd_launch<NW>(ctx, 0, g, axis, x, W.p[cur], nullptr, 0.0f, nullptr, W, mask));

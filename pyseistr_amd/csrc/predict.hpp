@@ -533,6 +533,152 @@ SH_HD void solve_factored(const Acc *in, const Acc *q, const bool *rev, const fl
     }
 }
 
+// M forward predictions that share the slopes, the directions and the
+// factor (the slots of one spray level and quadrant that arrive at the same
+// target trace): the filter taps and factor values are loaded once per
+// sample for all of them.  Right-hand side r uses in[r*NF + f], scratch
+// y[r] and writes out[r] when live[r]; a dead one must still point at
+// readable inputs.  Each live result is bit-identical to solve_factored.
+template <int NW, int NF, int M>
+SH_HD void solve_factored_multi(const Acc *in, const bool *live, const Acc *q, const bool *rev, const float *F,
+                                long fs, long fplane, const Sink *y, int n1, float e, const Sink *out) {
+    constexpr int NA = 2 * NW + 1, NB = 2 * NW;
+    float tb[M][4];
+#pragma unroll
+    for (int r = 0; r < M; r++) {
+        const Acc *ir = in + r * NF;
+        if (NF == 1) {
+            tb[r][0] = ir[0][0]; tb[r][1] = ir[0][1]; tb[r][2] = ir[0][n1 - 2]; tb[r][3] = ir[0][n1 - 1];
+        } else {
+            tb[r][0] = (float)(0.5 * (ir[0][0] + ir[1][0]));
+            tb[r][1] = (float)(0.5 * (ir[0][1] + ir[1][1]));
+            tb[r][2] = (float)(0.5 * (ir[0][n1 - 2] + ir[1][n1 - 2]));
+            tb[r][3] = (float)(0.5 * (ir[0][n1 - 1] + ir[1][n1 - 1]));
+        }
+    }
+    float A[NF][NA][NA], I[M][NF][NA], T[M][NF][NA];
+    float Lo[NB][NB], Bw[M][NB];
+#pragma unroll
+    for (int f = 0; f < NF; f++) {
+#pragma unroll
+        for (int w = 0; w < NA; w++) taps_at<NW>(q[f], w - NW, n1, rev[f], A[f][w]);
+#pragma unroll
+        for (int r = 0; r < M; r++) {
+            const Acc &ir = in[r * NF + f];
+#pragma unroll
+            for (int w = 0; w < NA; w++) I[r][f][w] = (w < n1) ? ir[w] : 0.0f;
+#pragma unroll
+            for (int w = 0; w < NA; w++) {
+                int kk = w - NW;
+                float t = 0.0f;
+                if (kk >= NW && kk < n1 - NW) {
+#pragma unroll
+                    for (int j = 0; j < NA; j++) t += A[f][w][j] * ((kk + j - NW) < n1 ? ir[kk + j - NW] : 0.0f);
+                }
+                T[r][f][w] = t;
+            }
+        }
+    }
+#pragma unroll
+    for (int rr = 0; rr < NB; rr++) {
+#pragma unroll
+        for (int r = 0; r < M; r++) Bw[r][rr] = 0.0f;
+#pragma unroll
+        for (int m = 0; m < NB; m++) Lo[rr][m] = 0.0f;
+    }
+    for (int k = 0; k < n1; k++) {
+        int m1 = k < NB ? k : NB;
+        float dk = F[(long)k * fs];
+        float ok[NB];
+#pragma unroll
+        for (int m = 0; m < NB; m++) ok[m] = F[(long)(1 + m) * fplane + (long)k * fs];
+        float bnew[M];
+#pragma unroll
+        for (int r = 0; r < M; r++) {
+            float rhs = 0.0f;
+#pragma unroll
+            for (int f = 0; f < NF; f++) {
+                float of = 0.0f;
+#pragma unroll
+                for (int j = 0; j < NA; j++) {
+                    int kk = k + j - NW;
+                    if (kk >= NW && kk < n1 - NW) of += A[f][j][j] * T[r][f][j];
+                }
+                rhs = (f == 0) ? of : (rhs + of);
+            }
+            if (k == 0) rhs += e * tb[r][0];
+            if (k == 1) rhs += e * tb[r][1];
+            if (k == n1 - 2) rhs += e * tb[r][2];
+            if (k == n1 - 1) rhs += e * tb[r][3];
+            float b = rhs;
+#pragma unroll
+            for (int m = 0; m < NB; m++)
+                if (m < m1) b -= (Lo[m][m]) * Bw[r][m];
+            bnew[r] = b;
+            if (live[r]) y[r][k] = b / dk;
+        }
+#pragma unroll
+        for (int rr = NB - 1; rr > 0; rr--) {
+#pragma unroll
+            for (int r = 0; r < M; r++) Bw[r][rr] = Bw[r][rr - 1];
+#pragma unroll
+            for (int m = 0; m < NB; m++) Lo[rr][m] = Lo[rr - 1][m];
+        }
+#pragma unroll
+        for (int r = 0; r < M; r++) Bw[r][0] = bnew[r];
+#pragma unroll
+        for (int m = 0; m < NB; m++) Lo[0][m] = ok[m];
+#pragma unroll
+        for (int f = 0; f < NF; f++) {
+#pragma unroll
+            for (int w = 0; w < NA - 1; w++)
+#pragma unroll
+                for (int j = 0; j < NA; j++) A[f][w][j] = A[f][w + 1][j];
+            taps_at<NW>(q[f], k + 1 + NW, n1, rev[f], A[f][NA - 1]);
+            int kn = k + 1 + 2 * NW;
+            int kk = k + 1 + NW;
+#pragma unroll
+            for (int r = 0; r < M; r++) {
+#pragma unroll
+                for (int w = 0; w < NA - 1; w++) {
+                    I[r][f][w] = I[r][f][w + 1];
+                    T[r][f][w] = T[r][f][w + 1];
+                }
+                I[r][f][NA - 1] = (kn < n1) ? in[r * NF + f][kn] : 0.0f;
+                float tn = 0.0f;
+                if (kk >= NW && kk < n1 - NW) {
+#pragma unroll
+                    for (int j = 0; j < NA; j++) tn += A[f][NA - 1][j] * I[r][f][j];
+                }
+                T[r][f][NA - 1] = tn;
+            }
+        }
+    }
+    float nx[M][NB];
+#pragma unroll
+    for (int r = 0; r < M; r++)
+#pragma unroll
+        for (int m = 0; m < NB; m++) nx[r][m] = 0.0f;
+    for (int k = n1 - 1; k >= 0; k--) {
+        int m1 = (n1 - k - 1) < NB ? (n1 - k - 1) : NB;
+        float ok[NB];
+#pragma unroll
+        for (int m = 0; m < NB; m++) ok[m] = (m < m1) ? F[(long)(1 + m) * fplane + (long)k * fs] : 0.0f;
+#pragma unroll
+        for (int r = 0; r < M; r++) {
+            if (!live[r]) continue;
+            float t = y[r][k];
+#pragma unroll
+            for (int m = 0; m < NB; m++)
+                if (m < m1) t -= ok[m] * nx[r][m];
+#pragma unroll
+            for (int m = NB - 1; m > 0; m--) nx[r][m] = nx[r][m - 1];
+            nx[r][0] = t;
+            out[r][k] = t;
+        }
+    }
+}
+
 // Adjoint prediction with a precomputed factor: s = S^{-1} in (forward and
 // backward substitution), then out = A^T A s + e * boundary(s)
 // (predict_step with adj, sof_cfuns.c:622-649).  y holds s.

@@ -22,6 +22,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 
 namespace {
 
@@ -176,6 +177,58 @@ spray3_level(const float *__restrict__ pi, const float *__restrict__ px, float *
     constexpr int NB = 2 * NW;
     const float *Fv = F + (long)node_variant(nd.update, nd.up2, nd.up3) * (NB + 1) * n + j;
     solve_factored<NW, NF>(in, q, rev, Fv, ntr, n, y, n1, e, out);
+}
+
+// Two-source slots grouped by (level, quadrant): every slot of a group
+// arrives at its target through the same slopes, directions and factor, so
+// one thread per target trace predicts up to GM of them together
+// (solve_factored_multi).  A slot whose source lies outside the volume
+// stays zero at that target.
+constexpr int GM = 4;
+struct Group3 {
+    int cnt;
+    Node3 nd[GM];
+};
+constexpr int MAX_GROUPS = 16;
+struct GroupLevel {
+    Group3 g[MAX_GROUPS];
+};
+
+template <int NW, int M>
+__global__ void __launch_bounds__(TB)
+spray3_group(const float *__restrict__ pi, const float *__restrict__ px, float *__restrict__ U,
+             const float *__restrict__ F, const GroupLevel gl, int n1, int n2, int n3, int np2, float e,
+             float *__restrict__ scr, long nthr) {
+    long ntr = (long)n2 * n3, n = (long)n1 * ntr;
+    long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= ntr) return;
+    const Group3 &G = gl.g[blockIdx.y];
+    const int j2 = (int)(j % n2), j3 = (int)(j / n2);
+    const int up2 = G.nd[0].up2, up3 = G.nd[0].up3;
+    // the predecessor traces toward every source of the group
+    if (up2 ? j2 < 1 : j2 > n2 - 2) return;
+    if (up3 ? j3 < 1 : j3 > n3 - 2) return;
+    Acc in[2 * M];
+    bool live[M];
+    Sink y[M], out[M];
+    bool any = false;
+#pragma unroll
+    for (int r = 0; r < M; r++) {
+        const Node3 &nd = G.nd[r < G.cnt ? r : 0];
+        int i2 = j2 - nd.o2, i3 = j3 - nd.o3;
+        live[r] = r < G.cnt && i2 >= 0 && i2 < n2 && i3 >= 0 && i3 < n3;
+        any = any || live[r];
+        in[2 * r] = Acc{U + (long)(up2 ? nd.slot - 1 : nd.slot + 1) * n + (up2 ? j - 1 : j + 1), ntr};
+        in[2 * r + 1] = Acc{U + (long)(up3 ? nd.slot - np2 : nd.slot + np2) * n + (up3 ? j - n2 : j + n2), ntr};
+        y[r] = Sink{scr + ((long)blockIdx.y * M + r) * gridDim.x * blockDim.x + j, nthr};
+        out[r] = Sink{U + (long)nd.slot * n + j, ntr};
+    }
+    if (!any) return;
+    Acc q[2] = {Acc{pi + (up2 ? j - 1 : j), ntr}, Acc{px + (up3 ? j - n2 : j), ntr}};
+    bool rev[2] = {up2 != 0, up3 != 0};
+    constexpr int NB = 2 * NW;
+    const float *Fv = F + (long)node_variant(3, up2, up3) * (NB + 1) * n + j;
+    solve_factored_multi<NW, 2, M>(in, live, q, rev, Fv, ntr, n, y, n1, e, out);
 }
 
 // centre slot = data (trace-fast copy)
@@ -438,7 +491,9 @@ int spray3_run(sh_ctx *ctx, int order, int n1, int n2, int n3, int ns2, int ns3,
     hipLaunchKernelGGL(copy_slot, dim3(g), dim3(TB), 0, ctx->stream, dT, U + (long)ctr * n, n);
     SH_TRY(sh_launch_check(ctx, "copy_slot"));
     int gx = ceil_div(ntr, TB);
-    long nthr = (long)gx * TB * (long)maxn;
+    // per-thread scratch planes: one per node of a level (ungrouped), or
+    // MAX_GROUPS x GM per launch (grouped)
+    long nthr = (long)gx * TB * (long)std::max<size_t>(maxn, (size_t)MAX_GROUPS * GM);
     int NB1 = 2 * order + 1;
     float *scr = nullptr, *F = nullptr;
     SH_TRY(sh_slot(ctx, SLOT_SPRAY0 + 4, (size_t)nthr * n1 * sizeof(float), (void **)&scr));
@@ -451,8 +506,51 @@ int spray3_run(sh_ctx *ctx, int order, int n1, int n2, int n3, int ns2, int ns3,
         PROF_B(ctx, "factor3_kernel", 2.0 * 4.0 * (double)n);
         SH_TRY(sh_launch_check(ctx, "factor3_kernel"));
     }
+    // SH_SPRAY_GROUP=g: at most g two-source slots per thread (1..4; 0 = one
+    // thread per slot, as for the one-source slots)
+    const char *ge = getenv("SH_SPRAY_GROUP");
+    const int gmax = ge ? std::max(0, std::min(GM, atoi(ge))) : 2;
+    const bool grouped = gmax > 0;
     for (size_t L = 1; L < levels.size(); L++) {
-        for (int nf = 1; nf <= 2; nf++) {
+        if (grouped) {  // two-source slots by (quadrant, group size)
+            std::vector<Group3> groups;
+            for (int quad = 0; quad < 4; quad++) {
+                Group3 cur{};
+                for (auto &nd : levels[L]) {
+                    if (nd.update != 3 || (nd.up2 | (nd.up3 << 1)) != quad) continue;
+                    cur.nd[cur.cnt++] = nd;
+                    if (cur.cnt == gmax) {
+                        groups.push_back(cur);
+                        cur = Group3{};
+                    }
+                }
+                if (cur.cnt) groups.push_back(cur);
+            }
+            for (int m = 1; m <= GM; m++) {
+                GroupLevel gl{};
+                int cnt = 0;
+                double by = 0;
+                for (auto &g : groups)
+                    if (g.cnt == m) {
+                        if (cnt == MAX_GROUPS) return sh_fail(ctx, SH_EUNSUP, "spray radius too large");
+                        gl.g[cnt++] = g;
+                        by += 20.0 * m * (double)n;
+                    }
+                if (!cnt) continue;
+                dim3 grid(gx, cnt);
+                PROF_A(ctx);
+#define SG(MM)                                                                                                  \
+    if (m == MM)                                                                                                \
+        SH_DISPATCH_NW(ctx, order, NW,                                                                          \
+                       hipLaunchKernelGGL((spray3_group<NW, MM>), grid, dim3(TB), 0, ctx->stream, piT, pxT, U, F, \
+                                          gl, n1, n2, n3, np2, e, scr, nthr));
+                SG(1) SG(2) SG(3) SG(4)
+#undef SG
+                PROF_B(ctx, "spray3_group", by);
+                SH_TRY(sh_launch_check(ctx, "spray3_group"));
+            }
+        }
+        for (int nf = 1; nf <= (grouped ? 1 : 2); nf++) {
             Level3 lv{};
             int cnt = 0;
             double by = 0;

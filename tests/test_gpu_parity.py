@@ -210,3 +210,21 @@ def test_fused_smoothing_matches_separable(ps, monkeypatch):
     b = np.stack(ps.dip3dc(v, niter=3, liter=10, rect=[5, 5, 5]))
     monkeypatch.delenv("SH_SMOOTH_FUSED")
     assert np.abs(a - b).max() <= 1e-5
+
+
+def test_dip3d_long_run_matches_oracle(ps):
+    """Thirty Gauss-Newton iterations (the slow drift of the reference's
+    heavily damped update, not a device artefact): the dips stay within the
+    dip bars of the FP32 oracle and no farther from the FP64 oracle than the
+    FP32 oracle is (+1e-4)."""
+    from oracle import oracle as O
+    from pyseistr_amd import synth
+    s = (96, 40, 36)
+    v, _ = synth.volume3d(s, seed=5, n_events=3, smax=3.0, sigma=0.3, spike_frac=0.01)
+    di, dx = ps.dip3dc(v, niter=30, liter=10, rect=[5, 5, 5])
+    g = np.concatenate([di.ravel(order="F"), dx.ravel(order="F")]).astype(np.float64)
+    o32 = O.dip(v, s, niter=30, liter=10, order=2, rect=(5, 5, 5))
+    o64 = O.dip(v, s, niter=30, liter=10, order=2, rect=(5, 5, 5), dtype=np.float64)
+    d = np.abs(g - o32)
+    assert d.max() <= 2e-3 and np.sqrt(np.mean(d ** 2)) <= 2e-4
+    assert np.abs(g - o64).max() <= np.abs(o32 - o64).max() + 1e-4
