@@ -507,9 +507,11 @@ int spray3_run(sh_ctx *ctx, int order, int n1, int n2, int n3, int ns2, int ns3,
         SH_TRY(sh_launch_check(ctx, "factor3_kernel"));
     }
     // SH_SPRAY_GROUP=g: at most g two-source slots per thread (1..4; 0 = one
-    // thread per slot, as for the one-source slots)
+    // thread per source trace and slot).  Measured at 512x256x256, radius 4:
+    // g = 1 (one thread per target trace) 33.3 ms, g = 0 35.6 ms, g = 2..4
+    // 37.5-40.1 ms (register pressure costs more than the shared loads save).
     const char *ge = getenv("SH_SPRAY_GROUP");
-    const int gmax = ge ? std::max(0, std::min(GM, atoi(ge))) : 2;
+    const int gmax = ge ? std::max(0, std::min(GM, atoi(ge))) : 1;
     const bool grouped = gmax > 0;
     for (size_t L = 1; L < levels.size(); L++) {
         if (grouped) {  // two-source slots by (quadrant, group size)

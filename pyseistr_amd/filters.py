@@ -73,15 +73,22 @@ def somean2dc(dn, dip, ns, order, eps, adj=0, verb=1):
     return np.squeeze(ds) if n3 == 1 else ds
 
 
+def somf2d_device(d, p, shape, ns, order=2, eps=0.01, option=1, out=None):
+    """Device call: flat F-order tensors; nmf = 2ns+1 as somf2dc passes it."""
+    n1, n2, n3 = shape
+    ctx = _lib.context()
+    out = _lib.empty(n1 * n2 * n3) if out is None else out
+    _lib.check(ctx, _lib.load().sh_somf2d(ctx, _lib.ptr(d), _lib.ptr(p), n1, n2, n3, int(ns), 2 * int(ns) + 1,
+                                          int(option), int(order), float(eps), _lib.ptr(out)), "somf2dc")
+    return out
+
+
 def somf2dc(dn, dip, ns, order, eps, option=1, verb=1):
     """Structure-oriented median filter over the 2ns+1 sprayed traces."""
     dn = np.asarray(dn)
     n1, n2, n3 = _shape3(dn)
-    ctx = _lib.context()
     d, p = _lib.to_device(dn), _lib.to_device(dip)
-    out = _lib.empty(n1 * n2 * n3)
-    _lib.check(ctx, _lib.load().sh_somf2d(ctx, _lib.ptr(d), _lib.ptr(p), n1, n2, n3, int(ns), 2 * int(ns) + 1,
-                                          int(option), int(order), float(eps), _lib.ptr(out)), "somf2dc")
+    out = somf2d_device(d, p, (n1, n2, n3), ns, order, eps, option)
     ds = _lib.to_host(out).reshape(n1, n2, n3, order="F")
     return np.squeeze(ds) if n3 == 1 else ds
 

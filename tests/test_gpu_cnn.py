@@ -142,26 +142,27 @@ def test_errors(ws):
 @pytest.mark.skipif(not os.path.exists(os.path.join(os.path.dirname(__file__), "..", "pyseistr_amd", "weights",
                                                     "warmstart.npz")), reason="no trained weights committed")
 def test_warm_start_gain(ws):
-    """The trained network's warm start reaches a smaller dip error after 2
-    Gauss-Newton iterations than the cold start after 2 (and after 5), on a
-    held-out volume (seed 5, not in the training seeds)."""
+    """On a held-out volume (the seed-5 generator; training used seeds
+    10000+), the trained network + 2 Gauss-Newton iterations is closer to the
+    converged dip3d than the reference's default 5-iteration cold start, and
+    the cold start needs at least 8 iterations to match it."""
     import torch
     from pyseistr_amd import dip, synth
     shape = (256, 128, 128)
     v, _ = synth.volume3d(shape, seed=5, n_events=4, smax=3.0, sigma=0.3, spike_frac=0.01)
     d = torch.from_numpy(v.ravel(order="F")).cuda()
     kw = dict(liter=10, order=2, rect=(5, 5, 5))
-    conv = dip.dip_device(d, shape, niter=30, **kw)
+    conv = dip.dip_device(d, shape, niter=400, **kw)
     net = ws.WarmStart.load()
 
     def rmse(a):
         return float(torch.sqrt(torch.mean((a - conv) ** 2)).item())
 
-    init = net.predict_device(d, shape)
-    warm2 = net.dip_device(d, shape, niter=2, init=init, **kw)
-    cold2 = dip.dip_device(d, shape, niter=2, **kw)
-    cold5 = dip.dip_device(d, shape, niter=5, **kw)
-    r = {"init": rmse(init), "warm2": rmse(warm2), "cold2": rmse(cold2), "cold5": rmse(cold5)}
-    print(r)
-    assert r["warm2"] < 0.5 * r["cold2"]
-    assert r["warm2"] < r["cold5"]
+    warm2 = net.dip_device(d, shape, niter=2, **kw)
+    cold, errs = None, []
+    for _ in range(8):
+        cold = dip.dip_device(d, shape, niter=1, init=cold, **kw)
+        errs.append(rmse(cold))
+    print({"warm2": rmse(warm2), "cold1..8": [round(e, 4) for e in errs]})
+    assert rmse(warm2) < errs[4]
+    assert all(e > rmse(warm2) for e in errs[:7])
