@@ -169,15 +169,17 @@ int sh_somf2d_batched(sh_ctx *ctx, const float *data, const float *dip, const lo
  *   W1[27][32] b1[32] {W[27][32 in][32 out] b[32]} x4 W6[27][32][2] b6[2],
  * tap t = offset (t%3-1, (t/3)%3-1, t/9-1) along (i1, i2, i3). */
 long sh_cnn_nparams(void);
+/* dil (nullable: all 1): dilation of each of the six layers, 1..8 (tap t
+ * then covers the offset dil * (t%3-1, (t/3)%3-1, t/9-1)). */
 /* dip[2N] = inline || crossline prediction for the volume x[N] * xscale. */
 int sh_cnn_forward(sh_ctx *ctx, const float *params, const float *x, float xscale,
-                   int n1, int n2, int n3, float *dip);
+                   const int *dil, int n1, int n2, int n3, float *dip);
 /* loss = sum wt (y - target)^2 / (2N) over both components (target[2N],
  * wt nullable [N]); grad[nparams] overwritten.  loss (nullable, host):
  * filling it synchronises. */
 int sh_cnn_grad(sh_ctx *ctx, const float *params, const float *x, float xscale,
-                const float *target, const float *wt, int n1, int n2, int n3,
-                float *grad, double *loss);
+                const int *dil, const float *target, const float *wt, int n1, int n2,
+                int n3, float *grad, double *loss);
 /* One Adam step (step >= 1; L2 term wd * params added to the gradient). */
 int sh_cnn_adam(sh_ctx *ctx, float *params, const float *grad, float *m, float *v,
                 float lr, float b1, float b2, float eps, int step, float wd);

@@ -12,8 +12,9 @@ Fortran order (i1 fastest); arrays here are (n3, n2, n1, channels).
 Flat parameter vector (float32):
   W1[27][32] b1[32] | {W[27][32 ci][32 co] b[32]} x 4 (layers 2..5) |
   W6[27][32 ci][2 co] b6[2]
-Tap t covers the offset (t % 3 - 1, (t // 3) % 3 - 1, t // 9 - 1) along
-(i1, i2, i3): out[v] = sum_t in[v + off(t)] W[t] (a correlation).
+Tap t covers the offset dil * (t % 3 - 1, (t // 3) % 3 - 1, t // 9 - 1) along
+(i1, i2, i3), dil the layer's dilation: out[v] = sum_t in[v + off(t)] W[t]
+(a correlation).
 
 ``quant=True`` mirrors the device arithmetic's rounding points: the
 activations of layers 1..5, the weights of layers 2..6 and the
@@ -34,8 +35,8 @@ OFF_B6 = OFF_W6 + NT * C * 2
 NPARAM = OFF_B6 + 2
 
 
-def offsets():
-    return [(t % 3 - 1, (t // 3) % 3 - 1, t // 9 - 1) for t in range(NT)]
+def offsets(dil=1):
+    return [(dil * (t % 3 - 1), dil * ((t // 3) % 3 - 1), dil * (t // 9 - 1)) for t in range(NT)]
 
 
 def bf16(a):
@@ -89,31 +90,32 @@ def init_params(seed=0):
 def _shift(a, d1, d2, d3):
     """a[v + (d1, d2, d3)] with zeros outside; a is (n3, n2, n1, c)."""
     n3, n2, n1 = a.shape[:3]
-    p = np.pad(a, ((1, 1), (1, 1), (1, 1), (0, 0)))
-    return p[1 + d3:1 + d3 + n3, 1 + d2:1 + d2 + n2, 1 + d1:1 + d1 + n1]
+    h = max(abs(d1), abs(d2), abs(d3), 1)
+    p = np.pad(a, ((h, h), (h, h), (h, h), (0, 0)))
+    return p[h + d3:h + d3 + n3, h + d2:h + d2 + n2, h + d1:h + d1 + n1]
 
 
-def conv(a, W, b=None):
+def conv(a, W, b=None, dil=1):
     """out[v, co] = sum_t sum_ci a[v + off(t), ci] W[t, ci, co] (+ b)."""
     out = np.zeros(a.shape[:3] + (W.shape[2],), np.float64)
-    for t, (d1, d2, d3) in enumerate(offsets()):
+    for t, (d1, d2, d3) in enumerate(offsets(dil)):
         out += _shift(a, d1, d2, d3) @ W[t]
     if b is not None:
         out += b
     return out
 
 
-def conv_wgrad(a, g):
+def conv_wgrad(a, g, dil=1):
     """dW[t, ci, co] = sum_v a[v + off(t), ci] g[v, co]."""
     ci, co = a.shape[3], g.shape[3]
     gf = g.reshape(-1, co)
-    return np.stack([_shift(a, d1, d2, d3).reshape(-1, ci).T @ gf for (d1, d2, d3) in offsets()])
+    return np.stack([_shift(a, d1, d2, d3).reshape(-1, ci).T @ gf for (d1, d2, d3) in offsets(dil)])
 
 
-def conv_input_grad(g, W):
+def conv_input_grad(g, W, dil=1):
     """dL/da[u, ci] = sum_t sum_co g[u - off(t), co] W[t, ci, co]."""
     out = np.zeros(g.shape[:3] + (W.shape[1],), np.float64)
-    for t, (d1, d2, d3) in enumerate(offsets()):
+    for t, (d1, d2, d3) in enumerate(offsets(dil)):
         out += _shift(g, -d1, -d2, -d3) @ W[t].T
     return out
 
@@ -123,37 +125,40 @@ def _vol(x, shape):
     return np.asarray(x, np.float64).reshape(n3, n2, n1, 1)
 
 
-def forward(params, x, shape, xscale=1.0, quant=True):
-    """Returns (activations [a1..a5], y) with y (n3, n2, n1, 2)."""
+def forward(params, x, shape, xscale=1.0, quant=True, dil=None):
+    """Returns (activations [a1..a5], y) with y (n3, n2, n1, 2); dil:
+    per-layer dilations (None = all 1)."""
     q = bf16 if quant else (lambda v: v)
+    D = [1] * 6 if dil is None else [int(v) for v in dil]
     W, b = unpack(params)
     xs = _vol(x, shape) * np.float64(np.float32(xscale))
     if quant:
         xs = np.asarray(xs, np.float32).astype(np.float64)
     acts = [None]
-    a = q(np.maximum(conv(xs, W[1], b[1]), 0.0))
+    a = q(np.maximum(conv(xs, W[1], b[1], D[0]), 0.0))
     acts.append(a)
     for l in (2, 3, 4, 5):
-        a = q(np.maximum(conv(a, q(W[l]), b[l]), 0.0))
+        a = q(np.maximum(conv(a, q(W[l]), b[l], D[l - 1]), 0.0))
         acts.append(a)
-    y = conv(a, q(W[6]), b[6])
+    y = conv(a, q(W[6]), b[6], D[5])
     return acts, y
 
 
-def predict(params, x, shape, xscale=1.0, quant=True):
+def predict(params, x, shape, xscale=1.0, quant=True, dil=None):
     """Dips as the device returns them: flat [inline N || crossline N]."""
-    _, y = forward(params, x, shape, xscale, quant)
+    _, y = forward(params, x, shape, xscale, quant, dil)
     return np.concatenate([y[..., 0].ravel(), y[..., 1].ravel()])
 
 
-def loss_and_grad(params, x, target, shape, wt=None, xscale=1.0, quant=True):
+def loss_and_grad(params, x, target, shape, wt=None, xscale=1.0, quant=True, dil=None):
     """loss = sum wt (y - target)^2 / (2N) over both components; returns
     (loss, grad[NPARAM])."""
     q = bf16 if quant else (lambda v: v)
+    D = [1] * 6 if dil is None else [int(v) for v in dil]
     n1, n2, n3 = shape
     n = n1 * n2 * n3
     W, b = unpack(params)
-    acts, y = forward(params, x, shape, xscale, quant)
+    acts, y = forward(params, x, shape, xscale, quant, D)
     t = np.asarray(target, np.float64).reshape(2, n3, n2, n1).transpose(1, 2, 3, 0)
     m = np.ones((n3, n2, n1, 1)) if wt is None else np.asarray(wt, np.float64).reshape(n3, n2, n1, 1)
     if quant:  # the device forms y - t in float32
@@ -163,17 +168,17 @@ def loss_and_grad(params, x, target, shape, wt=None, xscale=1.0, quant=True):
     loss = float(np.sum(d * d * m) / (2 * n))
     g = q(2.0 * d * m / (2 * n))
     dW, db = {}, {}
-    dW[6] = conv_wgrad(acts[5], g)
+    dW[6] = conv_wgrad(acts[5], g, D[5])
     db[6] = g.reshape(-1, 2).sum(0)
-    g = q(conv_input_grad(g, q(W[6])) * (acts[5] > 0))
+    g = q(conv_input_grad(g, q(W[6]), D[5]) * (acts[5] > 0))
     for l in (5, 4, 3, 2):
-        dW[l] = conv_wgrad(acts[l - 1], g)
+        dW[l] = conv_wgrad(acts[l - 1], g, D[l - 1])
         db[l] = g.reshape(-1, C).sum(0)
-        g = q(conv_input_grad(g, q(W[l])) * (acts[l - 1] > 0))
+        g = q(conv_input_grad(g, q(W[l]), D[l - 1]) * (acts[l - 1] > 0))
     xs = _vol(x, shape) * np.float64(np.float32(xscale))
     if quant:
         xs = np.asarray(xs, np.float32).astype(np.float64)
-    dW[1] = conv_wgrad(xs, g)
+    dW[1] = conv_wgrad(xs, g, D[0])
     db[1] = g.reshape(-1, C).sum(0)
     return loss, pack(dW, db)
 

@@ -68,9 +68,9 @@ _SIGS = {
     "sh_somf2d_batched": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int,
                                   c_int, c_int, c_float, c_void_p]),
     "sh_cnn_nparams": (ctypes.c_long, []),
-    "sh_cnn_forward": (c_int, [c_void_p, c_void_p, c_void_p, c_float, c_int, c_int, c_int, c_void_p]),
-    "sh_cnn_grad": (c_int, [c_void_p, c_void_p, c_void_p, c_float, c_void_p, c_void_p, c_int, c_int, c_int,
-                            c_void_p, ctypes.POINTER(ctypes.c_double)]),
+    "sh_cnn_forward": (c_int, [c_void_p, c_void_p, c_void_p, c_float, c_int_p, c_int, c_int, c_int, c_void_p]),
+    "sh_cnn_grad": (c_int, [c_void_p, c_void_p, c_void_p, c_float, c_int_p, c_void_p, c_void_p, c_int, c_int,
+                            c_int, c_void_p, ctypes.POINTER(ctypes.c_double)]),
     "sh_cnn_adam": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_float, c_float, c_float,
                             c_float, c_int, c_float]),
 }

@@ -67,7 +67,7 @@ __global__ void pack_weights(const float *__restrict__ P, __bf16 *__restrict__ W
 // Layer 1 (1 -> 32), FP32 input, ReLU, BF16 output.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(TB) conv1_fwd(const float *__restrict__ x, float xscale, const float *__restrict__ P,
-                                                __bf16 *__restrict__ out, int n1, int n2, int n3) {
+                                                __bf16 *__restrict__ out, int n1, int n2, int n3, int dil) {
     __shared__ float w[NT * C], b[C];
     for (int i = threadIdx.x; i < NT * C; i += blockDim.x) w[i] = P[OFF_W1 + i];
     for (int i = threadIdx.x; i < C; i += blockDim.x) b[i] = P[OFF_B1 + i];
@@ -81,7 +81,7 @@ __global__ void __launch_bounds__(TB) conv1_fwd(const float *__restrict__ x, flo
     float xv[NT];
 #pragma unroll
     for (int t = 0; t < NT; t++) {
-        int d1 = t % 3 - 1, d2 = (t / 3) % 3 - 1, d3 = t / 9 - 1;
+        int d1 = (t % 3 - 1) * dil, d2 = ((t / 3) % 3 - 1) * dil, d3 = (t / 9 - 1) * dil;
         int j1 = i1 + d1, j2 = i2 + d2, j3 = i3 + d3;
         bool in = j1 >= 0 && j1 < n1 && j2 >= 0 && j2 < n2 && j3 >= 0 && j3 < n3;
         xv[t] = in ? x[v + d1 + (long)n1 * (d2 + (long)n2 * d3)] * xscale : 0.0f;
@@ -113,68 +113,106 @@ __global__ void __launch_bounds__(TB) conv1_fwd(const float *__restrict__ x, flo
 // MODE 0: out = relu(D + b) BF16;  MODE 1: out[v][0..1] = D + b FP32;
 // MODE 2: out = D * (aux > 0) BF16 (input gradient through the ReLU).
 // ---------------------------------------------------------------------------
+constexpr int CW = 8;          // waves per conv block (one i2 row of the tile each)
+constexpr int CTB = 64 * CW;   // threads per conv block
+
+// A fragments of one (d2, d3) row of taps: [d1][m][half]
+struct RowFrag {
+    bf16x8 a[3][2][2];
+};
+
+__device__ inline void load_row(RowFrag &f, const __bf16 *__restrict__ in, bool plane, long rowbase, int x0,
+                                int lane, int hi, int n1, int dil) {
+#pragma unroll
+    for (int d = 0; d < 3; d++)
+#pragma unroll
+        for (int m = 0; m < 2; m++) {
+            int j1 = x0 + m * 32 + (lane & 31) + (d - 1) * dil;
+            if (plane && j1 >= 0 && j1 < n1) {
+                const bf16x8 *src = (const bf16x8 *)(in + (rowbase + j1) * C + hi * 16);
+                f.a[d][m][0] = src[0];
+                f.a[d][m][1] = src[1];
+            } else {
+#pragma unroll
+                for (int i = 0; i < 8; i++) {
+                    f.a[d][m][0][i] = (__bf16)0.0f;
+                    f.a[d][m][1][i] = (__bf16)0.0f;
+                }
+            }
+        }
+}
+
 template <int MODE>
-__global__ void __launch_bounds__(TB)
+__global__ void __launch_bounds__(CTB)
 conv32(const __bf16 *__restrict__ in, const __bf16 *__restrict__ Wg, const float *__restrict__ bias,
-       void *__restrict__ outp, const __bf16 *__restrict__ aux, int n1, int n2, int n3) {
+       void *__restrict__ outp, const __bf16 *__restrict__ aux, int n1, int n2, int n3, int dil) {
     __shared__ bf16x8 wl[NT * C * C / 8];  // [27][32 co][32 ci]
     for (int i = threadIdx.x; i < NT * C * C / 8; i += blockDim.x) wl[i] = ((const bf16x8 *)Wg)[i];
     __syncthreads();
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int ntx = (n1 + 63) / 64, nty = (n2 + 3) / 4;
+    const int ntx = (n1 + 63) / 64, nty = (n2 + CW - 1) / CW;
     const long ntiles = (long)ntx * nty * n3;
     const long n12 = (long)n1 * n2;
+    // tiles ordered column-major in i3 (item = column * n3 + i3); each block
+    // walks a contiguous range, so consecutive tiles of a block share two of
+    // their three input planes in its XCD's L2
+    const long per = (ntiles + gridDim.x - 1) / gridDim.x;
+    const long t_begin = (long)blockIdx.x * per, t_end = min(ntiles, t_begin + per);
     // K order of the two MFMAs of a tap: lane half hi holds input channels
     // hi*16 + [0, 8) for the first and hi*16 + [8, 16) for the second, so
     // each lane reads its 16 channels (32 bytes) of a voxel in one go.
     const int co = lane & 31, hi = lane >> 5;
     const float bco = (MODE == 2) ? 0.0f : bias[co];
-    // the weights stay in LDS while the block walks over tiles
-    for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-        long bid = tile;
-        const int x0 = (int)(bid % ntx) * 64;
-        bid /= ntx;
-        const int y = (int)(bid % nty) * 4 + w;
-        const int z = (int)(bid / nty);
+    // the weights stay in LDS while the block walks over tiles; the A
+    // fragments of the next (d2, d3) row of taps are loaded while the
+    // current row's 12 MFMAs run
+    for (long item = t_begin; item < t_end; item++) {
+        const long col = item / n3;
+        const int z = (int)(item % n3);
+        const int x0 = (int)(col % ntx) * 64;
+        const int y = (int)(col / ntx) * CW + w;
         f32x16 acc[2];
 #pragma unroll
         for (int m = 0; m < 2; m++)
 #pragma unroll
             for (int r = 0; r < 16; r++) acc[m][r] = 0.0f;
         const bool rowok = y < n2;
-        for (int t = 0; t < NT; t++) {
-            int d1 = t % 3 - 1, d2 = (t / 3) % 3 - 1, d3 = t / 9 - 1;
+        RowFrag fa, fb;
+        auto load = [&](RowFrag &f, int row) {
+            int d2 = (row % 3 - 1) * dil, d3 = (row / 3 - 1) * dil;
             int j2 = y + d2, j3 = z + d3;
             bool plane = rowok && j2 >= 0 && j2 < n2 && j3 >= 0 && j3 < n3;
-            long rowbase = (long)j2 * n1 + (long)j3 * n12;
-            const bf16x8 b0 = wl[((t * C + co) * C + hi * 16) / 8];
-            const bf16x8 b1 = wl[((t * C + co) * C + hi * 16 + 8) / 8];
+            load_row(f, in, plane, (long)j2 * n1 + (long)j3 * n12, x0, lane, hi, n1, dil);
+        };
+        auto compute = [&](const RowFrag &f, int row) {
 #pragma unroll
-            for (int m = 0; m < 2; m++) {
-                int j1 = x0 + m * 32 + (lane & 31) + d1;
-                bf16x8 a0, a1;
-                if (plane && j1 >= 0 && j1 < n1) {
-                    const bf16x8 *src = (const bf16x8 *)(in + (rowbase + j1) * C + hi * 16);
-                    a0 = src[0];
-                    a1 = src[1];
-                } else {
+            for (int d = 0; d < 3; d++) {
+                const int t = row * 3 + d;
+                const bf16x8 b0 = wl[((t * C + co) * C + hi * 16) / 8];
+                const bf16x8 b1 = wl[((t * C + co) * C + hi * 16 + 8) / 8];
 #pragma unroll
-                    for (int i = 0; i < 8; i++) {
-                        a0[i] = (__bf16)0.0f;
-                        a1[i] = (__bf16)0.0f;
-                    }
+                for (int m = 0; m < 2; m++) {
+                    acc[m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f.a[d][m][0], b0, acc[m], 0, 0, 0);
+                    acc[m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f.a[d][m][1], b1, acc[m], 0, 0, 0);
                 }
-                acc[m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b0, acc[m], 0, 0, 0);
-                acc[m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b1, acc[m], 0, 0, 0);
             }
+        };
+        load(fa, 0);
+#pragma unroll 1
+        for (int row = 0; row < 8; row += 2) {
+            load(fb, row + 1);
+            compute(fa, row);
+            load(fa, row + 2);
+            compute(fb, row + 1);
         }
+        compute(fa, 8);
         if (!rowok) continue;
 #pragma unroll
         for (int m = 0; m < 2; m++) {
 #pragma unroll
             for (int r = 0; r < 16; r++) {
-                int row = (r / 4) * 8 + hi * 4 + (r % 4);
-                int i1 = x0 + m * 32 + row;
+                int rr = (r / 4) * 8 + hi * 4 + (r % 4);
+                int i1 = x0 + m * 32 + rr;
                 if (i1 >= n1) continue;
                 long v = (long)i1 + (long)y * n1 + (long)z * n12;
                 float s = acc[m][r] + bco;
@@ -203,14 +241,18 @@ conv32(const __bf16 *__restrict__ in, const __bf16 *__restrict__ Wg, const float
 // [27][32][32] sums to part[block]; db partials go to pb[block][32].
 // ---------------------------------------------------------------------------
 constexpr int WG_X = 64, WG_Y = 4;
-constexpr int WG_HX = WG_X + 2, WG_HY = WG_Y + 2;
+constexpr int MAX_DIL = 8;
+constexpr int WG_HXM = WG_X + 2 * MAX_DIL, WG_HYM = WG_Y + 2 * MAX_DIL;
 
+// The input is staged one tap plane (d3) at a time, with an in-plane halo
+// of `dil` voxels; the 9 taps of a plane are split over the 4 waves.
 __global__ void __launch_bounds__(TB)
 conv32_wgrad(const __bf16 *__restrict__ in, const __bf16 *__restrict__ g, float *__restrict__ part,
-             float *__restrict__ pb, int n1, int n2, int n3) {
-    // input tile [ci][z 3][y WG_HY][x WG_HX], gradient tile [co][y][x]
-    __shared__ __bf16 ti[C * 3 * WG_HY * WG_HX];
+             float *__restrict__ pb, int n1, int n2, int n3, int dil) {
+    // input plane [ci][y HY][x HX], gradient tile [co][y][x]
+    __shared__ __bf16 ti[C * WG_HYM * WG_HXM];
     __shared__ __bf16 tg[C * WG_Y * WG_X];
+    const int HX = WG_X + 2 * dil, HY = WG_Y + 2 * dil;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int ntx = (n1 + WG_X - 1) / WG_X, nty = (n2 + WG_Y - 1) / WG_Y;
     long bid = blockIdx.x;
@@ -219,14 +261,6 @@ conv32_wgrad(const __bf16 *__restrict__ in, const __bf16 *__restrict__ g, float 
     const int y0 = (int)(bid % nty) * WG_Y;
     const int z = (int)(bid / nty);
     const long n12 = (long)n1 * n2;
-    // stage input with halo (zero outside)
-    for (int i = threadIdx.x; i < 3 * WG_HY * WG_HX; i += blockDim.x) {
-        int xx = i % WG_HX, yy = (i / WG_HX) % WG_HY, zz = i / (WG_HX * WG_HY);
-        int j1 = x0 - 1 + xx, j2 = y0 - 1 + yy, j3 = z - 1 + zz;
-        bool ok = j1 >= 0 && j1 < n1 && j2 >= 0 && j2 < n2 && j3 >= 0 && j3 < n3;
-        const __bf16 *src = in + ((long)j1 + (long)j2 * n1 + (long)j3 * n12) * C;
-        for (int c = 0; c < C; c++) ti[c * (3 * WG_HY * WG_HX) + i] = ok ? src[c] : (__bf16)0.0f;
-    }
     for (int i = threadIdx.x; i < WG_Y * WG_X; i += blockDim.x) {
         int xx = i % WG_X, yy = i / WG_X;
         int j1 = x0 + xx, j2 = y0 + yy;
@@ -234,35 +268,46 @@ conv32_wgrad(const __bf16 *__restrict__ in, const __bf16 *__restrict__ g, float 
         const __bf16 *src = g + ((long)j1 + (long)j2 * n1 + (long)z * n12) * C;
         for (int c = 0; c < C; c++) tg[c * (WG_Y * WG_X) + i] = ok ? src[c] : (__bf16)0.0f;
     }
-    __syncthreads();
     const int kq = (lane >> 5) * 8;
-    for (int t = w; t < NT; t += 4) {
-        int d1 = t % 3 - 1, d2 = (t / 3) % 3 - 1, d3 = t / 9 - 1;
-        f32x16 acc;
-#pragma unroll
-        for (int r = 0; r < 16; r++) acc[r] = 0.0f;
-        // K = voxels, 16 per MFMA: rows yy, x chunks of 16
-        for (int yy = 0; yy < WG_Y; yy++) {
-            for (int xc = 0; xc < WG_X; xc += 16) {
-                // A[row = ci][k = voxel]: lane -> ci = lane & 31, voxels xc + kq .. +7
-                bf16x8 a, b;
-                const __bf16 *ai = ti + (lane & 31) * (3 * WG_HY * WG_HX) + (d3 + 1) * (WG_HY * WG_HX) +
-                                   (yy + d2 + 1) * WG_HX + (xc + kq + d1 + 1);
-                const __bf16 *bg = tg + (lane & 31) * (WG_Y * WG_X) + yy * WG_X + xc + kq;
-#pragma unroll
-                for (int i = 0; i < 8; i++) {
-                    a[i] = ai[i];
-                    b[i] = bg[i];
-                }
-                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
-            }
+    for (int zz = 0; zz < 3; zz++) {
+        const int j3 = z + (zz - 1) * dil;
+        __syncthreads();  // the previous plane's readers are done
+        for (int i = threadIdx.x; i < HY * HX; i += blockDim.x) {
+            int xx = i % HX, yy = i / HX;
+            int j1 = x0 - dil + xx, j2 = y0 - dil + yy;
+            bool ok = j1 >= 0 && j1 < n1 && j2 >= 0 && j2 < n2 && j3 >= 0 && j3 < n3;
+            const __bf16 *src = in + ((long)j1 + (long)j2 * n1 + (long)j3 * n12) * C;
+            for (int c = 0; c < C; c++) ti[c * (HY * HX) + i] = ok ? src[c] : (__bf16)0.0f;
         }
-        // D[row = ci][col = co]
-        float *dst = part + (size_t)blockIdx.x * NT * C * C + (size_t)t * C * C;
+        __syncthreads();
+        for (int tt = w; tt < 9; tt += 4) {
+            const int t = zz * 9 + tt;
+            const int d1 = (tt % 3 - 1) * dil, d2 = (tt / 3 - 1) * dil;
+            f32x16 acc;
 #pragma unroll
-        for (int r = 0; r < 16; r++) {
-            int ci = (r / 4) * 8 + (lane >> 5) * 4 + (r % 4);
-            dst[ci * C + (lane & 31)] = acc[r];
+            for (int r = 0; r < 16; r++) acc[r] = 0.0f;
+            // K = voxels, 16 per MFMA: rows yy, x chunks of 16
+            for (int yy = 0; yy < WG_Y; yy++) {
+                for (int xc = 0; xc < WG_X; xc += 16) {
+                    // A[row = ci][k = voxel]: lane -> ci = lane & 31, voxels xc + kq .. +7
+                    bf16x8 a, b;
+                    const __bf16 *ai = ti + (lane & 31) * (HY * HX) + (yy + d2 + dil) * HX + (xc + kq + d1 + dil);
+                    const __bf16 *bg = tg + (lane & 31) * (WG_Y * WG_X) + yy * WG_X + xc + kq;
+#pragma unroll
+                    for (int i = 0; i < 8; i++) {
+                        a[i] = ai[i];
+                        b[i] = bg[i];
+                    }
+                    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+                }
+            }
+            // D[row = ci][col = co]
+            float *dst = part + (size_t)blockIdx.x * NT * C * C + (size_t)t * C * C;
+#pragma unroll
+            for (int r = 0; r < 16; r++) {
+                int ci = (r / 4) * 8 + (lane >> 5) * 4 + (r % 4);
+                dst[ci * C + (lane & 31)] = acc[r];
+            }
         }
     }
     // bias partials: sum of g over the tile per channel
@@ -277,14 +322,14 @@ conv32_wgrad(const __bf16 *__restrict__ in, const __bf16 *__restrict__ g, float 
 // partials over 256 voxels; thread (t, co) pairs.
 __global__ void __launch_bounds__(1024)
 conv1_wgrad(const float *__restrict__ x, float xscale, const __bf16 *__restrict__ g, float *__restrict__ part, int n1,
-            int n2, int n3, int vox_per_block) {
+            int n2, int n3, int vox_per_block, int dil) {
     long n = (long)n1 * n2 * n3;
     long v0 = (long)blockIdx.x * vox_per_block;
     int tid = threadIdx.x;
     float s = 0.0f, sb = 0.0f;
     int t = tid / C, co = tid % C;
     if (tid < NT * C) {
-        int d1 = t % 3 - 1, d2 = (t / 3) % 3 - 1, d3 = t / 9 - 1;
+        int d1 = (t % 3 - 1) * dil, d2 = ((t / 3) % 3 - 1) * dil, d3 = (t / 9 - 1) * dil;
         for (long v = v0; v < std::min(n, v0 + vox_per_block); v++) {
             int i1 = (int)(v % n1);
             long r = v / n1;
@@ -420,40 +465,51 @@ int cnn_work(sh_ctx *ctx, long n, bool train, CnnWork &W, int n1, int n2, int n3
 }
 
 int conv_launch(sh_ctx *ctx, int mode, const __bf16 *in, const __bf16 *Wg, const float *bias, void *out,
-                const __bf16 *aux, int n1, int n2, int n3, const char *name) {
+                const __bf16 *aux, int n1, int n2, int n3, int dil, const char *name) {
     // persistent blocks: the 55 KB weight tile is staged once per block
-    long ntiles = (long)ceil_div(n1, 64) * ceil_div(n2, 4) * n3;
-    dim3 grid((unsigned)std::min<long>(ntiles, 2048));
+    long ntiles = (long)ceil_div(n1, 64) * ceil_div(n2, CW) * n3;
+    dim3 grid((unsigned)std::min<long>(ntiles, 1024));
     PROF_A(ctx);
     if (mode == 0)
-        hipLaunchKernelGGL((conv32<0>), grid, dim3(TB), 0, ctx->stream, in, Wg, bias, out, aux, n1, n2, n3);
+        hipLaunchKernelGGL((conv32<0>), grid, dim3(CTB), 0, ctx->stream, in, Wg, bias, out, aux, n1, n2, n3, dil);
     else if (mode == 1)
-        hipLaunchKernelGGL((conv32<1>), grid, dim3(TB), 0, ctx->stream, in, Wg, bias, out, aux, n1, n2, n3);
+        hipLaunchKernelGGL((conv32<1>), grid, dim3(CTB), 0, ctx->stream, in, Wg, bias, out, aux, n1, n2, n3, dil);
     else
-        hipLaunchKernelGGL((conv32<2>), grid, dim3(TB), 0, ctx->stream, in, Wg, bias, out, aux, n1, n2, n3);
+        hipLaunchKernelGGL((conv32<2>), grid, dim3(CTB), 0, ctx->stream, in, Wg, bias, out, aux, n1, n2, n3, dil);
     double v = (double)n1 * n2 * n3;
     PROF_B(ctx, name, v * (64.0 + (mode == 1 ? 8.0 : 64.0) + (mode == 2 ? 64.0 : 0.0)));
     return sh_launch_check(ctx, name);
 }
 
-int cnn_forward(sh_ctx *ctx, const float *P, const float *x, float xscale, int n1, int n2, int n3, CnnWork &W,
-                bool train) {
+int cnn_forward(sh_ctx *ctx, const float *P, const float *x, float xscale, const int *D, int n1, int n2, int n3,
+                CnnWork &W, bool train) {
     long n = (long)n1 * n2 * n3;
     long per = (long)NT * C * C;
     hipLaunchKernelGGL(pack_weights, dim3(ceil_div(5 * per, TB)), dim3(TB), 0, ctx->stream, P, W.Wf, W.Wb);
     SH_TRY(sh_launch_check(ctx, "pack_weights"));
     PROF_A(ctx);
-    hipLaunchKernelGGL(conv1_fwd, dim3(ceil_div(n, TB)), dim3(TB), 0, ctx->stream, x, xscale, P, W.act[1], n1, n2, n3);
+    hipLaunchKernelGGL(conv1_fwd, dim3(ceil_div(n, TB)), dim3(TB), 0, ctx->stream, x, xscale, P, W.act[1], n1, n2, n3, D[0]);
     PROF_B(ctx, "conv1_fwd", (double)n * (4.0 + 64.0));
     SH_TRY(sh_launch_check(ctx, "conv1_fwd"));
     __bf16 *cur = W.act[1];
     for (int l = 2; l <= 5; l++) {
         __bf16 *dst = train ? W.act[l] : (cur == W.act[1] ? W.act[2] : W.act[1]);
         SH_TRY(conv_launch(ctx, 0, cur, W.Wf + (size_t)(l - 2) * per, P + OFF_L(l) + per, dst, nullptr, n1, n2, n3,
-                           "conv32<relu>"));
+                           D[l - 1], "conv32<relu>"));
         cur = dst;
     }
-    return conv_launch(ctx, 1, cur, W.Wf + (size_t)4 * per, P + OFF_B6, W.y, nullptr, n1, n2, n3, "conv32<final>");
+    return conv_launch(ctx, 1, cur, W.Wf + (size_t)4 * per, P + OFF_B6, W.y, nullptr, n1, n2, n3, D[5],
+                       "conv32<final>");
+}
+
+// Per-layer dilations (NULL = all 1), each in [1, MAX_DIL].
+int read_dil(sh_ctx *ctx, const int *dil, int *D) {
+    for (int l = 0; l < 6; l++) {
+        D[l] = dil ? dil[l] : 1;
+        if (D[l] < 1 || D[l] > MAX_DIL) return sh_fail(ctx, SH_EINVAL, "cnn: dilation %d of layer %d not in 1..%d",
+                                                       D[l], l + 1, MAX_DIL);
+    }
+    return SH_OK;
 }
 
 }  // namespace
@@ -463,31 +519,35 @@ extern "C" {
 long sh_cnn_nparams(void) { return NPARAM; }
 
 // Predict (inline, crossline) dips dip[2N] from the normalised volume x[N].
-int sh_cnn_forward(sh_ctx *ctx, const float *params, const float *x, float xscale, int n1, int n2, int n3,
-                   float *dip) {
+int sh_cnn_forward(sh_ctx *ctx, const float *params, const float *x, float xscale, const int *dil, int n1, int n2,
+                   int n3, float *dip) {
     if (!ctx) return SH_ENULL;
     if (!params || !x || !dip) return sh_fail(ctx, SH_ENULL, "cnn_forward: NULL pointer");
     if (n1 < 1 || n2 < 1 || n3 < 1) return sh_fail(ctx, SH_EINVAL, "cnn_forward: bad shape");
     long n = (long)n1 * n2 * n3;
+    int D[6];
+    SH_TRY(read_dil(ctx, dil, D));
     CnnWork W{};
     SH_TRY(cnn_work(ctx, n, false, W, n1, n2, n3));
-    SH_TRY(cnn_forward(ctx, params, x, xscale, n1, n2, n3, W, false));
+    SH_TRY(cnn_forward(ctx, params, x, xscale, D, n1, n2, n3, W, false));
     hipLaunchKernelGGL(split_dips, dim3(ceil_div(n, TB)), dim3(TB), 0, ctx->stream, W.y, dip, n);
     return sh_launch_check(ctx, "split_dips");
 }
 
 // Loss and gradient: loss = sum wt (y - target)^2 / count over both dip
 // components (target [2][N], wt nullable weights [N]); grad[NPARAM] FP32.
-int sh_cnn_grad(sh_ctx *ctx, const float *params, const float *x, float xscale, const float *target, const float *wt,
-                int n1, int n2, int n3, float *grad, double *loss) {
+int sh_cnn_grad(sh_ctx *ctx, const float *params, const float *x, float xscale, const int *dil, const float *target,
+                const float *wt, int n1, int n2, int n3, float *grad, double *loss) {
     if (!ctx) return SH_ENULL;
     if (!params || !x || !target || !grad) return sh_fail(ctx, SH_ENULL, "cnn_grad: NULL pointer");
     if (n1 < 2 || n2 < 1 || n3 < 1) return sh_fail(ctx, SH_EINVAL, "cnn_grad: bad shape");
     long n = (long)n1 * n2 * n3;
     long per = (long)NT * C * C;
+    int D[6];
+    SH_TRY(read_dil(ctx, dil, D));
     CnnWork W{};
     SH_TRY(cnn_work(ctx, n, true, W, n1, n2, n3));
-    SH_TRY(cnn_forward(ctx, params, x, xscale, n1, n2, n3, W, true));
+    SH_TRY(cnn_forward(ctx, params, x, xscale, D, n1, n2, n3, W, true));
     int nb = ceil_div(n, TB);
     float scale = 1.0f / (float)(2 * n);
     hipLaunchKernelGGL(loss_grad, dim3(nb), dim3(TB), 0, ctx->stream, W.y, target, wt, W.gA, n, scale, W.red);
@@ -506,21 +566,24 @@ int sh_cnn_grad(sh_ctx *ctx, const float *params, const float *x, float xscale, 
     for (int l = 6; l >= 2; l--) {
         const __bf16 *ain = W.act[l - 1];
         PROF_A(ctx);
-        hipLaunchKernelGGL(conv32_wgrad, dim3(gwg), dim3(TB), 0, ctx->stream, ain, g, W.part, W.pb, n1, n2, n3);
+        hipLaunchKernelGGL(conv32_wgrad, dim3(gwg), dim3(TB), 0, ctx->stream, ain, g, W.part, W.pb, n1, n2, n3,
+                           D[l - 1]);
         PROF_B(ctx, "conv32_wgrad", (double)n * 128.0);
         SH_TRY(sh_launch_check(ctx, "conv32_wgrad"));
         hipLaunchKernelGGL(reduce_wgrad, dim3(nred), dim3(TB), 0, ctx->stream, W.part, W.pb, (int)gwg, l == 6 ? 1 : 0,
                            l == 6 ? 0L : OFF_L(l), grad);
         SH_TRY(sh_launch_check(ctx, "reduce_wgrad"));
         // input gradient through the ReLU of layer l-1
-        SH_TRY(conv_launch(ctx, 2, g, W.Wb + (size_t)(l - 2) * per, nullptr, gn, ain, n1, n2, n3, "conv32<bwd>"));
+        SH_TRY(conv_launch(ctx, 2, g, W.Wb + (size_t)(l - 2) * per, nullptr, gn, ain, n1, n2, n3, D[l - 1],
+                           "conv32<bwd>"));
         std::swap(g, gn);
     }
     // layer 1 weights
     int vpb = 256;
     int nb1 = ceil_div(n, vpb);
     SH_TRY(sh_slot(ctx, SLOT_INTERP0 + 15, (size_t)nb1 * (NT * C + C) * sizeof(float), (void **)&W.part));
-    hipLaunchKernelGGL(conv1_wgrad, dim3(nb1), dim3(1024), 0, ctx->stream, x, xscale, g, W.part, n1, n2, n3, vpb);
+    hipLaunchKernelGGL(conv1_wgrad, dim3(nb1), dim3(1024), 0, ctx->stream, x, xscale, g, W.part, n1, n2, n3, vpb,
+                       D[0]);
     SH_TRY(sh_launch_check(ctx, "conv1_wgrad"));
     hipLaunchKernelGGL(reduce_wgrad, dim3(ceil_div(NT * C + C, TB)), dim3(TB), 0, ctx->stream, W.part, nullptr, nb1, 2,
                        0L, grad);

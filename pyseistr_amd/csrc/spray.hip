@@ -138,22 +138,26 @@ factor3_kernel(const float *__restrict__ pi, const float *__restrict__ px, float
 }
 
 // One launch per (level, number of source traces NF): every thread predicts
-// one slot of one source trace with the shared factor of its target.
+// one slot at one target trace with the shared factor of the target.  A
+// slot whose source lies outside the volume is written as zeros there
+// (sof3d_cfuns.c:1478-1503 skips it; the slot stays zero).
 template <int NW, int NF>
 __global__ void __launch_bounds__(TB)
 spray3_level(const float *__restrict__ pi, const float *__restrict__ px, float *__restrict__ U,
              const float *__restrict__ F, const Level3 nodes, int n1, int n2, int n3, int np2, float e,
              float *__restrict__ scr, long nthr) {
     long ntr = (long)n2 * n3, n = (long)n1 * ntr;
-    long src = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (src >= ntr) return;
+    long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= ntr) return;
     Node3 nd = nodes.nd[blockIdx.y];
-    int i2 = (int)(src % n2), i3 = (int)(src / n2);
-    int j2 = i2 + nd.o2, j3 = i3 + nd.o3;
-    if (j2 < 0 || j2 >= n2 || j3 < 0 || j3 >= n3) return;
-    long j = j2 + (long)j3 * n2;
+    int j2 = (int)(j % n2), j3 = (int)(j / n2);
+    int i2 = j2 - nd.o2, i3 = j3 - nd.o3;
     Sink out{U + (long)nd.slot * n + j, ntr};
-    long tid = (long)blockIdx.y * gridDim.x * blockDim.x + src;
+    if (i2 < 0 || i2 >= n2 || i3 < 0 || i3 >= n3) {
+        for (int k = 0; k < n1; k++) out[k] = 0.0f;
+        return;
+    }
+    long tid = (long)blockIdx.y * gridDim.x * blockDim.x + j;
     Sink y{scr + tid, nthr};
     Acc in[2], q[2];
     bool rev[2];
@@ -205,25 +209,33 @@ spray3_group(const float *__restrict__ pi, const float *__restrict__ px, float *
     const Group3 &G = gl.g[blockIdx.y];
     const int j2 = (int)(j % n2), j3 = (int)(j / n2);
     const int up2 = G.nd[0].up2, up3 = G.nd[0].up3;
-    // the predecessor traces toward every source of the group
-    if (up2 ? j2 < 1 : j2 > n2 - 2) return;
-    if (up3 ? j3 < 1 : j3 > n3 - 2) return;
+    // a slot whose source lies outside the volume is zero at this target
+    bool any = false;
+    for (int r = 0; r < G.cnt; r++) {
+        const Node3 &nd = G.nd[r];
+        int i2 = j2 - nd.o2, i3 = j3 - nd.o3;
+        if (i2 >= 0 && i2 < n2 && i3 >= 0 && i3 < n3) {
+            any = true;
+        } else {
+            Sink z{U + (long)nd.slot * n + j, ntr};
+            for (int k = 0; k < n1; k++) z[k] = 0.0f;
+        }
+    }
+    // (a live source implies both predecessor traces exist)
+    if (!any) return;
     Acc in[2 * M];
     bool live[M];
     Sink y[M], out[M];
-    bool any = false;
 #pragma unroll
     for (int r = 0; r < M; r++) {
         const Node3 &nd = G.nd[r < G.cnt ? r : 0];
         int i2 = j2 - nd.o2, i3 = j3 - nd.o3;
         live[r] = r < G.cnt && i2 >= 0 && i2 < n2 && i3 >= 0 && i3 < n3;
-        any = any || live[r];
         in[2 * r] = Acc{U + (long)(up2 ? nd.slot - 1 : nd.slot + 1) * n + (up2 ? j - 1 : j + 1), ntr};
         in[2 * r + 1] = Acc{U + (long)(up3 ? nd.slot - np2 : nd.slot + np2) * n + (up3 ? j - n2 : j + n2), ntr};
         y[r] = Sink{scr + ((long)blockIdx.y * M + r) * gridDim.x * blockDim.x + j, nthr};
         out[r] = Sink{U + (long)nd.slot * n + j, ntr};
     }
-    if (!any) return;
     Acc q[2] = {Acc{pi + (up2 ? j - 1 : j), ntr}, Acc{px + (up3 ? j - n2 : j), ntr}};
     bool rev[2] = {up2 != 0, up3 != 0};
     constexpr int NB = 2 * NW;
@@ -484,9 +496,8 @@ int spray3_run(sh_ctx *ctx, int order, int n1, int n2, int n3, int ns2, int ns3,
     for (auto &l : levels) maxn = std::max(maxn, l.size());
     if (maxn > (size_t)MAX_LEVEL_NODES) return sh_fail(ctx, SH_EUNSUP, "spray radius too large");
     int g = ceil_div(n, TB);
-    // slots whose target lies outside the volume stay zero (sof3d_cfuns.c:1478-1503)
-    hipLaunchKernelGGL(zero_kernel, dim3(ceil_div(n * np, TB)), dim3(TB), 0, ctx->stream, U, n * np);
-    SH_TRY(sh_launch_check(ctx, "zero"));
+    // every slot of every target is written below: the prediction, or zeros
+    // where the slot's source lies outside the volume
     int ctr = ns3 * np2 + ns2;
     hipLaunchKernelGGL(copy_slot, dim3(g), dim3(TB), 0, ctx->stream, dT, U + (long)ctr * n, n);
     SH_TRY(sh_launch_check(ctx, "copy_slot"));

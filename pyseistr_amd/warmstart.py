@@ -48,27 +48,36 @@ def input_scale(d):
     return 1.0 / rms if rms > 0 else 1.0
 
 
-def cnn_forward(params, d, shape, xscale, out=None):
-    """Device call: dips [2N] (inline || crossline) for the flat volume d."""
+def _dil(dil):
+    if dil is None:
+        return None
+    d = (ctypes.c_int * 6)(*[int(v) for v in dil])
+    return ctypes.cast(d, _lib.c_int_p)
+
+
+def cnn_forward(params, d, shape, xscale, out=None, dil=None):
+    """Device call: dips [2N] (inline || crossline) for the flat volume d;
+    dil: per-layer dilations (None = all 1)."""
     n1, n2, n3 = (int(s) for s in shape)
     n = n1 * n2 * n3
     ctx = _lib.context()
     if out is None:
         out = _lib.empty(2 * n)
-    rc = _lib.load().sh_cnn_forward(ctx, _lib.ptr(params), _lib.ptr(d), float(xscale), n1, n2, n3, _lib.ptr(out))
+    rc = _lib.load().sh_cnn_forward(ctx, _lib.ptr(params), _lib.ptr(d), float(xscale), _dil(dil), n1, n2, n3,
+                                    _lib.ptr(out))
     _lib.check(ctx, rc, "cnn_forward")
     return out
 
 
-def cnn_grad(params, d, shape, xscale, target, wt=None, grad=None, want_loss=True):
+def cnn_grad(params, d, shape, xscale, target, wt=None, grad=None, want_loss=True, dil=None):
     """Device call: (loss or None, grad [NPARAM])."""
     n1, n2, n3 = (int(s) for s in shape)
     ctx = _lib.context()
     if grad is None:
         grad = _lib.empty(NPARAM)
     loss = ctypes.c_double(0.0)
-    rc = _lib.load().sh_cnn_grad(ctx, _lib.ptr(params), _lib.ptr(d), float(xscale), _lib.ptr(target), _lib.ptr(wt),
-                                 n1, n2, n3, _lib.ptr(grad), ctypes.byref(loss) if want_loss else None)
+    rc = _lib.load().sh_cnn_grad(ctx, _lib.ptr(params), _lib.ptr(d), float(xscale), _dil(dil), _lib.ptr(target),
+                                 _lib.ptr(wt), n1, n2, n3, _lib.ptr(grad), ctypes.byref(loss) if want_loss else None)
     _lib.check(ctx, rc, "cnn_grad")
     return (loss.value if want_loss else None), grad
 
@@ -91,6 +100,7 @@ class WarmStart:
             raise ValueError(f"expected {NPARAM} parameters, got {params.size}")
         self.params_host = params
         self.meta = dict(meta or {})
+        self.dil = [int(v) for v in self.meta.get("dilation", [1] * 6)]
         self._dev = None
 
     @classmethod
@@ -114,7 +124,7 @@ class WarmStart:
         """Initial dips [2N] for the flat Fortran-order device volume d."""
         if xscale is None:
             xscale = input_scale(d)
-        return cnn_forward(self.params, d, shape, xscale, out)
+        return cnn_forward(self.params, d, shape, xscale, out, dil=self.dil)
 
     def dip_device(self, d, shape, niter=2, liter=10, order=2, rect=(5, 5, 5), out=None, init=None, report=False):
         """Warm-started dip3d: CNN prediction, then niter GN iterations."""
@@ -200,7 +210,7 @@ def window_weights(win, margin):
 
 
 def train(data, steps, lr=1e-3, lr_end=1e-5, seed=0, win=(64, 64, 64), margin=4, log=print, log_every=100,
-          params=None):
+          params=None, dil=None):
     """Adam over random windows; cosine learning-rate decay.  Returns the
     FP32 parameters (host) and the loss history."""
     import torch
@@ -216,7 +226,7 @@ def train(data, steps, lr=1e-3, lr_end=1e-5, seed=0, win=(64, 64, 64), margin=4,
     for s in range(1, steps + 1):
         x, t, xs = data[int(rng.integers(len(data)))]
         want = s % 10 == 0 or s == 1
-        loss, _ = cnn_grad(p, x, win, xs, t, wt, grad=g, want_loss=want)
+        loss, _ = cnn_grad(p, x, win, xs, t, wt, grad=g, want_loss=want, dil=dil)
         a = lr_end + 0.5 * (lr - lr_end) * (1 + np.cos(np.pi * (s - 1) / max(steps - 1, 1)))
         cnn_adam(p, g, m, v, a, s)
         if want:
@@ -240,6 +250,9 @@ def main(argv=None):
     ap.add_argument("--seed0", type=int, default=10000)
     ap.add_argument("--out", type=str, default=WEIGHTS)
     ap.add_argument("--workers", type=int, default=8)
+    ap.add_argument("--dilation", type=str, default="1,1,1,1,1,1", help="per-layer dilations")
+    ap.add_argument("--window", type=int, default=64, help="cube edge of the training windows")
+    ap.add_argument("--margin", type=int, default=4, help="voxels at each window face left out of the loss")
     a = ap.parse_args(argv)
 
     def log(msg):
@@ -250,13 +263,16 @@ def main(argv=None):
     vol_shape = tuple(int(s) for s in a.vol_shape.split(","))
     seeds = list(range(a.seed0, a.seed0 + a.volumes))
     t0 = time.time()
-    data = make_dataset(seeds, vol_shape, (64, 64, 64), a.per_vol, a.target_niter, pool=pool, log=log)
+    win = (a.window,) * 3
+    data = make_dataset(seeds, vol_shape, win, a.per_vol, a.target_niter, pool=pool, log=log)
     pool.close()
     log(f"dataset ready: {len(data)} windows in {time.time() - t0:.0f} s")
-    params, hist = train(data, a.steps, lr=a.lr, log=log)
+    dil = [int(v) for v in a.dilation.split(",")]
+    params, hist = train(data, a.steps, lr=a.lr, log=log, dil=dil, win=win, margin=a.margin)
     meta = {"volumes": a.volumes, "vol_shape": vol_shape, "seeds": [a.seed0, a.seed0 + a.volumes - 1],
             "windows": len(data), "per_vol": a.per_vol, "target_niter": a.target_niter, "steps": a.steps,
-            "lr": a.lr, "final_loss": hist[-1][1] if hist else None,
+            "lr": a.lr, "dilation": dil, "window": a.window, "margin": a.margin,
+            "final_loss": hist[-1][1] if hist else None,
             "loss_history": hist[::max(1, len(hist) // 50)]}
     ws = WarmStart(params, meta)
     ws.save(a.out)

@@ -141,3 +141,38 @@ def test_input_grad_is_adjoint_of_conv(l):
     lhs = np.sum(K.conv(a, W) * g)
     rhs = np.sum(a * K.conv_input_grad(g, W))
     assert abs(lhs - rhs) < 1e-10 * abs(lhs)
+
+
+def test_dilated_network_matches_torch():
+    """Dilated layers (offsets scaled by the layer's dilation) against torch
+    conv3d with the same dilation, forward and gradient, float64."""
+    dil = (1, 2, 3, 2, 1, 1)
+    shape = (11, 9, 8)
+    p, x, t = _case(7, shape)
+    n1, n2, n3 = shape
+    n = n1 * n2 * n3
+    loss, g = K.loss_and_grad(p, x, t, shape, quant=False, dil=dil)
+    _, y = K.forward(p, x, shape, quant=False, dil=dil)
+    pt = torch.from_numpy(p.copy()).requires_grad_(True)
+    W, b = {}, {}
+    W[1] = pt[K.OFF_W1:K.OFF_B1].reshape(K.NT, 1, K.C)
+    b[1] = pt[K.OFF_B1:K.OFF_B1 + K.C]
+    for l in (2, 3, 4, 5):
+        o = K.OFF_L[l]
+        W[l] = pt[o:o + K.NT * K.C * K.C].reshape(K.NT, K.C, K.C)
+        b[l] = pt[o + K.NT * K.C * K.C:o + K._L]
+    W[6] = pt[K.OFF_W6:K.OFF_B6].reshape(K.NT, K.C, 2)
+    b[6] = pt[K.OFF_B6:K.OFF_B6 + 2]
+    a = torch.from_numpy(x.reshape(1, 1, n3, n2, n1))
+    for l in range(1, 7):
+        ci, co = W[l].shape[1], W[l].shape[2]
+        a = F.conv3d(a, W[l].reshape(3, 3, 3, ci, co).permute(4, 3, 0, 1, 2), b[l], padding=dil[l - 1],
+                     dilation=dil[l - 1])
+        if l < 6:
+            a = torch.relu(a)
+    assert np.abs(a[0].detach().numpy().transpose(1, 2, 3, 0) - y).max() < 1e-11 * max(1.0, np.abs(y).max())
+    L = torch.sum((a[0] - torch.from_numpy(t.reshape(2, n3, n2, n1))) ** 2) / (2 * n)
+    L.backward()
+    assert abs(L.item() - loss) < 1e-12 * max(1.0, abs(loss))
+    ref = pt.grad.numpy()
+    assert np.abs(g - ref).max() < 1e-11 * max(1.0, np.abs(ref).max())

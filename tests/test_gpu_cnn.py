@@ -94,6 +94,31 @@ def test_gradient_matches_oracle(ws, shape):
     assert abs(loss2 - l2) <= 1e-3 * abs(l2)
 
 
+@pytest.mark.parametrize("dil", [(1, 2, 4, 8, 2, 1), (3, 1, 2, 1, 5, 2)])
+def test_dilated_forward_and_gradient(ws, dil):
+    shape = (70, 21, 18)
+    p, x, t, wt = _case(4, shape)
+    xs = 0.6
+    got = ws.cnn_forward(_dev(p), _dev(x), shape, xs, dil=dil).cpu().numpy()
+    ref = K.predict(p, x, shape, xscale=xs, quant=True, dil=dil)
+    print("dilated forward rel err", _rel(got, ref))
+    assert _rel(got, ref) <= 5e-3
+    loss, g = ws.cnn_grad(_dev(p), _dev(x), shape, xs, _dev(t), _dev(wt), dil=dil)
+    g = g.cpu().numpy()
+    lref, gref = K.loss_and_grad(p, x, t, shape, wt=wt, xscale=xs, quant=True, dil=dil)
+    assert abs(loss - lref) <= 1e-3 * abs(lref)
+    for name, sl in _blocks().items():
+        e = _rel(g[sl], gref[sl])
+        print(f"grad {name}: rel err {e:.2e}")
+        assert e <= 2e-2, name
+
+
+def test_dilation_out_of_range_rejected(ws):
+    with pytest.raises(ValueError):
+        ws.cnn_forward(_dev(K.init_params(0)), _dev(np.zeros(64, np.float32)), (4, 4, 4), 1.0,
+                       dil=(1, 1, 9, 1, 1, 1))
+
+
 def test_adam_matches_formula(ws):
     rng = np.random.default_rng(3)
     p = rng.standard_normal(K.NPARAM).astype(np.float32)
@@ -133,8 +158,8 @@ def test_errors(ws):
     ctx = _lib.context()
     lib = _lib.load()
     p = torch.zeros(K.NPARAM, device="cuda")
-    assert lib.sh_cnn_forward(ctx, None, _lib.ptr(p), 1.0, 4, 4, 4, _lib.ptr(p)) == -2
-    assert lib.sh_cnn_forward(ctx, _lib.ptr(p), _lib.ptr(p), 1.0, 0, 4, 4, _lib.ptr(p)) == -1
+    assert lib.sh_cnn_forward(ctx, None, _lib.ptr(p), 1.0, None, 4, 4, 4, _lib.ptr(p)) == -2
+    assert lib.sh_cnn_forward(ctx, _lib.ptr(p), _lib.ptr(p), 1.0, None, 0, 4, 4, _lib.ptr(p)) == -1
     assert lib.sh_cnn_adam(ctx, _lib.ptr(p), _lib.ptr(p), _lib.ptr(p), _lib.ptr(p), 1e-3, 0.9, 0.999, 1e-8, 0,
                            0.0) == -1
 
