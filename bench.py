@@ -50,7 +50,7 @@ def parse():
     ap.add_argument("--cold-niter", type=int, default=5)
     ap.add_argument("--warm-niter", type=int, default=2)
     ap.add_argument("--target-rmse", type=float, default=0.02)
-    ap.add_argument("--conv-niter", type=int, default=600, help="cap of the convergence run")
+    ap.add_argument("--conv-niter", type=int, default=1000, help="cap of the convergence run")
     ap.add_argument("--conv-tol", type=float, default=1e-3, help="RMS change per iteration of p_conv")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-conv", action="store_true",
@@ -260,6 +260,28 @@ def main():
     roofline = {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                 "launches": int(cnt), "avg_us": round(ms / cnt * 1e3, 2), "bytes_per_launch": by / cnt}
+    # HBM bytes per launch of the same kernel from the committed rocprofv3
+    # FETCH_SIZE / WRITE_SIZE runs (scaled by a copy kernel), when present
+    tpath = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            tj = json.load(f)
+        def split(k):
+            k = k.replace(" ", "")
+            b, _, a = k.partition("<")
+            return b, [x for x in a.rstrip(">").split(",") if x]
+
+        base, args = split(name)
+        hit = []
+        for k, v in tj.get("kernels", {}).items():
+            kb, ka = split(k)
+            # the profiler's name carries the trailing template arguments
+            if kb == base and (not args or ka[len(ka) - len(args):] == args):
+                hit.append(v)
+        if hit:
+            # average over the template instances recorded for this kernel
+            roofline["traffic"] = sum(h["fetch_bytes"] + h["write_bytes"] for h in hit) / len(hit)
+            roofline["traffic_source"] = "profiles/traffic.json (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE)"
     top = sorted(prof.items(), key=lambda kv: -kv[1][1])[:8]
     breakdown = {k: {"ms_per_step": round(v[1] / args.steps, 3),
                      "GB/s": round(v[2] / (v[1] / 1e3) / 1e9, 1) if v[1] > 0 else None} for k, v in top}

@@ -13,6 +13,7 @@
 #include "common.hpp"
 
 #include <algorithm>
+#include <cstdlib>
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
@@ -190,11 +191,14 @@ conv32(const __bf16 *__restrict__ in, const __bf16 *__restrict__ Wg, const float
                 const int t = row * 3 + d;
                 const bf16x8 b0 = wl[((t * C + co) * C + hi * 16) / 8];
                 const bf16x8 b1 = wl[((t * C + co) * C + hi * 16 + 8) / 8];
+                // independent accumulators back to back (no dependent MFMA
+                // pair); each accumulator still sums half 0 then half 1
 #pragma unroll
-                for (int m = 0; m < 2; m++) {
+                for (int m = 0; m < 2; m++)
                     acc[m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f.a[d][m][0], b0, acc[m], 0, 0, 0);
+#pragma unroll
+                for (int m = 0; m < 2; m++)
                     acc[m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f.a[d][m][1], b1, acc[m], 0, 0, 0);
-                }
             }
         };
         load(fa, 0);
@@ -223,6 +227,117 @@ conv32(const __bf16 *__restrict__ in, const __bf16 *__restrict__ Wg, const float
                 } else {
                     float gate = (float)aux[v * C + co] > 0.0f ? 1.0f : 0.0f;
                     ((__bf16 *)outp)[v * C + co] = (__bf16)(s * gate);
+                }
+            }
+        }
+    }
+}
+
+// Wide-tile variant for small dilations: lane l of M-tile s holds voxel
+// x0 + 4l + s (four interleaved M-tiles, 128 voxels along i1).  The input
+// fragment of tile s and tap offset d1 is then the voxel group s + d1*dil,
+// so one (d2, d3) row of taps needs only 4 + 2*dil distinct A fragments
+// for its 12 MFMA pairs (against 12 in conv32): half the operand loads for
+// dil = 1.  The accumulation order per output is that of conv32, so both
+// give identical results.
+template <int DIL>
+struct RowFragW {
+    bf16x8 a[4 + 2 * DIL][2];
+};
+
+template <int DIL>
+__device__ inline void load_row_w(RowFragW<DIL> &f, const __bf16 *__restrict__ in, bool plane, long rowbase, int x0,
+                                  int lane, int hi, int n1) {
+#pragma unroll
+    for (int g = 0; g < 4 + 2 * DIL; g++) {
+        int j1 = x0 + 4 * (lane & 31) + g - DIL;
+        if (plane && j1 >= 0 && j1 < n1) {
+            const bf16x8 *src = (const bf16x8 *)(in + (rowbase + j1) * C + hi * 16);
+            f.a[g][0] = src[0];
+            f.a[g][1] = src[1];
+        } else {
+#pragma unroll
+            for (int i = 0; i < 8; i++) {
+                f.a[g][0][i] = (__bf16)0.0f;
+                f.a[g][1][i] = (__bf16)0.0f;
+            }
+        }
+    }
+}
+
+template <int MODE, int DIL>
+__global__ void __launch_bounds__(CTB)
+conv32w(const __bf16 *__restrict__ in, const __bf16 *__restrict__ Wg, const float *__restrict__ bias,
+        void *__restrict__ outp, const __bf16 *__restrict__ aux, int n1, int n2, int n3) {
+    __shared__ bf16x8 wl[NT * C * C / 8];  // [27][32 co][32 ci]
+    for (int i = threadIdx.x; i < NT * C * C / 8; i += blockDim.x) wl[i] = ((const bf16x8 *)Wg)[i];
+    __syncthreads();
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int ntx = (n1 + 127) / 128, nty = (n2 + CW - 1) / CW;
+    const long ntiles = (long)ntx * nty * n3;
+    const long n12 = (long)n1 * n2;
+    const long per = (ntiles + gridDim.x - 1) / gridDim.x;
+    const long t_begin = (long)blockIdx.x * per, t_end = min(ntiles, t_begin + per);
+    const int co = lane & 31, hi = lane >> 5;
+    const float bco = (MODE == 2) ? 0.0f : bias[co];
+    for (long item = t_begin; item < t_end; item++) {
+        const long col = item / n3;
+        const int z = (int)(item % n3);
+        const int x0 = (int)(col % ntx) * 128;
+        const int y = (int)(col / ntx) * CW + w;
+        f32x16 acc[4];
+#pragma unroll
+        for (int m = 0; m < 4; m++)
+#pragma unroll
+            for (int r = 0; r < 16; r++) acc[m][r] = 0.0f;
+        const bool rowok = y < n2;
+        RowFragW<DIL> fa, fb;
+        auto load = [&](RowFragW<DIL> &f, int row) {
+            int d2 = (row % 3 - 1) * DIL, d3 = (row / 3 - 1) * DIL;
+            int j2 = y + d2, j3 = z + d3;
+            bool plane = rowok && j2 >= 0 && j2 < n2 && j3 >= 0 && j3 < n3;
+            load_row_w<DIL>(f, in, plane, (long)j2 * n1 + (long)j3 * n12, x0, lane, hi, n1);
+        };
+        auto compute = [&](const RowFragW<DIL> &f, int row) {
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                const int t = row * 3 + d;
+                const bf16x8 b0 = wl[((t * C + co) * C + hi * 16) / 8];
+                const bf16x8 b1 = wl[((t * C + co) * C + hi * 16 + 8) / 8];
+#pragma unroll
+                for (int m = 0; m < 4; m++)
+                    acc[m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f.a[m + (d - 1) * DIL + DIL][0], b0, acc[m], 0, 0, 0);
+#pragma unroll
+                for (int m = 0; m < 4; m++)
+                    acc[m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f.a[m + (d - 1) * DIL + DIL][1], b1, acc[m], 0, 0, 0);
+            }
+        };
+        load(fa, 0);
+#pragma unroll 1
+        for (int row = 0; row < 8; row += 2) {
+            load(fb, row + 1);
+            compute(fa, row);
+            load(fa, row + 2);
+            compute(fb, row + 1);
+        }
+        compute(fa, 8);
+        if (!rowok) continue;
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+#pragma unroll
+            for (int r = 0; r < 16; r++) {
+                int rr = (r / 4) * 8 + hi * 4 + (r % 4);
+                int i1 = x0 + 4 * rr + m;
+                if (i1 >= n1) continue;
+                long v = (long)i1 + (long)y * n1 + (long)z * n12;
+                float sv = acc[m][r] + bco;
+                if (MODE == 0) {
+                    ((__bf16 *)outp)[v * C + co] = (__bf16)fmaxf(sv, 0.0f);
+                } else if (MODE == 1) {
+                    if (co < 2) ((float *)outp)[v * 2 + co] = sv;
+                } else {
+                    float gate = (float)aux[v * C + co] > 0.0f ? 1.0f : 0.0f;
+                    ((__bf16 *)outp)[v * C + co] = (__bf16)(sv * gate);
                 }
             }
         }
@@ -467,6 +582,18 @@ int cnn_work(sh_ctx *ctx, long n, bool train, CnnWork &W, int n1, int n2, int n3
 int conv_launch(sh_ctx *ctx, int mode, const __bf16 *in, const __bf16 *Wg, const float *bias, void *out,
                 const __bf16 *aux, int n1, int n2, int n3, int dil, const char *name) {
     // persistent blocks: the 55 KB weight tile is staged once per block
+    const char *wv = getenv("SH_CNN_NARROW");
+    if (dil == 1 && n1 >= 256 && !(wv && wv[0] == '1')) {  // wide tiles (half the operand loads)
+        long ntiles = (long)ceil_div(n1, 128) * ceil_div(n2, CW) * n3;
+        dim3 grid((unsigned)std::min<long>(ntiles, 1024));
+        PROF_A(ctx);
+#define CW_L(MM) hipLaunchKernelGGL((conv32w<MM, 1>), grid, dim3(CTB), 0, ctx->stream, in, Wg, bias, out, aux, n1, n2, n3)
+        if (mode == 0) CW_L(0); else if (mode == 1) CW_L(1); else CW_L(2);
+#undef CW_L
+        double v = (double)n1 * n2 * n3;
+        PROF_B(ctx, name, v * (64.0 + (mode == 1 ? 8.0 : 64.0) + (mode == 2 ? 64.0 : 0.0)));
+        return sh_launch_check(ctx, name);
+    }
     long ntiles = (long)ceil_div(n1, 64) * ceil_div(n2, CW) * n3;
     dim3 grid((unsigned)std::min<long>(ntiles, 1024));
     PROF_A(ctx);

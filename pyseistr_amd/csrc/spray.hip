@@ -258,38 +258,33 @@ __global__ void slot_mean(const float *__restrict__ U, float *__restrict__ out, 
     out[i] = s / (float)np;
 }
 
-// Exact q-th smallest of N values (rank counting); ties resolved to the
-// first value in window order.
-template <int N>
-__device__ inline float select_rank(const float *v, int q) {
-    float ans = v[0];
-    bool found = false;
-#pragma unroll
-    for (int i = 0; i < N; i++) {
-        int lt = 0, le = 0;
-#pragma unroll
-        for (int j = 0; j < N; j++) {
-            lt += (v[j] < v[i]) ? 1 : 0;
-            le += (v[j] <= v[i]) ? 1 : 0;
-        }
-        bool hit = !found && lt <= q && q < le;
-        ans = hit ? v[i] : ans;
-        found = found || hit;
-    }
-    return ans;
-}
-
 // Median of the nmf slots [c-m, c+m] of each sample (mf, axis 2, centre
-// column; sof3d_cfuns.c:1707-1736).  Window padded to N with +inf.
+// column; sof3d_cfuns.c:1707-1736): a pruned selection network on N
+// registers (median_net.hpp).  With nmf < N the window is padded with
+// (N-nmf)/2 values of -inf and as many of +inf, which leaves the median of
+// the real values in the network's median register.  The order statistic
+// is exact; only the sign of a zero median may differ from quickselect.
+template <int N> struct MedianNet;
+#include "median_net.hpp"
+
 template <int N>
 __global__ void __launch_bounds__(TB)
 slot_median(const float *__restrict__ U, float *__restrict__ out, long n, int first, int nmf) {
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    const int pad = (N - nmf) / 2;
     float v[N];
 #pragma unroll
-    for (int k = 0; k < N; k++) v[k] = (k < nmf) ? U[(long)(first + k) * n + i] : INFINITY;
-    out[i] = select_rank<N>(v, (nmf - 1) / 2);
+    for (int k = 0; k < N; k++)
+        v[k] = (k < nmf) ? U[(long)(first + k) * n + i] : (k - nmf < pad ? -INFINITY : INFINITY);
+#pragma unroll
+    for (int c = 0; c < MedianNet<N>::NOPS; c++) {
+        const int a = MedianNet<N>::OPS[c][0], b = MedianNet<N>::OPS[c][1];
+        const float x = v[a], y = v[b];
+        v[a] = fminf(x, y);
+        v[b] = fmaxf(x, y);
+    }
+    out[i] = v[MedianNet<N>::MED];
 }
 
 int launch_median(sh_ctx *ctx, const float *U, float *out, long n, int first, int nmf) {
@@ -297,7 +292,7 @@ int launch_median(sh_ctx *ctx, const float *U, float *out, long n, int first, in
 #define MED(NN) \
     if (nmf <= NN) { PROF_A(ctx); hipLaunchKernelGGL((slot_median<NN>), dim3(g), dim3(TB), 0, ctx->stream, U, out, n, first, nmf); \
         PROF_B(ctx, "slot_median", (nmf + 1) * 4.0 * (double)n); return sh_launch_check(ctx, "slot_median"); }
-    MED(9) MED(17) MED(33) MED(49) MED(65) MED(81)
+    MED(9) MED(13) MED(17) MED(33) MED(49) MED(65) MED(81)
 #undef MED
     return sh_fail(ctx, SH_EUNSUP, "median window %d exceeds 81", nmf);
 }
@@ -486,12 +481,42 @@ struct SprayBufs {
 };
 
 // 3-D spray into U (trace-fast, [slot][k][trace]) from trace-fast inputs.
+// Slots that a median over the linear slot window [first, first+nmf) reads,
+// plus every slot they are predicted from (their predecessors toward the
+// centre, recursively).  The other slots never reach the output.
+std::vector<char> needed_slots(int ns2, int ns3, int first, int nmf) {
+    int np2 = 2 * ns2 + 1, np = np2 * (2 * ns3 + 1);
+    std::vector<char> need(np, 0);
+    std::vector<int> stack;
+    for (int s = first; s < first + nmf; s++) stack.push_back(s);
+    while (!stack.empty()) {
+        int s = stack.back();
+        stack.pop_back();
+        if (s < 0 || s >= np || need[s]) continue;
+        need[s] = 1;
+        int a = s % np2 - ns2, b = s / np2 - ns3;
+        if (a != 0) stack.push_back(s + (a > 0 ? -1 : 1));
+        if (b != 0) stack.push_back(s + (b > 0 ? -np2 : np2));
+    }
+    return need;
+}
+
+// need (nullable): only these slots are predicted (the others are left
+// unwritten and must not be read).
 int spray3_run(sh_ctx *ctx, int order, int n1, int n2, int n3, int ns2, int ns3, float e, const float *dT,
-               const float *piT, const float *pxT, float *U, float **scr_io) {
+               const float *piT, const float *pxT, float *U, float **scr_io, const std::vector<char> *need = nullptr) {
     long ntr = (long)n2 * n3, n = (long)n1 * ntr;
     int np2 = 2 * ns2 + 1, np = np2 * (2 * ns3 + 1);
     std::vector<std::vector<Node3>> levels;
     plan3(ns2, ns3, levels);
+    if (need) {
+        for (auto &l : levels) {
+            std::vector<Node3> kept;
+            for (auto &nd : l)
+                if ((*need)[nd.slot]) kept.push_back(nd);
+            l.swap(kept);
+        }
+    }
     size_t maxn = 0;
     for (auto &l : levels) maxn = std::max(maxn, l.size());
     if (maxn > (size_t)MAX_LEVEL_NODES) return sh_fail(ctx, SH_EUNSUP, "spray radius too large");
@@ -517,12 +542,13 @@ int spray3_run(sh_ctx *ctx, int order, int n1, int n2, int n3, int ns2, int ns3,
         PROF_B(ctx, "factor3_kernel", 2.0 * 4.0 * (double)n);
         SH_TRY(sh_launch_check(ctx, "factor3_kernel"));
     }
-    // SH_SPRAY_GROUP=g: at most g two-source slots per thread (1..4; 0 = one
-    // thread per source trace and slot).  Measured at 512x256x256, radius 4:
-    // g = 1 (one thread per target trace) 33.3 ms, g = 0 35.6 ms, g = 2..4
-    // 37.5-40.1 ms (register pressure costs more than the shared loads save).
+    // SH_SPRAY_GROUP=g: at most g two-source slots of a quadrant per thread
+    // (1..4); 0 (default) = one thread per target trace and slot.  Measured
+    // at 512x256x256 for the 24 two-source slots somf3d needs: g = 0
+    // 15.3 ms, g = 1 16.3 ms, g = 2..4 17.5 ms (the register pressure of
+    // several right-hand sides costs more than the shared loads save).
     const char *ge = getenv("SH_SPRAY_GROUP");
-    const int gmax = ge ? std::max(0, std::min(GM, atoi(ge))) : 1;
+    const int gmax = ge ? std::max(0, std::min(GM, atoi(ge))) : 0;
     const bool grouped = gmax > 0;
     for (size_t L = 1; L < levels.size(); L++) {
         if (grouped) {  // two-source slots by (quadrant, group size)
@@ -756,8 +782,11 @@ int sh_somf3d(sh_ctx *ctx, const float *din, const float *dipi, const float *dip
     SH_TRY(stage3(ctx, din, dipi, dipx, n1, ntr, &dT, &piT, &pxT));
     SH_TRY(sh_slot(ctx, SLOT_SPRAY0 + 2, (size_t)n * np * sizeof(float), (void **)&U));
     const float e = 0.01f * 0.01f;  // eps forced to 0.01 (sof3d_cfuns.c:1586)
-    SH_TRY(spray3_run(ctx, order, n1, n2, n3, r1, r2, e, dT, piT, pxT, U, nullptr));
     int c = (np - 1) / 2, m = (nmf - 1) / 2;
+    // only the slots the median window reads and their predecessors (37 of
+    // 81 for r1 = r2 = 4, nmf = 33); the result is the same
+    std::vector<char> need = needed_slots(r1, r2, c - m, nmf);
+    SH_TRY(spray3_run(ctx, order, n1, n2, n3, r1, r2, e, dT, piT, pxT, U, nullptr, &need));
     SH_TRY(launch_median(ctx, U, dT, n, c - m, nmf));
     return transpose(ctx, dT, out, n1, ntr, false);
 }

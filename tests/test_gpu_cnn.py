@@ -113,6 +113,29 @@ def test_dilated_forward_and_gradient(ws, dil):
         assert e <= 2e-2, name
 
 
+def test_wide_tiles_match_oracle_and_narrow(ws, monkeypatch):
+    """n1 >= 256 selects the wide-tile kernel (4 interleaved M-tiles); it
+    must agree with the oracle and be bit-identical to the narrow kernel
+    (same MFMA accumulation order)."""
+    shape = (300, 10, 6)
+    p, x, t, wt = _case(5, shape)
+    xs = 0.4
+    got = ws.cnn_forward(_dev(p), _dev(x), shape, xs).cpu().numpy()
+    ref = K.predict(p, x, shape, xscale=xs, quant=True)
+    assert _rel(got, ref) <= 5e-3
+    loss, g = ws.cnn_grad(_dev(p), _dev(x), shape, xs, _dev(t), _dev(wt))
+    g = g.cpu().numpy()
+    monkeypatch.setenv("SH_CNN_NARROW", "1")
+    got_n = ws.cnn_forward(_dev(p), _dev(x), shape, xs).cpu().numpy()
+    loss_n, g_n = ws.cnn_grad(_dev(p), _dev(x), shape, xs, _dev(t), _dev(wt))
+    monkeypatch.delenv("SH_CNN_NARROW")
+    assert np.array_equal(got, got_n)
+    assert loss == loss_n and np.array_equal(g, g_n.cpu().numpy())
+    lref, gref = K.loss_and_grad(p, x, t, shape, wt=wt, xscale=xs, quant=True)
+    for name, sl in _blocks().items():
+        assert _rel(g[sl], gref[sl]) <= 2e-2, name
+
+
 def test_dilation_out_of_range_rejected(ws):
     with pytest.raises(ValueError):
         ws.cnn_forward(_dev(K.init_params(0)), _dev(np.zeros(64, np.float32)), (4, 4, 4), 1.0,
