@@ -133,6 +133,33 @@ def batched_2d(torch, count=2048, sample=128):
             "per_patch_s_est": round(tp, 3), "per_patch_sample": sample, "speedup": round(tp / tb, 2)}
 
 
+def measured_traffic(name, path=None):
+    """HBM bytes per launch of kernel `name` (as the library's timers name
+    it, e.g. "spray3_level<2>") from profiles/traffic.json, averaged over
+    the template instances whose trailing arguments match; None if absent."""
+    path = path or os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        tj = json.load(f)
+
+    def split(k):
+        k = k.replace(" ", "")
+        b, _, a = k.partition("<")
+        return b, [x for x in a.rstrip(">").split(",") if x]
+
+    base, targs = split(name)
+    hit = []
+    for k, v in tj.get("kernels", {}).items():
+        kb, ka = split(k)
+        # the profiler's name carries every template argument
+        if kb == base and (not targs or ka[len(ka) - len(targs):] == targs):
+            hit.append(v)
+    if not hit:
+        return None
+    return sum(h["fetch_bytes"] + h["write_bytes"] for h in hit) / len(hit)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -262,26 +289,10 @@ def main():
                 "launches": int(cnt), "avg_us": round(ms / cnt * 1e3, 2), "bytes_per_launch": by / cnt}
     # HBM bytes per launch of the same kernel from the committed rocprofv3
     # FETCH_SIZE / WRITE_SIZE runs (scaled by a copy kernel), when present
-    tpath = os.path.join(ROOT, "profiles", "traffic.json")
-    if os.path.exists(tpath):
-        with open(tpath) as f:
-            tj = json.load(f)
-        def split(k):
-            k = k.replace(" ", "")
-            b, _, a = k.partition("<")
-            return b, [x for x in a.rstrip(">").split(",") if x]
-
-        base, args = split(name)
-        hit = []
-        for k, v in tj.get("kernels", {}).items():
-            kb, ka = split(k)
-            # the profiler's name carries the trailing template arguments
-            if kb == base and (not args or ka[len(ka) - len(args):] == args):
-                hit.append(v)
-        if hit:
-            # average over the template instances recorded for this kernel
-            roofline["traffic"] = sum(h["fetch_bytes"] + h["write_bytes"] for h in hit) / len(hit)
-            roofline["traffic_source"] = "profiles/traffic.json (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE)"
+    traffic = measured_traffic(name)
+    if traffic is not None:
+        roofline["traffic"] = traffic
+        roofline["traffic_source"] = "profiles/traffic.json (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE)"
     top = sorted(prof.items(), key=lambda kv: -kv[1][1])[:8]
     breakdown = {k: {"ms_per_step": round(v[1] / args.steps, 3),
                      "GB/s": round(v[2] / (v[1] / 1e3) / 1e9, 1) if v[1] > 0 else None} for k, v in top}

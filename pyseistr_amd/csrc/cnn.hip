@@ -344,6 +344,179 @@ conv32w(const __bf16 *__restrict__ in, const __bf16 *__restrict__ Wg, const floa
     }
 }
 
+// Raw buffer access (32-bit byte offsets against a resource of known size):
+// loads past the size return zeros and stores past it are dropped.
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+constexpr unsigned BUF_OOB = 0xFFFFE000u;  // offset past every buffer used here
+__device__ inline __amdgpu_buffer_rsrc_t buf_rsrc(const void *p, unsigned bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes, 0x00020000);
+}
+__device__ inline bf16x8 ld_frag(__amdgpu_buffer_rsrc_t r, unsigned off, bool ok) {
+    u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
+    u32x4 z = {0u, 0u, 0u, 0u};
+    return __builtin_bit_cast(bf16x8, ok ? v : z);
+}
+// Row of A fragments through a buffer resource: o1 = byte offset of voxel
+// x0 + (lane & 31) of the row (+ the lane's channel half), BUF_OOB for a
+// row outside the volume (the hardware returns zeros); xm bit m*3+d clears
+// fragment (d, m) whose voxel lies outside [0, n1) along i1.
+__device__ inline void load_row_b(RowFrag &f, __amdgpu_buffer_rsrc_t r, unsigned o1, unsigned xm) {
+    const unsigned o0 = o1 - 64u;
+#pragma unroll
+    for (int m = 0; m < 2; m++)
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const unsigned o = (d == 0 ? o0 : o1) + m * 2048u + (d == 2 ? 64u : 0u);
+            const bool ok = (xm >> (m * 3 + d)) & 1u;
+            f.a[d][m][0] = ld_frag(r, o, ok);
+            f.a[d][m][1] = ld_frag(r, o + 16u, ok);
+        }
+}
+
+// Plane-scatter variant for dilation 1.  The block walks its z-range one
+// INPUT plane at a time: the three (d2) rows of plane p are loaded once
+// and feed the taps of the three outputs that read them, z = p + 1
+// (d3 = -1), z = p (d3 = 0) and z = p - 1 (d3 = +1), held in three rolling
+// accumulators; after plane p, output p - 1 is complete and is stored.
+// Each input row is thus read once per block and z-range instead of three
+// times, with 36 MFMAs per row load (12 in conv32).  Every output still
+// sums its taps in conv32's order (rows (d3, d2) ascending, then d1, then
+// the channel half), so both give identical results.  Rows are read and
+// outputs written through buffer resources (one lane offset per row or
+// plane plus immediates), which keeps the kernel within 256 VGPRs without
+// spills; volumes need n * 64 bytes < 4e9.
+template <int MODE>
+__global__ void __launch_bounds__(CTB)
+conv32z(const __bf16 *__restrict__ in, const __bf16 *__restrict__ Wg, const float *__restrict__ bias,
+        void *__restrict__ outp, const __bf16 *__restrict__ aux, int n1, int n2, int n3) {
+    __shared__ bf16x8 wl[NT * C * C / 8];  // [27][32 co][32 ci]
+    for (int i = threadIdx.x; i < NT * C * C / 8; i += blockDim.x) wl[i] = ((const bf16x8 *)Wg)[i];
+    __syncthreads();
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int ntx = (n1 + 63) / 64, nty = (n2 + CW - 1) / CW;
+    const long ntiles = (long)ntx * nty * n3;
+    const long n12 = (long)n1 * n2;
+    const long per = (ntiles + gridDim.x - 1) / gridDim.x;
+    const long t_begin = (long)blockIdx.x * per, t_end = min(ntiles, t_begin + per);
+    const int co = lane & 31, hi = lane >> 5;
+    const float bco = (MODE == 2) ? 0.0f : bias[co];
+    const unsigned nvox = (unsigned)((long)n12 * n3);
+    const __amdgpu_buffer_rsrc_t rin = buf_rsrc(in, nvox * 64u);
+    const __amdgpu_buffer_rsrc_t rout = buf_rsrc(outp, nvox * (MODE == 1 ? 8u : 64u));
+    const __amdgpu_buffer_rsrc_t raux = buf_rsrc(MODE == 2 ? (const void *)aux : (const void *)in, nvox * 64u);
+    long item = t_begin;
+    while (item < t_end) {
+        // one segment: outputs z in [za, zb) of one column
+        const long col = item / n3;
+        const int za = (int)(item % n3);
+        const int zb = (int)min<long>((long)n3, za + (t_end - item));
+        item += zb - za;
+        const int x0 = (int)(col % ntx) * 64;
+        const int y = (int)(col / ntx) * CW + w;
+        const bool rowok = y < n2;
+        f32x16 alo[2], amid[2], ahi[2];  // outputs p - 1, p, p + 1
+#pragma unroll
+        for (int m = 0; m < 2; m++)
+#pragma unroll
+            for (int r = 0; r < 16; r++) {
+                alo[m][r] = 0.0f;
+                amid[m][r] = 0.0f;
+                ahi[m][r] = 0.0f;
+            }
+        unsigned xm = 0;
+#pragma unroll
+        for (int m = 0; m < 2; m++)
+#pragma unroll
+            for (int d = 0; d < 3; d++)
+                if ((unsigned)(x0 + m * 32 + (lane & 31) + d - 1) < (unsigned)n1) xm |= 1u << (m * 3 + d);
+        const unsigned lb = (unsigned)(x0 + (lane & 31)) * 64u + (unsigned)hi * 32u;
+        auto load = [&](RowFrag &f, int p, int r2) {
+            int j2 = y + r2 - 1;
+            bool plane = rowok && j2 >= 0 && j2 < n2 && p >= 0 && p < n3;
+            unsigned o1 = plane ? lb + ((unsigned)j2 * (unsigned)n1 + (unsigned)p * (unsigned)n12) * 64u : BUF_OOB;
+            load_row_b(f, rin, o1, xm);
+        };
+        // taps of row (d3, d2) = (row / 3 - 1, row % 3 - 1) into acc
+        auto taps = [&](const RowFrag &f, int row, f32x16 *acc) {
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                const int t = row * 3 + d;
+                const bf16x8 b0 = wl[((t * C + co) * C + hi * 16) / 8];
+                const bf16x8 b1 = wl[((t * C + co) * C + hi * 16 + 8) / 8];
+#pragma unroll
+                for (int m = 0; m < 2; m++)
+                    acc[m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f.a[d][m][0], b0, acc[m], 0, 0, 0);
+#pragma unroll
+                for (int m = 0; m < 2; m++)
+                    acc[m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f.a[d][m][1], b1, acc[m], 0, 0, 0);
+            }
+        };
+        auto compute = [&](const RowFrag &f, int r2, bool dlo, bool dmid, bool dhi) {
+            if (dhi) taps(f, r2, ahi);        // d3 = -1
+            if (dmid) taps(f, 3 + r2, amid);  // d3 = 0
+            if (dlo) taps(f, 6 + r2, alo);    // d3 = +1
+        };
+        // stores through buffer resources: one lane offset per plane plus
+        // an immediate per element; the i1 tail tile checks each element
+        const bool xfull = x0 + 64 <= n1;
+        auto store = [&](const f32x16 *acc, int z) {
+            if (!rowok) return;
+            const unsigned vb = (unsigned)(x0 + hi * 4) + (unsigned)y * (unsigned)n1 + (unsigned)z * (unsigned)n12;
+#pragma unroll
+            for (int m = 0; m < 2; m++) {
+#pragma unroll
+                for (int r = 0; r < 16; r++) {
+                    const unsigned e = (unsigned)(m * 32 + (r / 4) * 8 + (r % 4));
+                    const bool ok = xfull || x0 + m * 32 + (r / 4) * 8 + hi * 4 + (r % 4) < n1;
+                    float sv = acc[m][r] + bco;
+                    if (MODE == 1) {
+                        if (co < 2)
+                            __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, sv), rout,
+                                                                  ok ? (vb + e) * 8u + co * 4u : BUF_OOB, 0, 0);
+                    } else {
+                        const unsigned o = ok ? (vb + e) * 64u + co * 2u : BUF_OOB;
+                        if (MODE == 2) {
+                            unsigned short g = __builtin_amdgcn_raw_buffer_load_b16(raux, o, 0, 0);
+                            sv = (float)__builtin_bit_cast(__bf16, g) > 0.0f ? sv : sv * 0.0f;
+                        } else {
+                            sv = fmaxf(sv, 0.0f);
+                        }
+                        __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, (__bf16)sv), rout, o, 0,
+                                                              0);
+                    }
+                }
+            }
+        };
+        // one input plane; f0 holds its row 0 on entry, f1 holds the next
+        // plane's row 0 on exit
+        auto plane = [&](RowFrag &f0, RowFrag &f1, int p) {
+            const bool dlo = p - 1 >= za, dmid = p >= za && p < zb, dhi = p + 1 < zb;
+            load(f1, p, 1);
+            compute(f0, 0, dlo, dmid, dhi);
+            load(f0, p, 2);
+            compute(f1, 1, dlo, dmid, dhi);
+            load(f1, p + 1, 0);
+            compute(f0, 2, dlo, dmid, dhi);
+            if (dlo) store(alo, p - 1);
+#pragma unroll
+            for (int m = 0; m < 2; m++) {
+                alo[m] = amid[m];
+                amid[m] = ahi[m];
+#pragma unroll
+                for (int r = 0; r < 16; r++) ahi[m][r] = 0.0f;
+            }
+        };
+        RowFrag fa, fb;
+        load(fa, za - 1, 0);
+#pragma unroll 1
+        for (int p = za - 1; p <= zb; p += 2) {
+            plane(fa, fb, p);
+            if (p + 1 > zb) break;
+            plane(fb, fa, p + 1);
+        }
+    }
+}
+
 // Input gradient of layer 1's output when layer 2 is the consumer is given
 // by conv32<2>; layer 1 needs no input gradient.
 
@@ -581,9 +754,27 @@ int cnn_work(sh_ctx *ctx, long n, bool train, CnnWork &W, int n1, int n2, int n3
 
 int conv_launch(sh_ctx *ctx, int mode, const __bf16 *in, const __bf16 *Wg, const float *bias, void *out,
                 const __bf16 *aux, int n1, int n2, int n3, int dil, const char *name) {
-    // persistent blocks: the 55 KB weight tile is staged once per block
-    const char *wv = getenv("SH_CNN_NARROW");
-    if (dil == 1 && n1 >= 256 && !(wv && wv[0] == '1')) {  // wide tiles (half the operand loads)
+    // persistent blocks: the 55 KB weight tile is staged once per block.
+    // SH_CNN_CONV selects the dilation-1 kernel: z (default) plane-scatter,
+    // w wide tiles, n the plain one (identical results, for timing)
+    const char *cv = getenv("SH_CNN_CONV");
+    const char kind = cv && cv[0] ? cv[0] : 'z';
+    // (the plane-scatter kernel addresses its buffers with 32-bit offsets)
+    if (dil == 1 && kind == 'z' && (double)n1 * n2 * n3 * 64.0 < 4.0e9) {
+        long ntiles = (long)ceil_div(n1, 64) * ceil_div(n2, CW) * n3;
+        dim3 grid((unsigned)std::min<long>(ntiles, 1024));
+        PROF_A(ctx);
+        if (mode == 0)
+            hipLaunchKernelGGL((conv32z<0>), grid, dim3(CTB), 0, ctx->stream, in, Wg, bias, out, aux, n1, n2, n3);
+        else if (mode == 1)
+            hipLaunchKernelGGL((conv32z<1>), grid, dim3(CTB), 0, ctx->stream, in, Wg, bias, out, aux, n1, n2, n3);
+        else
+            hipLaunchKernelGGL((conv32z<2>), grid, dim3(CTB), 0, ctx->stream, in, Wg, bias, out, aux, n1, n2, n3);
+        double v = (double)n1 * n2 * n3;
+        PROF_B(ctx, name, v * (64.0 + (mode == 1 ? 8.0 : 64.0) + (mode == 2 ? 64.0 : 0.0)));
+        return sh_launch_check(ctx, name);
+    }
+    if (dil == 1 && kind == 'w') {  // wide tiles (half the operand loads)
         long ntiles = (long)ceil_div(n1, 128) * ceil_div(n2, CW) * n3;
         dim3 grid((unsigned)std::min<long>(ntiles, 1024));
         PROF_A(ctx);

@@ -137,6 +137,24 @@ factor3_kernel(const float *__restrict__ pi, const float *__restrict__ px, float
     else factor_sweep<NW, 2>(q, rev, n1, e, Fs, n);
 }
 
+// Block -> (trace block, node) of a level launch over a 1-D grid of
+// nodes x gx8 blocks (gx8 = trace blocks rounded up to a multiple of 8).
+// Workgroups go to the 8 XCDs round robin (block b to XCD b % 8).  With
+// xcd != 0 the nodes of one trace block are consecutive blocks of one XCD,
+// so the slopes and factor traces they share (nodes of one variant read
+// the same ones) are fetched into that XCD's L2 once, while the nodes walk
+// the samples side by side; xcd = 0 is the plain node-major order.
+__device__ inline int level_block(int xcd, int cnt, long gx8, long &jb) {
+    const long b = blockIdx.x;
+    if (xcd) {
+        const long l = b >> 3;
+        jb = (l / cnt) * 8 + (b & 7);
+        return (int)(l % cnt);
+    }
+    jb = b % gx8;
+    return (int)(b / gx8);
+}
+
 // One launch per (level, number of source traces NF): every thread predicts
 // one slot at one target trace with the shared factor of the target.  A
 // slot whose source lies outside the volume is written as zeros there
@@ -144,12 +162,15 @@ factor3_kernel(const float *__restrict__ pi, const float *__restrict__ px, float
 template <int NW, int NF>
 __global__ void __launch_bounds__(TB)
 spray3_level(const float *__restrict__ pi, const float *__restrict__ px, float *__restrict__ U,
-             const float *__restrict__ F, const Level3 nodes, int n1, int n2, int n3, int np2, float e,
-             float *__restrict__ scr, long nthr) {
+             const float *__restrict__ F, const Level3 nodes, int cnt, int xcd, int n1, int n2, int n3, int np2,
+             float e, float *__restrict__ scr, long nthr) {
     long ntr = (long)n2 * n3, n = (long)n1 * ntr;
-    long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long gx8 = (ntr + 8L * TB - 1) / (8L * TB) * 8;
+    long jb;
+    const int node = level_block(xcd, cnt, gx8, jb);
+    long j = jb * TB + threadIdx.x;
     if (j >= ntr) return;
-    Node3 nd = nodes.nd[blockIdx.y];
+    Node3 nd = nodes.nd[node];
     int j2 = (int)(j % n2), j3 = (int)(j / n2);
     int i2 = j2 - nd.o2, i3 = j3 - nd.o3;
     Sink out{U + (long)nd.slot * n + j, ntr};
@@ -157,7 +178,7 @@ spray3_level(const float *__restrict__ pi, const float *__restrict__ px, float *
         for (int k = 0; k < n1; k++) out[k] = 0.0f;
         return;
     }
-    long tid = (long)blockIdx.y * gridDim.x * blockDim.x + j;
+    long tid = (long)node * gx8 * TB + j;
     Sink y{scr + tid, nthr};
     Acc in[2], q[2];
     bool rev[2];
@@ -527,9 +548,10 @@ int spray3_run(sh_ctx *ctx, int order, int n1, int n2, int n3, int ns2, int ns3,
     hipLaunchKernelGGL(copy_slot, dim3(g), dim3(TB), 0, ctx->stream, dT, U + (long)ctr * n, n);
     SH_TRY(sh_launch_check(ctx, "copy_slot"));
     int gx = ceil_div(ntr, TB);
+    const long gx8 = (long)ceil_div(gx, 8) * 8;  // level launches: gx8 blocks per node
     // per-thread scratch planes: one per node of a level (ungrouped), or
     // MAX_GROUPS x GM per launch (grouped)
-    long nthr = (long)gx * TB * (long)std::max<size_t>(maxn, (size_t)MAX_GROUPS * GM);
+    long nthr = gx8 * TB * (long)std::max<size_t>(maxn, (size_t)MAX_GROUPS * GM);
     int NB1 = 2 * order + 1;
     float *scr = nullptr, *F = nullptr;
     SH_TRY(sh_slot(ctx, SLOT_SPRAY0 + 4, (size_t)nthr * n1 * sizeof(float), (void **)&scr));
@@ -550,6 +572,10 @@ int spray3_run(sh_ctx *ctx, int order, int n1, int n2, int n3, int ns2, int ns3,
     const char *ge = getenv("SH_SPRAY_GROUP");
     const int gmax = ge ? std::max(0, std::min(GM, atoi(ge))) : 0;
     const bool grouped = gmax > 0;
+    // SH_SPRAY_XCD=0: node-major block order in the level launches instead
+    // of the XCD-grouped one (same results, for timing)
+    const char *xe = getenv("SH_SPRAY_XCD");
+    const int xcd = (xe && xe[0] == '0') ? 0 : 1;
     for (size_t L = 1; L < levels.size(); L++) {
         if (grouped) {  // two-source slots by (quadrant, group size)
             std::vector<Group3> groups;
@@ -593,23 +619,26 @@ int spray3_run(sh_ctx *ctx, int order, int n1, int n2, int n3, int ns2, int ns3,
             Level3 lv{};
             int cnt = 0;
             double by = 0;
-            for (auto &nd : levels[L])
-                if ((nd.update == 3 ? 2 : 1) == nf) {
-                    lv.nd[cnt++] = nd;
-                    by += (nf == 2 ? 20.0 : 12.0) * (double)n;  // per slot-sample: inputs, slopes, output
-                }
+            // nodes of one factor variant side by side (they share slopes
+            // and factor); the nodes of a level are independent of each other
+            for (int v = 0; v < 8; v++)
+                for (auto &nd : levels[L])
+                    if ((nd.update == 3 ? 2 : 1) == nf && node_variant(nd.update, nd.up2, nd.up3) == v) {
+                        lv.nd[cnt++] = nd;
+                        by += (nf == 2 ? 20.0 : 12.0) * (double)n;  // per slot-sample: inputs, slopes, output
+                    }
             if (!cnt) continue;
-            dim3 grid(gx, cnt);
+            dim3 grid((unsigned)(gx8 * cnt));
             PROF_A(ctx);
             if (nf == 1) {
                 SH_DISPATCH_NW(ctx, order, NW,
                                hipLaunchKernelGGL((spray3_level<NW, 1>), grid, dim3(TB), 0, ctx->stream, piT, pxT, U,
-                                                  F, lv, n1, n2, n3, np2, e, scr, nthr));
+                                                  F, lv, cnt, xcd, n1, n2, n3, np2, e, scr, nthr));
                 PROF_B(ctx, "spray3_level<1>", by);
             } else {
                 SH_DISPATCH_NW(ctx, order, NW,
                                hipLaunchKernelGGL((spray3_level<NW, 2>), grid, dim3(TB), 0, ctx->stream, piT, pxT, U,
-                                                  F, lv, n1, n2, n3, np2, e, scr, nthr));
+                                                  F, lv, cnt, xcd, n1, n2, n3, np2, e, scr, nthr));
                 PROF_B(ctx, "spray3_level<2>", by);
             }
             SH_TRY(sh_launch_check(ctx, "spray3_level"));
