@@ -433,7 +433,7 @@ bfactor2(const float *__restrict__ dipT, float *__restrict__ F, const BPatch *__
     Acc qa[1] = {Acc{dipT + q.off + (v == 0 ? i : i - 1), q.n2}};
     bool rev[1] = {v == 1};
     Sink Fs{F + (long)v * (NB + 1) * ntot + q.off + i, q.n2};
-    factor_sweep<NW, 1>(qa, rev, q.n1, e, Fs, ntot);
+    factor_sweep_pf<NW, 1, 8>(qa, rev, q.n1, e, Fs, ntot);
 }
 
 // Forward spray chains of pwspray_lop (sof_cfuns.c:998-1028), one thread per
@@ -469,7 +469,7 @@ bspray2(const float *__restrict__ xT, const float *__restrict__ dipT, const floa
         Acc qs[1] = {qa};
         bool rv[1] = {dir == 1};
         const float *Fv = F + (long)dir * (NB + 1) * ntot + q.off + ip;
-        solve_factored<NW, 1>(in, qs, rv, Fv, n2, ntot, y, n1, e, out);
+        solve_factored_pf<NW, 1, 8>(in, qs, rv, Fv, n2, ntot, y, n1, e, out);
         cur = Acc{out.p, n2};
     }
 }

@@ -422,6 +422,123 @@ SH_HD void factor_sweep(const Acc *q, const bool *rev, int n1, float e, Sink F, 
     }
 }
 
+// Taps from a slope value already loaded (taps_at without the load).
+template <int NW>
+SH_HD inline void taps_from(float qv, int kk, int n1, bool rev, float *a) {
+    constexpr int NA = 2 * NW + 1;
+    if (kk < 0 || kk >= n1) {
+#pragma unroll
+        for (int j = 0; j < NA; j++) a[j] = 0.0f;
+        return;
+    }
+    float b[NA];
+    apf_pass<NW>(qv, b);
+#pragma unroll
+    for (int j = 0; j < NA; j++) a[j] = rev ? b[NA - 1 - j] : b[j];
+}
+
+// factor_sweep with the slope loads software-pipelined D samples ahead (a
+// register ring; loop unrolled by D); same arithmetic, identical factor.
+template <int NW, int NF, int D>
+SH_HD void factor_sweep_pf(const Acc *q, const bool *rev, int n1, float e, Sink F, long fplane) {
+    constexpr int NA = 2 * NW + 1, NB = 2 * NW;
+    float A[NF][NA][NA];
+    float Ld[NB], Lo[NB][NB];
+#pragma unroll
+    for (int f = 0; f < NF; f++)
+#pragma unroll
+        for (int w = 0; w < NA; w++) taps_at<NW>(q[f], w - NW, n1, rev[f], A[f][w]);
+#pragma unroll
+    for (int r = 0; r < NB; r++) {
+        Ld[r] = 0.0f;
+#pragma unroll
+        for (int m = 0; m < NB; m++) Lo[r][m] = 0.0f;
+    }
+    float rq[D][NF];  // slope at sample k + 1 + NW
+    auto fetch = [&](int s, int k) {
+        const int kq = k + 1 + NW < n1 ? k + 1 + NW : n1 - 1;
+#pragma unroll
+        for (int f = 0; f < NF; f++) rq[s][f] = q[f][kq];
+    };
+    auto step = [&](int s, int k) {
+        float diag, offd[NB];
+        reg_row<NB>(k, n1, e, diag, offd);
+#pragma unroll
+        for (int f = 0; f < NF; f++) {
+#pragma unroll
+            for (int j = 0; j < NA; j++) {
+                int kk = k + j - NW;
+                if (kk >= NW && kk < n1 - NW) {
+                    float aj = A[f][j][j];
+                    diag += aj * aj;
+                }
+            }
+#pragma unroll
+            for (int m = 0; m < NB; m++) {
+#pragma unroll
+                for (int j = m + 1; j < NA; j++) {
+                    int kk = k + j - NW;
+                    if (kk >= NW && kk < n1 - NW) {
+                        float aj = A[f][j][j], am = A[f][j][j - m - 1];
+                        offd[m] += am * aj;
+                    }
+                }
+            }
+        }
+        float t = diag;
+        int m1 = k < NB ? k : NB;
+#pragma unroll
+        for (int m = 0; m < NB; m++)
+            if (m < m1) t -= (Lo[m][m]) * (Lo[m][m]) * (Ld[m]);
+        float dk = t;
+        float ok[NB];
+        int nn = (n1 - k - 1) < NB ? (n1 - k - 1) : NB;
+#pragma unroll
+        for (int nidx = 0; nidx < NB; nidx++) {
+            ok[nidx] = 0.0f;
+            if (nidx < nn) {
+                float sm = offd[nidx];
+                int mm = k < (NB - nidx - 1) ? k : (NB - nidx - 1);
+#pragma unroll
+                for (int m = 0; m < NB; m++)
+                    if (m < mm) sm -= (Lo[m][m]) * (Lo[m][nidx + m + 1]) * (Ld[m]);
+                ok[nidx] = sm / dk;
+            }
+        }
+        F.p[(long)k * F.s] = dk;
+#pragma unroll
+        for (int m = 0; m < NB; m++) F.p[(long)(1 + m) * fplane + (long)k * F.s] = ok[m];
+#pragma unroll
+        for (int r = NB - 1; r > 0; r--) {
+            Ld[r] = Ld[r - 1];
+#pragma unroll
+            for (int m = 0; m < NB; m++) Lo[r][m] = Lo[r - 1][m];
+        }
+        Ld[0] = dk;
+#pragma unroll
+        for (int m = 0; m < NB; m++) Lo[0][m] = ok[m];
+#pragma unroll
+        for (int f = 0; f < NF; f++) {
+#pragma unroll
+            for (int w = 0; w < NA - 1; w++)
+#pragma unroll
+                for (int j = 0; j < NA; j++) A[f][w][j] = A[f][w + 1][j];
+            taps_from<NW>(rq[s][f], k + 1 + NW, n1, rev[f], A[f][NA - 1]);
+        }
+        fetch(s, k + D);
+    };
+#pragma unroll
+    for (int s = 0; s < D; s++) fetch(s, s);
+    int k0 = 0;
+    for (; k0 + D <= n1; k0 += D) {
+#pragma unroll
+        for (int s = 0; s < D; s++) step(s, k0 + s);
+    }
+#pragma unroll
+    for (int s = 0; s < D; s++)
+        if (k0 + s < n1) step(s, k0 + s);
+}
+
 // Forward prediction with a precomputed factor F; y is per-thread scratch.
 template <int NW, int NF>
 SH_HD void solve_factored(const Acc *in, const Acc *q, const bool *rev, const float *F, long fs, long fplane,
@@ -531,6 +648,169 @@ SH_HD void solve_factored(const Acc *in, const Acc *q, const bool *rev, const fl
         nx[0] = t;
         out[k] = t;
     }
+}
+
+// solve_factored with its per-sample loads software-pipelined: the factor
+// values, the slope and the input sample that iteration k needs are loaded
+// at iteration k - D into a register ring (the sample loops are unrolled
+// by D so each ring slot is a fixed register), so a thread waits on memory
+// about once per D samples instead of every sample.  Ring loads clamp
+// their index into the trace (they are only consumed where the unclamped
+// index is valid).  The arithmetic is solve_factored's, operation for
+// operation, so the results are identical.
+template <int NW, int NF, int D>
+SH_HD void solve_factored_pf(const Acc *in, const Acc *q, const bool *rev, const float *F, long fs, long fplane,
+                             Sink y, int n1, float e, Sink out) {
+    constexpr int NA = 2 * NW + 1, NB = 2 * NW;
+    float tb[4];
+    if (NF == 1) {
+        tb[0] = in[0][0]; tb[1] = in[0][1]; tb[2] = in[0][n1 - 2]; tb[3] = in[0][n1 - 1];
+    } else {
+        tb[0] = (float)(0.5 * (in[0][0] + in[1][0]));
+        tb[1] = (float)(0.5 * (in[0][1] + in[1][1]));
+        tb[2] = (float)(0.5 * (in[0][n1 - 2] + in[1][n1 - 2]));
+        tb[3] = (float)(0.5 * (in[0][n1 - 1] + in[1][n1 - 1]));
+    }
+    float A[NF][NA][NA], I[NF][NA], T[NF][NA];
+    float Lo[NB][NB], Bw[NB];
+#pragma unroll
+    for (int f = 0; f < NF; f++) {
+#pragma unroll
+        for (int w = 0; w < NA; w++) taps_at<NW>(q[f], w - NW, n1, rev[f], A[f][w]);
+#pragma unroll
+        for (int w = 0; w < NA; w++) I[f][w] = (w < n1) ? in[f][w] : 0.0f;
+#pragma unroll
+        for (int w = 0; w < NA; w++) {
+            int kk = w - NW;
+            float t = 0.0f;
+            if (kk >= NW && kk < n1 - NW) {
+#pragma unroll
+                for (int j = 0; j < NA; j++) t += A[f][w][j] * ((kk + j - NW) < n1 ? in[f][kk + j - NW] : 0.0f);
+            }
+            T[f][w] = t;
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < NB; r++) {
+        Bw[r] = 0.0f;
+#pragma unroll
+        for (int m = 0; m < NB; m++) Lo[r][m] = 0.0f;
+    }
+    // forward ring: d[k], o[m][k], slope at k + 1 + NW, input at k + 1 + 2 NW
+    float rd[D], ro[D][NB], rq[D][NF], ri[D][NF];
+    auto fetch = [&](int s, int k) {
+        const long kc = k < n1 ? k : n1 - 1;
+        rd[s] = F[kc * fs];
+#pragma unroll
+        for (int m = 0; m < NB; m++) ro[s][m] = F[(long)(1 + m) * fplane + kc * fs];
+        const int kq = k + 1 + NW < n1 ? k + 1 + NW : n1 - 1;
+        const int ki = k + 1 + 2 * NW < n1 ? k + 1 + 2 * NW : n1 - 1;
+#pragma unroll
+        for (int f = 0; f < NF; f++) {
+            rq[s][f] = q[f][kq];
+            ri[s][f] = in[f][ki];
+        }
+    };
+    auto step = [&](int s, int k) {
+        float rhs = 0.0f;
+#pragma unroll
+        for (int f = 0; f < NF; f++) {
+            float of = 0.0f;
+#pragma unroll
+            for (int j = 0; j < NA; j++) {
+                int kk = k + j - NW;
+                if (kk >= NW && kk < n1 - NW) of += A[f][j][j] * T[f][j];
+            }
+            rhs = (f == 0) ? of : (rhs + of);
+        }
+        if (k == 0) rhs += e * tb[0];
+        if (k == 1) rhs += e * tb[1];
+        if (k == n1 - 2) rhs += e * tb[2];
+        if (k == n1 - 1) rhs += e * tb[3];
+        int m1 = k < NB ? k : NB;
+        float b = rhs;
+#pragma unroll
+        for (int m = 0; m < NB; m++)
+            if (m < m1) b -= (Lo[m][m]) * Bw[m];
+        float dk = rd[s];
+        y[k] = b / dk;
+        float ok[NB];
+#pragma unroll
+        for (int m = 0; m < NB; m++) ok[m] = ro[s][m];
+#pragma unroll
+        for (int r = NB - 1; r > 0; r--) {
+            Bw[r] = Bw[r - 1];
+#pragma unroll
+            for (int m = 0; m < NB; m++) Lo[r][m] = Lo[r - 1][m];
+        }
+        Bw[0] = b;
+#pragma unroll
+        for (int m = 0; m < NB; m++) Lo[0][m] = ok[m];
+#pragma unroll
+        for (int f = 0; f < NF; f++) {
+#pragma unroll
+            for (int w = 0; w < NA - 1; w++) {
+#pragma unroll
+                for (int j = 0; j < NA; j++) A[f][w][j] = A[f][w + 1][j];
+                I[f][w] = I[f][w + 1];
+                T[f][w] = T[f][w + 1];
+            }
+            taps_from<NW>(rq[s][f], k + 1 + NW, n1, rev[f], A[f][NA - 1]);
+            int kn = k + 1 + 2 * NW;
+            I[f][NA - 1] = (kn < n1) ? ri[s][f] : 0.0f;
+            int kk = k + 1 + NW;
+            float tn = 0.0f;
+            if (kk >= NW && kk < n1 - NW) {
+#pragma unroll
+                for (int j = 0; j < NA; j++) tn += A[f][NA - 1][j] * I[f][j];
+            }
+            T[f][NA - 1] = tn;
+        }
+        fetch(s, k + D);
+    };
+#pragma unroll
+    for (int s = 0; s < D; s++) fetch(s, s);
+    int k0 = 0;
+    for (; k0 + D <= n1; k0 += D) {
+#pragma unroll
+        for (int s = 0; s < D; s++) step(s, k0 + s);
+    }
+#pragma unroll
+    for (int s = 0; s < D; s++)
+        if (k0 + s < n1) step(s, k0 + s);
+    // backward ring: y[k], o[m][k]
+    float nx[NB];
+#pragma unroll
+    for (int m = 0; m < NB; m++) nx[m] = 0.0f;
+    float ry[D], rb[D][NB];
+    auto fetchb = [&](int s, int k) {
+        const long kc = k >= 0 ? k : 0;
+        ry[s] = y[kc];
+#pragma unroll
+        for (int m = 0; m < NB; m++) rb[s][m] = F[(long)(1 + m) * fplane + kc * fs];
+    };
+    auto stepb = [&](int s, int k) {
+        float t = ry[s];
+        int m1 = (n1 - k - 1) < NB ? (n1 - k - 1) : NB;
+#pragma unroll
+        for (int m = 0; m < NB; m++)
+            if (m < m1) t -= rb[s][m] * nx[m];
+#pragma unroll
+        for (int m = NB - 1; m > 0; m--) nx[m] = nx[m - 1];
+        nx[0] = t;
+        out[k] = t;
+        fetchb(s, k - D);
+    };
+#pragma unroll
+    for (int s = 0; s < D; s++) fetchb(s, n1 - 1 - s);
+    int kb = n1 - 1;
+    for (; kb - D + 1 >= 0; kb -= D) {
+#pragma unroll
+        for (int s = 0; s < D; s++) stepb(s, kb - s);
+    }
+#pragma unroll
+    for (int s = 0; s < D; s++)
+        if (kb - s >= 0) stepb(s, kb - s);
 }
 
 // M forward predictions that share the slopes, the directions and the

@@ -133,8 +133,9 @@ factor3_kernel(const float *__restrict__ pi, const float *__restrict__ px, float
     }
     constexpr int NB = 2 * NW;
     Sink Fs{F + (long)v * (NB + 1) * n + j, ntr};
-    if (nf == 1) factor_sweep<NW, 1>(q, rev, n1, e, Fs, n);
-    else factor_sweep<NW, 2>(q, rev, n1, e, Fs, n);
+    // slope loads pipelined 8 samples ahead (identical factor)
+    if (nf == 1) factor_sweep_pf<NW, 1, 8>(q, rev, n1, e, Fs, n);
+    else factor_sweep_pf<NW, 2, 8>(q, rev, n1, e, Fs, n);
 }
 
 // Block -> (trace block, node) of a level launch over a 1-D grid of
@@ -159,7 +160,7 @@ __device__ inline int level_block(int xcd, int cnt, long gx8, long &jb) {
 // one slot at one target trace with the shared factor of the target.  A
 // slot whose source lies outside the volume is written as zeros there
 // (sof3d_cfuns.c:1478-1503 skips it; the slot stays zero).
-template <int NW, int NF>
+template <int NW, int NF, int PF>
 __global__ void __launch_bounds__(TB)
 spray3_level(const float *__restrict__ pi, const float *__restrict__ px, float *__restrict__ U,
              const float *__restrict__ F, const Level3 nodes, int cnt, int xcd, int n1, int n2, int n3, int np2,
@@ -201,7 +202,10 @@ spray3_level(const float *__restrict__ pi, const float *__restrict__ px, float *
     }
     constexpr int NB = 2 * NW;
     const float *Fv = F + (long)node_variant(nd.update, nd.up2, nd.up3) * (NB + 1) * n + j;
-    solve_factored<NW, NF>(in, q, rev, Fv, ntr, n, y, n1, e, out);
+    if constexpr (PF > 0)
+        solve_factored_pf<NW, NF, PF>(in, q, rev, Fv, ntr, n, y, n1, e, out);
+    else
+        solve_factored<NW, NF>(in, q, rev, Fv, ntr, n, y, n1, e, out);
 }
 
 // Two-source slots grouped by (level, quadrant): every slot of a group
@@ -341,7 +345,7 @@ factor2_kernel(const float *__restrict__ dip, float *__restrict__ F, int n1, int
     Acc q[1] = {Acc{dip + i3 * slice + (v == 0 ? i : i - 1), n2}};
     bool rev[1] = {v == 1};
     Sink Fs{F + (long)v * (NB + 1) * n + i3 * slice + i, n2};
-    factor_sweep<NW, 1>(q, rev, n1, e, Fs, n);
+    factor_sweep_pf<NW, 1, 8>(q, rev, n1, e, Fs, n);
 }
 
 template <int NW>
@@ -375,7 +379,7 @@ spray2_fwd(const float *__restrict__ x, const float *__restrict__ dip, const flo
         Acc qs[1] = {q};
         bool rv[1] = {dir == 1};
         const float *Fv = F + (long)dir * (NB + 1) * n + i3 * slice + ip;
-        solve_factored<NW, 1>(in, qs, rv, Fv, n2, n, y, n1, e, out);
+        solve_factored_pf<NW, 1, 8>(in, qs, rv, Fv, n2, n, y, n1, e, out);
         cur = Acc{out.p, n2};
     }
 }
@@ -576,6 +580,13 @@ int spray3_run(sh_ctx *ctx, int order, int n1, int n2, int n3, int ns2, int ns3,
     // of the XCD-grouped one (same results, for timing)
     const char *xe = getenv("SH_SPRAY_XCD");
     const int xcd = (xe && xe[0] == '0') ? 0 : 1;
+    // SH_SPRAY_PF=d: prefetch distance (samples) of the level predictions,
+    // 4, 8 or 0 (no prefetch); identical results.  Default 8 for one source
+    // trace, 4 for two (more registers per sample; at 8 the kernel drops to
+    // one wave per SIMD).  Measured at 512x256x256 (somf3d, r = 4): two
+    // sources 14.4 / 9.1 / 12.0 ms and one source 6.3 / 4.7 / 4.4 ms for
+    // d = 0 / 4 / 8.
+    const char *pe = getenv("SH_SPRAY_PF");
     for (size_t L = 1; L < levels.size(); L++) {
         if (grouped) {  // two-source slots by (quadrant, group size)
             std::vector<Group3> groups;
@@ -630,17 +641,19 @@ int spray3_run(sh_ctx *ctx, int order, int n1, int n2, int n3, int ns2, int ns3,
             if (!cnt) continue;
             dim3 grid((unsigned)(gx8 * cnt));
             PROF_A(ctx);
+#define SL(NF_, PF_)                                                                                          \
+    SH_DISPATCH_NW(ctx, order, NW,                                                                            \
+                   hipLaunchKernelGGL((spray3_level<NW, NF_, PF_>), grid, dim3(TB), 0, ctx->stream, piT, pxT, U, \
+                                      F, lv, cnt, xcd, n1, n2, n3, np2, e, scr, nthr))
+            const int pf = pe ? atoi(pe) : (nf == 1 ? 8 : 4);
             if (nf == 1) {
-                SH_DISPATCH_NW(ctx, order, NW,
-                               hipLaunchKernelGGL((spray3_level<NW, 1>), grid, dim3(TB), 0, ctx->stream, piT, pxT, U,
-                                                  F, lv, cnt, xcd, n1, n2, n3, np2, e, scr, nthr));
+                if (pf == 8) { SL(1, 8); } else if (pf == 4) { SL(1, 4); } else { SL(1, 0); }
                 PROF_B(ctx, "spray3_level<1>", by);
             } else {
-                SH_DISPATCH_NW(ctx, order, NW,
-                               hipLaunchKernelGGL((spray3_level<NW, 2>), grid, dim3(TB), 0, ctx->stream, piT, pxT, U,
-                                                  F, lv, cnt, xcd, n1, n2, n3, np2, e, scr, nthr));
+                if (pf == 8) { SL(2, 8); } else if (pf == 4) { SL(2, 4); } else { SL(2, 0); }
                 PROF_B(ctx, "spray3_level<2>", by);
             }
+#undef SL
             SH_TRY(sh_launch_check(ctx, "spray3_level"));
         }
     }
