@@ -646,12 +646,16 @@ int spray3_run(sh_ctx *ctx, int order, int n1, int n2, int n3, int ns2, int ns3,
                    hipLaunchKernelGGL((spray3_level<NW, NF_, PF_>), grid, dim3(TB), 0, ctx->stream, piT, pxT, U, \
                                       F, lv, cnt, xcd, n1, n2, n3, np2, e, scr, nthr))
             const int pf = pe ? atoi(pe) : (nf == 1 ? 8 : 4);
+            // timer names carry the kernel's trailing template arguments
+            // (sources, prefetch distance)
             if (nf == 1) {
-                if (pf == 8) { SL(1, 8); } else if (pf == 4) { SL(1, 4); } else { SL(1, 0); }
-                PROF_B(ctx, "spray3_level<1>", by);
+                if (pf == 8) { SL(1, 8); PROF_B(ctx, "spray3_level<1,8>", by); }
+                else if (pf == 4) { SL(1, 4); PROF_B(ctx, "spray3_level<1,4>", by); }
+                else { SL(1, 0); PROF_B(ctx, "spray3_level<1,0>", by); }
             } else {
-                if (pf == 8) { SL(2, 8); } else if (pf == 4) { SL(2, 4); } else { SL(2, 0); }
-                PROF_B(ctx, "spray3_level<2>", by);
+                if (pf == 8) { SL(2, 8); PROF_B(ctx, "spray3_level<2,8>", by); }
+                else if (pf == 4) { SL(2, 4); PROF_B(ctx, "spray3_level<2,4>", by); }
+                else { SL(2, 0); PROF_B(ctx, "spray3_level<2,0>", by); }
             }
 #undef SL
             SH_TRY(sh_launch_check(ctx, "spray3_level"));
