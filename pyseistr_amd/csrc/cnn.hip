@@ -18,6 +18,19 @@
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
+// Raw buffer access (32-bit byte offsets against a resource of known size):
+// loads past the size return zeros and stores past it are dropped.
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+constexpr unsigned BUF_OOB = 0xFFFFE000u;  // offset past every buffer used here
+__device__ inline __amdgpu_buffer_rsrc_t buf_rsrc(const void *p, unsigned bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes, 0x00020000);
+}
+__device__ inline bf16x8 ld_frag(__amdgpu_buffer_rsrc_t r, unsigned off, bool ok) {
+    u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
+    u32x4 z = {0u, 0u, 0u, 0u};
+    return __builtin_bit_cast(bf16x8, ok ? v : z);
+}
+
 namespace {
 
 constexpr int C = 32, NT = 27, TB = 256;
@@ -67,12 +80,13 @@ __global__ void pack_weights(const float *__restrict__ P, __bf16 *__restrict__ W
 // ---------------------------------------------------------------------------
 // Layer 1 (1 -> 32), FP32 input, ReLU, BF16 output.
 // ---------------------------------------------------------------------------
+// The weights and biases are read with wave-uniform indices from the
+// read-only parameter vector, so they arrive through scalar loads and feed
+// the FMAs as scalar operands (no LDS traffic per FMA).
 __global__ void __launch_bounds__(TB) conv1_fwd(const float *__restrict__ x, float xscale, const float *__restrict__ P,
                                                 __bf16 *__restrict__ out, int n1, int n2, int n3, int dil) {
-    __shared__ float w[NT * C], b[C];
-    for (int i = threadIdx.x; i < NT * C; i += blockDim.x) w[i] = P[OFF_W1 + i];
-    for (int i = threadIdx.x; i < C; i += blockDim.x) b[i] = P[OFF_B1 + i];
-    __syncthreads();
+    const float *__restrict__ w = P + OFF_W1;
+    const float *__restrict__ b = P + OFF_B1;
     long n = (long)n1 * n2 * n3;
     long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= n) return;
@@ -92,7 +106,7 @@ __global__ void __launch_bounds__(TB) conv1_fwd(const float *__restrict__ x, flo
     for (int co = 0; co < C; co++) {
         float s = b[co];
 #pragma unroll
-        for (int t = 0; t < NT; t++) s += w[t * C + co] * xv[t];
+        for (int t = 0; t < NT; t++) s = fmaf(w[t * C + co], xv[t], s);
         o[co] = (__bf16)fmaxf(s, 0.0f);
     }
     bf16x8 *dst = (bf16x8 *)(out + v * C);
@@ -344,18 +358,6 @@ conv32w(const __bf16 *__restrict__ in, const __bf16 *__restrict__ Wg, const floa
     }
 }
 
-// Raw buffer access (32-bit byte offsets against a resource of known size):
-// loads past the size return zeros and stores past it are dropped.
-typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
-constexpr unsigned BUF_OOB = 0xFFFFE000u;  // offset past every buffer used here
-__device__ inline __amdgpu_buffer_rsrc_t buf_rsrc(const void *p, unsigned bytes) {
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes, 0x00020000);
-}
-__device__ inline bf16x8 ld_frag(__amdgpu_buffer_rsrc_t r, unsigned off, bool ok) {
-    u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
-    u32x4 z = {0u, 0u, 0u, 0u};
-    return __builtin_bit_cast(bf16x8, ok ? v : z);
-}
 // Row of A fragments through a buffer resource: o1 = byte offset of voxel
 // x0 + (lane & 31) of the row (+ the lane's channel half), BUF_OOB for a
 // row outside the volume (the hardware returns zeros); xm bit m*3+d clears
@@ -806,7 +808,8 @@ int cnn_forward(sh_ctx *ctx, const float *P, const float *x, float xscale, const
     hipLaunchKernelGGL(pack_weights, dim3(ceil_div(5 * per, TB)), dim3(TB), 0, ctx->stream, P, W.Wf, W.Wb);
     SH_TRY(sh_launch_check(ctx, "pack_weights"));
     PROF_A(ctx);
-    hipLaunchKernelGGL(conv1_fwd, dim3(ceil_div(n, TB)), dim3(TB), 0, ctx->stream, x, xscale, P, W.act[1], n1, n2, n3, D[0]);
+    hipLaunchKernelGGL(conv1_fwd, dim3(ceil_div(n, TB)), dim3(TB), 0, ctx->stream, x, xscale, P, W.act[1], n1, n2, n3,
+                       D[0]);
     PROF_B(ctx, "conv1_fwd", (double)n * (4.0 + 64.0));
     SH_TRY(sh_launch_check(ctx, "conv1_fwd"));
     __bf16 *cur = W.act[1];
