@@ -121,18 +121,41 @@ smooth_contig(const float *__restrict__ in, float *__restrict__ out, int n1, lon
     double acc[3] = {0.0, 0.0, 0.0};
     const long per = (ntask + gridDim.x - 1) / gridDim.x;
     const long t0 = (long)blockIdx.x * per, t1 = min(ntask, t0 + per);
+    // the next task's inputs are loaded into registers while the current
+    // task is smoothed (up to KS samples per thread: SM_CH1 + 2 h <= 766)
+    constexpr int KS = (SM_CH1 + 2 * SM_HMAX + 255) / 256;
+    constexpr int NR = (PRO == PRO_PLAIN) ? 1 : 5;
+    float raw[KS][NR];
+    long gi[KS];
+    bool own[KS];
+    auto fetch = [&](long task) {
+        const long base = (task / nch) * n1;
+        const int c0 = (int)(task % nch) * SM_CH1;
+        const int len = min(SM_CH1, n1 - c0);
+#pragma unroll
+        for (int k = 0; k < KS; k++) {
+            const int t = threadIdx.x + k * 256;
+            if (t < len + 2 * h) {
+                const int pos = c0 - h + t;
+                gi[k] = base + fold_idx(pos, n1);
+                own[k] = pos >= c0 && pos < c0 + len;
+                sm_fetch<PRO>(in, gi[k], A, pend, cur, raw[k]);
+            }
+        }
+    };
+    if (t0 < t1) fetch(t0);
     for (long task = t0; task < t1; task++) {
         long base = (task / nch) * n1;
         int c0 = (int)(task % nch) * SM_CH1;
         int len = min(SM_CH1, n1 - c0);
         __syncthreads();  // the previous task's readers are done with tile (and wts is ready)
-        for (int t = threadIdx.x; t < len + 2 * h; t += blockDim.x) {
-            int pos = c0 - h + t;
-            int q = fold_idx(pos, n1);
-            bool own = pos >= c0 && pos < c0 + len;
-            tile[t] = sm_load<PRO>(in, base + q, own, A, pend, a2, cur);
+#pragma unroll
+        for (int k = 0; k < KS; k++) {
+            const int t = threadIdx.x + k * 256;
+            if (t < len + 2 * h) tile[t] = sm_finish<PRO>(raw[k], gi[k], own[k], A, pend, a2, cur);
         }
         __syncthreads();
+        if (task + 1 < t1) fetch(task + 1);
         for (int t = threadIdx.x; t < len; t += blockDim.x) {
             const float *c = tile + t + h;
             float s = wts[0] * c[0];
@@ -143,52 +166,84 @@ smooth_contig(const float *__restrict__ in, float *__restrict__ out, int n1, lon
     sm_partials<EPI>(acc, part, gridDim.x, blockIdx.x, sh);
 }
 
-// Axes 2/3 (strided): task = 32 consecutive i1 x a chunk along the axis x
-// one position of the other strided axis (ntx x nty x nother tasks, i1
-// chunk fastest); index = i1 + o*ostride + q*stride.
+// Axes 2/3 (strided): task = tw = blockDim.x consecutive i1 x a chunk of chs
+// samples along the axis x one position of the other strided axis (ntx x
+// nty x nother tasks, i1 chunk fastest); index = i1 + o*ostride + q*stride.
 template <int PRO, int EPI>
 __global__ void __launch_bounds__(256)
 smooth_strided(const float *__restrict__ in, float *__restrict__ out, int n1, int naxis, long stride,
-               long ostride, int nother, int nb, CgArgs A, const DipScal *sc, double *part, int first) {
+               long ostride, int nother, int nb, int chs, CgArgs A, const DipScal *sc, double *part, int first) {
     if (sc->cg_done) return;
-    extern __shared__ float tile[];  // [(SM_CHS + 2h) rows][32]
+    extern __shared__ float tile[];  // [(chs + 2h) rows][tw]
     __shared__ float wts[SM_HMAX + 1];
     __shared__ double sh[16];
     int h = nb - 1;
-    int tx = threadIdx.x, ty = threadIdx.y;
+    const int tx = threadIdx.x, ty = threadIdx.y, tw = blockDim.x;
     bool pend = (PRO == PRO_CG || EPI == EPI_GP) ? (sc->pending != 0) : false;
     float a2 = pend ? (float)sc->alpha2 : 0.0f;
     float a1 = (EPI == EPI_S) ? (float)sc->alpha : 0.0f;
     int cur = (PRO == PRO_CG) ? sc->cur : 0;
     sm_weights(wts, nb);
-    const int ntx = (n1 + 31) / 32, nty = (naxis + SM_CHS - 1) / SM_CHS;
+    const int ntx = (n1 + tw - 1) / tw, nty = (naxis + chs - 1) / chs;
     const long ntask = (long)ntx * nty * nother;
     const long per = (ntask + gridDim.x - 1) / gridDim.x;
     const long t0 = (long)blockIdx.x * per, t1 = min(ntask, t0 + per);
     double acc[3] = {0.0, 0.0, 0.0};
+    // plain-in/plain-out passes: the next task's rows are loaded into
+    // registers while the current task is smoothed (up to KR rows per
+    // thread).  Not in the passes with a CG epilogue: their extra registers
+    // cost more occupancy than the prefetch gains (measured).
+    constexpr int KR = 18;
+    constexpr bool PIPE = PRO == PRO_PLAIN && EPI == EPI_PLAIN;
+    const bool pipe = PIPE && (chs + 2 * h + (int)blockDim.y - 1) / (int)blockDim.y <= KR;
+    float nxt[PIPE ? KR : 1];
+    auto fetch = [&](long task) {
+        const int bx = (int)(task % ntx);
+        const long rest = task / ntx;
+        const int by = (int)(rest % nty), bz = (int)(rest / nty);
+        const int i1 = bx * tw + tx;
+        const int c0 = by * chs, len = min(chs, naxis - c0);
+        const long base = (long)bz * ostride + i1;
+        if (!PIPE || i1 >= n1) return;
+#pragma unroll
+        for (int k = 0; k < (PIPE ? KR : 1); k++) {
+            const int row = ty + k * (int)blockDim.y;
+            if (row < len + 2 * h) nxt[k] = in[base + (long)fold_idx(c0 - h + row, naxis) * stride];
+        }
+    };
+    if (pipe && t0 < t1) fetch(t0);
     for (long task = t0; task < t1; task++) {
         int bx = (int)(task % ntx);
         long rest = task / ntx;
         int by = (int)(rest % nty), bz = (int)(rest / nty);
-        int i1 = bx * 32 + tx;
-        int c0 = by * SM_CHS;
-        int len = min(SM_CHS, naxis - c0);
+        int i1 = bx * tw + tx;
+        int c0 = by * chs;
+        int len = min(chs, naxis - c0);
         long base = (long)bz * ostride + i1;
         __syncthreads();
         if (i1 < n1) {
-            for (int row = ty; row < len + 2 * h; row += blockDim.y) {
-                int pos = c0 - h + row;
-                int q = fold_idx(pos, naxis);
-                bool own = pos >= c0 && pos < c0 + len;
-                tile[row * 32 + tx] = sm_load<PRO>(in, base + (long)q * stride, own, A, pend, a2, cur);
+            if (pipe) {
+#pragma unroll
+                for (int k = 0; k < (PIPE ? KR : 1); k++) {
+                    const int row = ty + k * (int)blockDim.y;
+                    if (row < len + 2 * h) tile[row * tw + tx] = nxt[k];
+                }
+            } else {
+                for (int row = ty; row < len + 2 * h; row += blockDim.y) {
+                    int pos = c0 - h + row;
+                    int q = fold_idx(pos, naxis);
+                    bool own = pos >= c0 && pos < c0 + len;
+                    tile[row * tw + tx] = sm_load<PRO>(in, base + (long)q * stride, own, A, pend, a2, cur);
+                }
             }
         }
         __syncthreads();
+        if (pipe && task + 1 < t1) fetch(task + 1);
         if (i1 < n1) {
             for (int row = ty; row < len; row += blockDim.y) {
-                const float *c = tile + (row + h) * 32 + tx;
+                const float *c = tile + (row + h) * tw + tx;
                 float s = wts[0] * c[0];
-                for (int k = 1; k <= h; k++) s += wts[k] * (c[-k * 32] + c[k * 32]);
+                for (int k = 1; k <= h; k++) s += wts[k] * (c[-k * tw] + c[k * tw]);
                 sm_store<EPI>(s, out, base + (long)(c0 + row) * stride, A, pend, a2, a1, first, acc);
             }
         }
@@ -435,14 +490,24 @@ int smooth_apply(sh_ctx *ctx, const DipGeom &g, const SmoothPlan &sp, const floa
             long stride = ax == 1 ? g.n1 : (long)g.n1 * g.n2;
             int nother = ax == 1 ? g.n3 : g.n2;
             long ostride = ax == 1 ? (long)g.n1 * g.n2 : g.n1;
-            long ntask = (long)ceil_div(g.n1, 32) * ceil_div(naxis, SM_CHS) * nother;
+            // tile: tw i1-samples x chs samples along the axis (+ halos);
+            // SH_SMOOTH_TILE=TWxCHS overrides (tw in {32, 64})
+            int tw = SM_TW, chs = SM_CHS;
+            if (const char *te = getenv("SH_SMOOTH_TILE")) {
+                int a = 0, b = 0;
+                if (sscanf(te, "%dx%d", &a, &b) == 2 && (a == 32 || a == 64) && b >= 8 && b <= 512) {
+                    tw = a;
+                    chs = b;
+                }
+            }
+            long ntask = (long)ceil_div(g.n1, tw) * ceil_div(naxis, chs) * nother;
             dim3 grid((unsigned)std::min<long>(ntask, DIP_GRID_CAP));
             nblk = grid.x;
-            size_t lds = (size_t)(SM_CHS + 2 * (nb - 1)) * 32 * sizeof(float);
+            size_t lds = (size_t)(chs + 2 * (nb - 1)) * tw * sizeof(float);
 #define SM_S(PP, EE)                                                                          \
     if (P == PP && E == EE)                                                                   \
-        hipLaunchKernelGGL((smooth_strided<PP, EE>), grid, dim3(32, 8), lds, ctx->stream, src, dst, g.n1, \
-                           naxis, stride, ostride, nother, nb, A, W.sc, W.part, first);
+        hipLaunchKernelGGL((smooth_strided<PP, EE>), grid, dim3(tw, 256 / tw), lds, ctx->stream, src, dst, g.n1, \
+                           naxis, stride, ostride, nother, nb, chs, A, W.sc, W.part, first);
             SM_S(0, 0) SM_S(0, 1) SM_S(0, 2) SM_S(1, 0) SM_S(1, 1) SM_S(1, 2)
 #undef SM_S
         }
@@ -576,6 +641,7 @@ extern "C" int sh_dip(sh_ctx *ctx, const float *data, const float *mask, const f
     nblk_max = std::max<long>(nblk_max, (long)ceil_div(n1, 32) * ceil_div(n2, SM_CHS) * n3);
     nblk_max = std::max<long>(nblk_max, (long)ceil_div(n1, 32) * ceil_div(n3, SM_CHS) * n2);
     nblk_max = std::max<long>(nblk_max, (long)ceil_div(n1, F_TI1) * ceil_div(n2, F_TI2) * n3);
+    nblk_max = std::max<long>(nblk_max, DIP_GRID_CAP);  // every capped launch, whatever its tile
     W.nblk_max = (int)nblk_max;
     SH_TRY(sh_slot(ctx, SLOT_RED, (size_t)nblk_max * 3 * sizeof(double), (void **)&W.part));
     SH_TRY(sh_slot(ctx, SLOT_SCAL, sizeof(DipScal), (void **)&W.sc));

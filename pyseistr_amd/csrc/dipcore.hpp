@@ -7,7 +7,8 @@
 namespace dipcore {
 
 constexpr int SM_CH1 = 512;   // samples of a line per block, axis-1 pass
-constexpr int SM_CHS = 128;   // samples along the axis per block, strided pass
+constexpr int SM_CHS = 64;    // samples along the axis per task, strided pass
+constexpr int SM_TW = 64;     // consecutive i1 samples per row of a strided pass
 constexpr int SM_HMAX = 127;  // largest supported smoothing half-width (rect-1)
 
 // Device-resident scalar state of one dip direction.
@@ -129,6 +130,40 @@ __device__ inline float sm_load(const float *in, long gi, bool own, const CgArgs
         }
     }
     return (-xv) + rv * A.w[gi];
+}
+
+// sm_load split in two for software pipelining: sm_fetch issues the loads
+// (raw values into registers), sm_finish applies the same arithmetic and
+// the own-sample writes later.
+template <int PRO>
+__device__ inline void sm_fetch(const float *in, long gi, const CgArgs &A, bool pend, int cur, float *raw) {
+    if (PRO == PRO_PLAIN) {
+        raw[0] = in[gi];
+        return;
+    }
+    raw[0] = (cur ? A.xb : A.xa)[gi];
+    raw[1] = (cur ? A.rb : A.ra)[gi];
+    raw[2] = A.w[gi];
+    if (pend) {
+        raw[3] = A.sx[gi];
+        raw[4] = A.sr[gi];
+    }
+}
+
+template <int PRO>
+__device__ inline float sm_finish(const float *raw, long gi, bool own, const CgArgs &A, bool pend, float a2,
+                                  int cur) {
+    if (PRO == PRO_PLAIN) return raw[0];
+    float xv = raw[0], rv = raw[1];
+    if (pend) {
+        xv = xv + a2 * raw[3];
+        rv = rv + a2 * raw[4];
+        if (own) {
+            (cur ? A.xa : A.xb)[gi] = xv;
+            (cur ? A.ra : A.rb)[gi] = rv;
+        }
+    }
+    return (-xv) + rv * raw[2];
 }
 
 template <int EPI>
