@@ -70,6 +70,14 @@ hipEvent_t sh_prof_event(sh_ctx *ctx) {
     return ctx->pool[ctx->pool_used++];
 }
 
+int sh_aux_stream(sh_ctx *ctx) {
+    if (ctx->aux) return SH_OK;
+    SH_HIP(ctx, hipStreamCreateWithFlags(&ctx->aux, hipStreamNonBlocking));
+    SH_HIP(ctx, hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
+    SH_HIP(ctx, hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming));
+    return SH_OK;
+}
+
 extern "C" {
 
 int sh_prof_enable(sh_ctx *ctx, int on) {
@@ -123,6 +131,7 @@ sh_ctx *sh_ctx_create(void *hip_stream) {
 int sh_ctx_trim(sh_ctx *ctx) {
     if (!ctx) return SH_ENULL;
     SH_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (ctx->aux) SH_HIP(ctx, hipStreamSynchronize(ctx->aux));
     for (int i = 0; i < SLOT_COUNT; i++) {
         if (ctx->slot_ptr[i]) (void)hipFree(ctx->slot_ptr[i]);
         ctx->slot_ptr[i] = nullptr;
@@ -135,6 +144,11 @@ void sh_ctx_destroy(sh_ctx *ctx) {
     if (!ctx) return;
     sh_ctx_trim(ctx);
     for (auto e : ctx->pool) (void)hipEventDestroy(e);
+    if (ctx->aux) {
+        (void)hipStreamDestroy(ctx->aux);
+        (void)hipEventDestroy(ctx->ev_fork);
+        (void)hipEventDestroy(ctx->ev_join);
+    }
     delete ctx;
 }
 

@@ -27,6 +27,10 @@ struct sh_prof_rec {
 
 struct sh_ctx {
     hipStream_t stream = nullptr;
+    // second stream (default priority) for work that runs beside the main
+    // stream inside one call, joined back before the call returns
+    hipStream_t aux = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     std::string err;
     std::vector<void *> slot_ptr;
     std::vector<size_t> slot_size;
@@ -46,6 +50,8 @@ hipEvent_t sh_prof_event(sh_ctx *ctx);
     (void)hipEventRecord(b_, (ctx)->stream); (ctx)->recs.push_back({(name), (double)(bytes), prof_a_, b_}); } } while (0)
 
 int sh_fail(sh_ctx *ctx, int code, const char *fmt, ...);
+// Create the context's second stream and its fork/join events (once).
+int sh_aux_stream(sh_ctx *ctx);
 
 #define SH_HIP(ctx, call)                                                        \
     do {                                                                         \

@@ -658,18 +658,54 @@ extern "C" int sh_dip(sh_ctx *ctx, const float *data, const float *mask, const f
         rep->ndir = ndir;
         rep->niter = niter;
     }
+    // The two directions of a 3-D estimate are independent Gauss-Newton
+    // problems: direction 1 runs on the context's second stream with its
+    // own workspace, beside direction 0, and is joined back at the end
+    // (SH_DIP_STREAMS=1: both on the main stream, one after the other).
+    const char *se = getenv("SH_DIP_STREAMS");
+    const bool two = ndir == 2 && !(se && se[0] == '1');
+    DipWork W2{};
+    if (two) {
+        SH_TRY(sh_aux_stream(ctx));
+        float *fbuf2 = nullptr;
+        SH_TRY(sh_slot(ctx, SLOT_DIP0 + 2, fb * NF, (void **)&fbuf2));
+        float **slots2[NF] = {&W2.p[0], &W2.p[1], &W2.u1, &W2.u2, &W2.w, &W2.xa, &W2.xb, &W2.ra, &W2.rb,
+                              &W2.sp, &W2.sx, &W2.sr, &W2.gp, &W2.pc, &W2.t1, &W2.t2, &W2.dp};
+        for (int q = 0; q < NF; q++) *slots2[q] = fbuf2 + (size_t)q * n;
+        W2.nblk_max = W.nblk_max;
+        SH_TRY(sh_slot(ctx, SLOT_DIP0 + 3, (size_t)nblk_max * 3 * sizeof(double), (void **)&W2.part));
+        SH_TRY(sh_slot(ctx, SLOT_DIP0 + 4, sizeof(DipScal), (void **)&W2.sc));
+        SH_HIP(ctx, hipMemsetAsync(W2.sc, 0, sizeof(DipScal), ctx->stream));
+        SH_HIP(ctx, hipEventRecord(ctx->ev_fork, ctx->stream));
+        SH_HIP(ctx, hipStreamWaitEvent(ctx->aux, ctx->ev_fork, 0));
+    }
+    // restores the main stream on every return; on an early (error) return
+    // the second stream is drained so no work of this call is left behind
+    struct StreamGuard {
+        sh_ctx *c;
+        hipStream_t main;
+        bool joined = false;
+        ~StreamGuard() {
+            if (c->stream != main) {
+                c->stream = main;
+                if (!joined && c->aux) (void)hipStreamSynchronize(c->aux);
+            }
+        }
+    } guard{ctx, ctx->stream};
     for (int d = 0; d < ndir; d++) {
+        DipWork &Wd = (two && d == 1) ? W2 : W;
+        if (two && d == 1) ctx->stream = ctx->aux;
         if (init)
-            SH_HIP(ctx, hipMemcpyAsync(W.p[0], init + (size_t)d * n, fb, hipMemcpyDeviceToDevice, ctx->stream));
+            SH_HIP(ctx, hipMemcpyAsync(Wd.p[0], init + (size_t)d * n, fb, hipMemcpyDeviceToDevice, ctx->stream));
         else {
-            hipLaunchKernelGGL(fill_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, ctx->stream, W.p[0], 0.0f, n);
+            hipLaunchKernelGGL(fill_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, ctx->stream, Wd.p[0], 0.0f, n);
             SH_TRY(sh_launch_check(ctx, "fill"));
         }
         int fin = 0;
         const unsigned char *m = mask ? (d == 0 ? W.m0 : W.m1) : nullptr;
-        SH_DISPATCH_NW(ctx, order, NW, SH_TRY(dip_direction<NW>(ctx, g, plan, d == 0 ? 2 : 3, data, m, niter, liter, W, &fin)));
-        SH_HIP(ctx, hipMemcpyAsync(dip + (size_t)d * n, W.p[fin], fb, hipMemcpyDeviceToDevice, ctx->stream));
-        if (rep) {
+        SH_DISPATCH_NW(ctx, order, NW, SH_TRY(dip_direction<NW>(ctx, g, plan, d == 0 ? 2 : 3, data, m, niter, liter, Wd, &fin)));
+        SH_HIP(ctx, hipMemcpyAsync(dip + (size_t)d * n, Wd.p[fin], fb, hipMemcpyDeviceToDevice, ctx->stream));
+        if (rep && !two) {
             DipScal hs;
             SH_HIP(ctx, hipMemcpyAsync(&hs, W.sc, sizeof hs, hipMemcpyDeviceToHost, ctx->stream));
             SH_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -677,6 +713,23 @@ extern "C" int sh_dip(sh_ctx *ctx, const float *data, const float *mask, const f
                 rep->cg_iters[d][it] = hs.rec[2 * it];
                 rep->ls_step[d][it] = hs.rec[2 * it + 1];
             }
+        }
+    }
+    if (two) {
+        SH_HIP(ctx, hipEventRecord(ctx->ev_join, ctx->aux));
+        ctx->stream = guard.main;
+        guard.joined = true;
+        SH_HIP(ctx, hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
+        if (rep) {
+            DipScal hs[2];
+            SH_HIP(ctx, hipMemcpyAsync(&hs[0], W.sc, sizeof(DipScal), hipMemcpyDeviceToHost, ctx->stream));
+            SH_HIP(ctx, hipMemcpyAsync(&hs[1], W2.sc, sizeof(DipScal), hipMemcpyDeviceToHost, ctx->stream));
+            SH_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            for (int d = 0; d < 2; d++)
+                for (int it = 0; it < niter && it < 64; it++) {
+                    rep->cg_iters[d][it] = hs[d].rec[2 * it];
+                    rep->ls_step[d][it] = hs[d].rec[2 * it + 1];
+                }
         }
     }
     return SH_OK;

@@ -212,6 +212,22 @@ def test_fused_smoothing_matches_separable(ps, monkeypatch):
     assert np.abs(a - b).max() <= 1e-5
 
 
+@pytest.mark.parametrize("tile", ["32x128", "64x128", "64x32"])
+def test_strided_smoothing_tiles_agree(ps, monkeypatch, tile):
+    """The strided smoothing passes give the same stencils for every tile
+    shape (SH_SMOOTH_TILE; 64x128 needs more rows per thread than the
+    prefetch ring holds, so it also covers the unpipelined path); only the
+    block partition of the double-precision CG sums differs, so the dips
+    agree with the default tiles to 1e-5 samples/trace."""
+    from pyseistr_amd import synth
+    v, _ = synth.volume3d((96, 40, 36), seed=9, n_events=3, smax=2.0)
+    a = np.stack(ps.dip3dc(v, niter=3, liter=10, rect=[5, 5, 5]))
+    monkeypatch.setenv("SH_SMOOTH_TILE", tile)
+    b = np.stack(ps.dip3dc(v, niter=3, liter=10, rect=[5, 5, 5]))
+    monkeypatch.delenv("SH_SMOOTH_TILE")
+    assert np.abs(a - b).max() <= 1e-5
+
+
 def test_dip3d_long_run_matches_oracle(ps):
     """Thirty Gauss-Newton iterations (the slow drift of the reference's
     heavily damped update, not a device artefact): the dips stay within the
