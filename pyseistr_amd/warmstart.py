@@ -104,7 +104,10 @@ class WarmStart:
         self._dev = None
 
     @classmethod
-    def load(cls, path=WEIGHTS):
+    def load(cls, path=None):
+        """Weights from `path`, else from $SH_WARMSTART_WEIGHTS, else the
+        committed pyseistr_amd/weights/warmstart.npz."""
+        path = path or os.environ.get("SH_WARMSTART_WEIGHTS") or WEIGHTS
         with np.load(path, allow_pickle=False) as z:
             meta = json.loads(str(z["meta"])) if "meta" in z else {}
             return cls(z["params"], meta)
