@@ -184,8 +184,12 @@ def main():
 
     info = {}
     net = None
-    if os.path.exists(warmstart.WEIGHTS):
-        net = warmstart.WarmStart.load()
+    wpath = os.environ.get("SH_WARMSTART_WEIGHTS") or warmstart.WEIGHTS
+    if os.path.exists(wpath):
+        net = warmstart.WarmStart.load(wpath)
+        # which network: its training set and schedule
+        info["warm_start_net"] = {k: net.meta.get(k) for k in ("target_niter", "volumes", "windows", "steps")
+                                  if k in net.meta}
     init_buf = torch.empty(2 * n, dtype=torch.float32, device="cuda")
     nc, nw = args.cold_niter, args.warm_niter
 

@@ -256,6 +256,8 @@ def main(argv=None):
     ap.add_argument("--dilation", type=str, default="1,1,1,1,1,1", help="per-layer dilations")
     ap.add_argument("--window", type=int, default=64, help="cube edge of the training windows")
     ap.add_argument("--margin", type=int, default=4, help="voxels at each window face left out of the loss")
+    ap.add_argument("--init", type=str, default=None,
+                    help="continue from these weights (npz) instead of a fresh initialisation")
     a = ap.parse_args(argv)
 
     def log(msg):
@@ -271,12 +273,18 @@ def main(argv=None):
     pool.close()
     log(f"dataset ready: {len(data)} windows in {time.time() - t0:.0f} s")
     dil = [int(v) for v in a.dilation.split(",")]
-    params, hist = train(data, a.steps, lr=a.lr, log=log, dil=dil, win=win, margin=a.margin)
+    prior = WarmStart.load(a.init) if a.init else None
+    if prior is not None and prior.dil != dil:
+        raise SystemExit("--init weights have dilations %s, not %s" % (prior.dil, dil))
+    params, hist = train(data, a.steps, lr=a.lr, log=log, dil=dil, win=win, margin=a.margin,
+                         params=prior.params_host if prior is not None else None)
     meta = {"volumes": a.volumes, "vol_shape": vol_shape, "seeds": [a.seed0, a.seed0 + a.volumes - 1],
             "windows": len(data), "per_vol": a.per_vol, "target_niter": a.target_niter, "steps": a.steps,
             "lr": a.lr, "dilation": dil, "window": a.window, "margin": a.margin,
             "final_loss": hist[-1][1] if hist else None,
             "loss_history": hist[::max(1, len(hist) // 50)]}
+    if prior is not None:  # the training this run continued
+        meta["continued_from"] = {k: v for k, v in prior.meta.items() if k != "loss_history"}
     ws = WarmStart(params, meta)
     ws.save(a.out)
     log(f"saved {a.out}; total {time.time() - t0:.0f} s")
