@@ -15,6 +15,9 @@ rect=5x5x5; somf3d with r1=r2=4, order=2 (median of 2*r1*r2+1 = 33 slots).
 4. Timed: --steps steps of [warm start + somf3d], then --steps of
    [cold start + somf3d], each between synchronisations with inputs
    resident in HBM.
+5. Per-kernel profile: one more pass of the warm steps with the library's
+   HIP-event timers (outside the timed region), dip3d's two directions
+   serialized so that overlapping streams do not stretch kernel times.
 Also reported: the RMSE of the reference's default cold start (niter=5),
 how many cold iterations reach --target-rmse of p_conv, and the lowest error
 the warm start reaches.
@@ -22,9 +25,9 @@ the warm start reaches.
 Prints one JSON line: value = samples/s of the warm path (the cold
 5-iteration path when no trained network is present), the cold rate at the
 same error and the speed-up in config, the roofline of the dominant plain
-kernel from live HIP-event timing, the batched 2-D side measurement, and the
-CPU baseline = the reference's own C extension on a bounded sample, one host
-core.
+kernel from the HIP-event timing of step 5, the batched 2-D side
+measurement, and the CPU baseline = the reference's own C extension on a
+bounded sample, one host core.
 """
 import argparse
 import json
@@ -274,13 +277,27 @@ def main():
             _lib.profile_enable(False)
         return ev0.elapsed_time(ev1) / steps, wall, prof
 
-    ms_step, wall, prof = timed(step, args.steps, True)
+    ms_step, wall, _ = timed(step, args.steps, False)
     value = n / (ms_step / 1e3)
     if use_warm:
         cold_ms, _, _ = timed(cold_step, args.steps, False)
         info["cold_ms_per_step"] = round(cold_ms, 3)
         info["cold_samples_per_s"] = round(n / (cold_ms / 1e3), 1)
         info["speedup_warm_vs_cold"] = round(cold_ms / ms_step, 3)
+    # Per-kernel times: a separate pass of the same steps with the library's
+    # HIP-event timers, outside the timed region.  dip3d's two directions
+    # run on two streams; their kernels overlap and would each look slower,
+    # so this pass runs the directions one after the other.
+    prev = os.environ.get("SH_DIP_STREAMS")
+    os.environ["SH_DIP_STREAMS"] = "1"
+    try:
+        _, _, prof = timed(step, args.steps, True)
+    finally:
+        if prev is None:
+            del os.environ["SH_DIP_STREAMS"]
+        else:
+            os.environ["SH_DIP_STREAMS"] = prev
+    info["kernel_profile"] = "separate pass, dip directions serialized"
 
     # dominant plain (HBM-bound) kernel by total time; achieved =
     # algorithmic bytes / time.  The network's convolutions are MFMA work and
