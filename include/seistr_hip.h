@@ -64,7 +64,11 @@ typedef struct {
  * or 2N): initial dip instead of the reference's zeros (the commented
  * idip/xdip inputs, dip_cfuns.c:1861-1875).  eps_dv is accepted and has no
  * effect, as in the reference (its shared `eps` static is reset to 1).
- * rep (nullable, host memory): iteration record; filling it synchronises. */
+ * rep (nullable, host memory): iteration record; filling it synchronises.
+ * With n3 > 1 the two directions are independent: the crossline one runs
+ * on a second stream owned by the context, forked from and joined back to
+ * the context's stream inside the call, so every result is ordered on the
+ * caller's stream as usual (environment SH_DIP_STREAMS=1: one stream). */
 int sh_dip(sh_ctx *ctx, const float *data, const float *mask, const float *init,
            int n1, int n2, int n3, int niter, int liter, int order, float eps_dv,
            int r1, int r2, int r3, float *dip, sh_dip_report *rep);

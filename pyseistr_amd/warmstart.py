@@ -177,7 +177,7 @@ def make_dataset(seeds, vol_shape, win, per_vol, target_niter, liter=10, rect=(5
     wn = w1 * w2 * w3
     out = []
     rng = np.random.default_rng(12345)
-    t0 = time.time()
+    t0 = t_log = time.time()
     jobs = [(s, vol_shape) for s in seeds]
     gen = pool.imap(_gen, jobs, chunksize=2) if pool is not None else map(_gen, jobs)
     for k, v in enumerate(gen):
@@ -197,7 +197,8 @@ def make_dataset(seeds, vol_shape, win, per_vol, target_niter, liter=10, rect=(5
             t = torch.cat([di[sl].contiguous().view(-1), dx[sl].contiguous().view(-1)])
             assert x.numel() == wn
             out.append((x, t, xs))
-        if (k + 1) % 25 == 0:
+        if (k + 1) % 25 == 0 or time.time() - t_log > 30:
+            t_log = time.time()
             log(f"dataset: {k + 1}/{len(seeds)} volumes, {len(out)} windows, {time.time() - t0:.0f} s")
     return out
 
