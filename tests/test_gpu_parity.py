@@ -212,6 +212,29 @@ def test_fused_smoothing_matches_separable(ps, monkeypatch):
     assert np.abs(a - b).max() <= 1e-5
 
 
+def test_dip_directions_on_two_streams_identical(ps, monkeypatch):
+    """dip3d runs its two directions side by side (the crossline one on the
+    context's second stream, with its own workspace); they are independent,
+    so dips and iteration records are bit-identical to running them one
+    after the other on one stream (SH_DIP_STREAMS=1), with a mask and a
+    warm start as well; a following call on the same context sees the
+    joined results."""
+    from pyseistr_amd import synth
+    v, _ = synth.volume3d((80, 30, 28), seed=12, n_events=3, smax=2.0, spike_frac=0.01)
+    mask = (np.random.default_rng(4).random(v.shape) > 0.1).astype(np.float32)
+    init = (0.3 * np.ones(v.shape, np.float32), -0.2 * np.ones(v.shape, np.float32))
+    kw = dict(niter=3, liter=10, rect=[5, 5, 5], report=True)
+    a, ra = ps.dip3dc(v, mask=mask, init=init, **kw)
+    a2, _ = ps.dip3dc(v, **kw)
+    monkeypatch.setenv("SH_DIP_STREAMS", "1")
+    b, rb = ps.dip3dc(v, mask=mask, init=init, **kw)
+    b2, _ = ps.dip3dc(v, **kw)
+    monkeypatch.delenv("SH_DIP_STREAMS")
+    for x, y in ((a, b), (a2, b2)):
+        assert np.array_equal(x[0], y[0]) and np.array_equal(x[1], y[1])
+    assert ra == rb
+
+
 @pytest.mark.parametrize("tile", ["32x128", "64x128", "64x32"])
 def test_strided_smoothing_tiles_agree(ps, monkeypatch, tile):
     """The strided smoothing passes give the same stencils for every tile
