@@ -166,8 +166,8 @@ def measured_traffic(name, path=None):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(multi-GPU sharding is design-only, see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py: the benchmark runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the path is not sharded over several GPUs)", file=sys.stderr)
         return 2
     import numpy as np
     import torch

@@ -83,7 +83,7 @@ def load():
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
-            raise SeistrError(f"{LIB_PATH} is not built; run __graft_entry__.build() "
+            raise SeistrError(f"{LIB_PATH} is not built; run pyseistr_amd/csrc/build.sh "
                               "(hipcc --offload-arch=gfx950)")
         lib = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in _SIGS.items():

@@ -5,8 +5,8 @@ dipc, pyseistr/src/dip_cfuns.c:1694) needs fewer iterations.
 The network runs entirely in libseistr_hip.so (sh_cnn_forward / sh_cnn_grad /
 sh_cnn_adam: BF16 3x3x3 convolutions on the matrix cores, FP32 accumulation);
 torch only holds the device buffers.  The plain cold-start dip3d stays the
-fallback and the judge of the result: the prediction is only an ``init`` for
-sh_dip, whose iterations still decide the answer.
+fallback and the measure of the result: the prediction is only an ``init``
+for sh_dip, whose iterations still decide the answer.
 
 Training (``python -m pyseistr_amd.warmstart train``): seeded synthetic
 volumes with 2-5 curved Ricker events, slopes within +-3 samples/trace and
