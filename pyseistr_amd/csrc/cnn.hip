@@ -131,6 +131,20 @@ __global__ void __launch_bounds__(TB) conv1_fwd(const float *__restrict__ x, flo
 constexpr int CW = 8;          // waves per conv block (one i2 row of the tile each)
 constexpr int CTB = 64 * CW;   // threads per conv block
 
+// B operands in LDS: 16-byte entries [tap][h][hi][co] (h: which 8 of the
+// lane's 16 input channels, hi = lane / 32, co = lane % 32), so the 64
+// lanes of one fragment read cover 1 KB contiguously and hit distinct
+// banks.  (Entries in [tap][co][ci] order put lanes co, co + 4, ... 64 B
+// apart on the same banks; at 512x256x256 the layers ran equally fast
+// either way, so bank conflicts are not what bounds them.)
+__device__ inline void stage_weights(bf16x8 *wl, const __bf16 *__restrict__ Wg) {
+    for (int i = threadIdx.x; i < NT * C * C / 8; i += blockDim.x) {
+        const int co = i % 32, hi = (i / 32) % 2, h = (i / 64) % 2, t = i / 128;
+        wl[i] = ((const bf16x8 *)Wg)[((t * C + co) * C + hi * 16 + h * 8) / 8];
+    }
+}
+__device__ inline int wl_at(int t, int h, int hi, int co) { return ((t * 2 + h) * 2 + hi) * 32 + co; }
+
 // A fragments of one (d2, d3) row of taps: [d1][m][half]
 struct RowFrag {
     bf16x8 a[3][2][2];
@@ -161,8 +175,8 @@ template <int MODE>
 __global__ void __launch_bounds__(CTB)
 conv32(const __bf16 *__restrict__ in, const __bf16 *__restrict__ Wg, const float *__restrict__ bias,
        void *__restrict__ outp, const __bf16 *__restrict__ aux, int n1, int n2, int n3, int dil) {
-    __shared__ bf16x8 wl[NT * C * C / 8];  // [27][32 co][32 ci]
-    for (int i = threadIdx.x; i < NT * C * C / 8; i += blockDim.x) wl[i] = ((const bf16x8 *)Wg)[i];
+    __shared__ bf16x8 wl[NT * C * C / 8];  // [27][h][hi][32 co] entries of 8 ci
+    stage_weights(wl, Wg);
     __syncthreads();
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int ntx = (n1 + 63) / 64, nty = (n2 + CW - 1) / CW;
@@ -203,8 +217,8 @@ conv32(const __bf16 *__restrict__ in, const __bf16 *__restrict__ Wg, const float
 #pragma unroll
             for (int d = 0; d < 3; d++) {
                 const int t = row * 3 + d;
-                const bf16x8 b0 = wl[((t * C + co) * C + hi * 16) / 8];
-                const bf16x8 b1 = wl[((t * C + co) * C + hi * 16 + 8) / 8];
+                const bf16x8 b0 = wl[wl_at(t, 0, hi, co)];
+                const bf16x8 b1 = wl[wl_at(t, 1, hi, co)];
                 // independent accumulators back to back (no dependent MFMA
                 // pair); each accumulator still sums half 0 then half 1
 #pragma unroll
@@ -283,8 +297,8 @@ template <int MODE, int DIL>
 __global__ void __launch_bounds__(CTB)
 conv32w(const __bf16 *__restrict__ in, const __bf16 *__restrict__ Wg, const float *__restrict__ bias,
         void *__restrict__ outp, const __bf16 *__restrict__ aux, int n1, int n2, int n3) {
-    __shared__ bf16x8 wl[NT * C * C / 8];  // [27][32 co][32 ci]
-    for (int i = threadIdx.x; i < NT * C * C / 8; i += blockDim.x) wl[i] = ((const bf16x8 *)Wg)[i];
+    __shared__ bf16x8 wl[NT * C * C / 8];  // [27][h][hi][32 co] entries of 8 ci
+    stage_weights(wl, Wg);
     __syncthreads();
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int ntx = (n1 + 127) / 128, nty = (n2 + CW - 1) / CW;
@@ -316,8 +330,8 @@ conv32w(const __bf16 *__restrict__ in, const __bf16 *__restrict__ Wg, const floa
 #pragma unroll
             for (int d = 0; d < 3; d++) {
                 const int t = row * 3 + d;
-                const bf16x8 b0 = wl[((t * C + co) * C + hi * 16) / 8];
-                const bf16x8 b1 = wl[((t * C + co) * C + hi * 16 + 8) / 8];
+                const bf16x8 b0 = wl[wl_at(t, 0, hi, co)];
+                const bf16x8 b1 = wl[wl_at(t, 1, hi, co)];
 #pragma unroll
                 for (int m = 0; m < 4; m++)
                     acc[m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f.a[m + (d - 1) * DIL + DIL][0], b0, acc[m], 0, 0, 0);
@@ -391,8 +405,8 @@ template <int MODE>
 __global__ void __launch_bounds__(CTB)
 conv32z(const __bf16 *__restrict__ in, const __bf16 *__restrict__ Wg, const float *__restrict__ bias,
         void *__restrict__ outp, const __bf16 *__restrict__ aux, int n1, int n2, int n3) {
-    __shared__ bf16x8 wl[NT * C * C / 8];  // [27][32 co][32 ci]
-    for (int i = threadIdx.x; i < NT * C * C / 8; i += blockDim.x) wl[i] = ((const bf16x8 *)Wg)[i];
+    __shared__ bf16x8 wl[NT * C * C / 8];  // [27][h][hi][32 co] entries of 8 ci
+    stage_weights(wl, Wg);
     __syncthreads();
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int ntx = (n1 + 63) / 64, nty = (n2 + CW - 1) / CW;
@@ -443,8 +457,8 @@ conv32z(const __bf16 *__restrict__ in, const __bf16 *__restrict__ Wg, const floa
 #pragma unroll
             for (int d = 0; d < 3; d++) {
                 const int t = row * 3 + d;
-                const bf16x8 b0 = wl[((t * C + co) * C + hi * 16) / 8];
-                const bf16x8 b1 = wl[((t * C + co) * C + hi * 16 + 8) / 8];
+                const bf16x8 b0 = wl[wl_at(t, 0, hi, co)];
+                const bf16x8 b1 = wl[wl_at(t, 1, hi, co)];
 #pragma unroll
                 for (int m = 0; m < 2; m++)
                     acc[m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f.a[d][m][0], b0, acc[m], 0, 0, 0);
