@@ -73,6 +73,15 @@ int sh_dip(sh_ctx *ctx, const float *data, const float *mask, const float *init,
            int n1, int n2, int n3, int niter, int liter, int order, float eps_dv,
            int r1, int r2, int r3, float *dip, sh_dip_report *rep);
 
+/* Triangle (or box) smoothing with optional derivative, per axis whose
+ * radius r exceeds 1, axis 1 first, each line `repeat` times.
+ * Replaces dipcfun.smoothcf (pyseistr/src/dip_cfuns.c:2006-2123), the C
+ * entry behind smoothc (pyseistr/smooth.py:115); adj != 0 applies
+ * ps_smooth, adj == 0 ps_smooth2 (dip_cfuns.c:594-625), as there.
+ * Bit-identical to the reference.  out may equal in. */
+int sh_smooth(sh_ctx *ctx, const float *in, float *out, int n1, int n2, int n3, int repeat, int adj,
+              int r1, int r2, int r3, int diff1, int diff2, int diff3, int box1, int box2, int box3);
+
 /* 3-D plane-wave spray, slots visited by distance from the source trace.
  * Replaces the spray loop of csomean3d/csomf3d (pyseistr/src/
  * sof3d_cfuns.c:1639-1705) and the pure pwspray3d (pyseistr/pwspray3d.py:2).

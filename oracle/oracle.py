@@ -95,6 +95,14 @@ def trianglen(x, shape, rect, dtype=np.float32):
     return x
 
 
+def smoothc(x, shape, rect=(1, 1, 1), diff=(0, 0, 0), box=(0, 0, 0), repeat=1, adj=1, dtype=np.float32):
+    """smoothcf (dip_cfuns.c:2006): flat F-order in and out."""
+    x = _arr(x, dtype).copy()
+    iv = lambda v: (ctypes.c_int * 3)(*(int(a) for a in v))  # noqa: E731
+    _lib(dtype).or_smoothc(*(int(v) for v in shape), int(repeat), int(adj), iv(rect), iv(diff), iv(box), _ptr(x))
+    return x
+
+
 def dip(din, shape, niter=5, liter=20, order=2, eps_dv=0.01, rect=(1, 1, 1), mask=None,
         p_init=None, dtype=np.float32, report=False):
     """dipc: returns flat dips (N for n3==1, else 2N = inline || crossline)."""

@@ -38,13 +38,17 @@ def _mod(name):
 @contextlib.contextmanager
 def _quiet():
     """The reference prints from C (printf); silence fd 1 around calls."""
+    import ctypes
     sys.stdout.flush()
+    libc = ctypes.CDLL(None)
+    libc.fflush(None)
     saved = os.dup(1)
     null = os.open(os.devnull, os.O_WRONLY)
     os.dup2(null, 1)
     try:
         yield
     finally:
+        libc.fflush(None)  # C stdio buffers the reference's printf output
         os.dup2(saved, 1)
         os.close(saved)
         os.close(null)
@@ -52,6 +56,17 @@ def _quiet():
 
 def _f(x):
     return np.ascontiguousarray(np.asarray(x, dtype=np.float32).ravel(order="F"))
+
+
+def smoothcf(din, shape, rect=(1, 1, 1), diff=(0, 0, 0), box=(0, 0, 0), repeat=1, adj=1):
+    """dipcfun.smoothcf, format "Oiiiiiiiiiiiiii" = (din, n1, n2, n3, repeat,
+    adj, r1, r2, r3, diff1, diff2, diff3, box1, box2, box3)
+    (dip_cfuns.c:2021)."""
+    n1, n2, n3 = shape
+    with _quiet():
+        out = _mod("dipcfun").smoothcf(_f(din), n1, n2, n3, int(repeat), int(adj), *(int(r) for r in rect),
+                                       *(int(v) for v in diff), *(int(v) for v in box))
+    return np.asarray(out, np.float32).ravel()
 
 
 def dipc(din, shape, niter=5, liter=20, order=2, eps_dv=0.01, rect=(1, 1, 1), mask=None):

@@ -282,6 +282,89 @@ EXPORT void or_trianglen(int n1, int n2, int n3, int r1, int r2, int r3, real *x
 }
 
 /* ------------------------------------------------------------------ */
+/* Stand-alone triangle / box smoothing with derivative and repeat:    */
+/* smoothcf (dip_cfuns.c:2006-2123) over ps_smooth / ps_smooth2        */
+/* (dip_cfuns.c:594-625) with fold, doubint, triple and their adjoints */
+/* (433-592); the object's weight from ps_triangle_init (403-427).     */
+/* ------------------------------------------------------------------ */
+
+/* adj != 0: ps_smooth = fold -> doubint -> triple; adj == 0: ps_smooth2 =
+ * triple2 -> doubint2 -> fold2.  der: derivative; box: box, not triangle. */
+static void smooth_line(int nb, int nx, int box, int der, int adj, real *x, size_t o, size_t d, real *tmp) {
+    int np = nx + 2 * nb;
+    real wt = box ? (real)(1.0 / (2 * nb - 1)) : (real)(1.0 / (nb * nb)); /* 420-424 */
+    int one = box || der;                                               /* doubint stops after one sum */
+    if (adj) {
+        /* fold (433-456): the even-reflection extension, a plain copy */
+        for (int e = 0; e < np; e++) tmp[e] = x[o + (size_t)fold_index(e - nb, nx) * d];
+        /* doubint (486-507): backward running sum, then forward */
+        real t = 0;
+        for (int i = np - 1; i >= 0; i--) { t += tmp[i]; tmp[i] = t; }
+        if (!one) {
+            t = 0;
+            for (int i = 0; i < np; i++) { t += tmp[i]; tmp[i] = t; }
+        }
+        /* triple (532-552): the triangle case is evaluated in double */
+        for (int i = 0; i < nx; i++) {
+            if (box) x[o + i * d] = (tmp[i + 1] - tmp[i + 2 * nb]) * wt;
+            else x[o + i * d] = (real)((2. * tmp[i + nb] - tmp[i] - tmp[i + 2 * nb]) * wt);
+        }
+    } else {
+        /* triple2 (567-586): scaled copies added into zeros, in saxpy order */
+        for (int i = 0; i < np; i++) tmp[i] = 0;
+        if (box) {
+            real mw = -wt;
+            for (int i = 0; i < nx; i++) tmp[i + 1] += wt * x[o + i * d];
+            for (int i = 0; i < nx; i++) tmp[i + 2 * nb] += mw * x[o + i * d];
+        } else {
+            real mw = -wt, w2 = (real)(2. * wt);
+            for (int i = 0; i < nx; i++) tmp[i] += mw * x[o + i * d];
+            for (int i = 0; i < nx; i++) tmp[i + nb] += w2 * x[o + i * d];
+            for (int i = 0; i < nx; i++) tmp[i + 2 * nb] += mw * x[o + i * d];
+        }
+        /* doubint2 (509-530): forward running sum, then backward */
+        real t = 0;
+        for (int i = 0; i < np; i++) { t += tmp[i]; tmp[i] = t; }
+        if (!one) {
+            t = 0;
+            for (int i = np - 1; i >= 0; i--) { t += tmp[i]; tmp[i] = t; }
+        }
+        /* fold2 (458-484): middle, right tail ascending, left tail descending */
+        for (int i = 0; i < nx; i++) x[o + i * d] = tmp[i + nb];
+        for (int e = nb + nx; e < np; e++) x[o + (size_t)fold_index(e - nb, nx) * d] += tmp[e];
+        for (int e = nb - 1; e >= 0; e--) x[o + (size_t)fold_index(e - nb, nx) * d] += tmp[e];
+    }
+}
+
+/* smoothcf's loops (2090-2110): every axis whose radius exceeds 1, axis 1
+ * first; each line is smoothed `repeat` times before the next.  (The
+ * reference's reshaping of n1/n2 by dim1 only groups the same lines.)
+ * In place. */
+EXPORT void or_smoothc(int n1, int n2, int n3, int repeat, int adj, const int *rect, const int *diff,
+                       const int *box, real *x) {
+    int n[3] = {n1, n2, n3};
+    size_t s[3] = {1, (size_t)n1, (size_t)n1 * n2};
+    int mx = 0, rmax = 0;
+    for (int a = 0; a < 3; a++) {
+        if (n[a] > mx) mx = n[a];
+        if (rect[a] > rmax) rmax = rect[a];
+    }
+    real *tmp = xcalloc(mx + 2 * rmax + 2, sizeof(real));
+    for (int ax = 0; ax < 3; ax++) {
+        if (rect[ax] <= 1) continue;
+        for (int a = 0; a < (ax == 0 ? n2 : n1); a++)
+            for (int c = 0; c < (ax == 2 ? n2 : n3); c++) {
+                size_t o;
+                if (ax == 0) o = (size_t)a * s[1] + (size_t)c * s[2];
+                else if (ax == 1) o = (size_t)a + (size_t)c * s[2];
+                else o = (size_t)a + (size_t)c * s[1];
+                for (int r = 0; r < repeat; r++) smooth_line(rect[ax], n[ax], box[ax], diff[ax], adj, x, o, s[ax], tmp);
+            }
+    }
+    free(tmp);
+}
+
+/* ------------------------------------------------------------------ */
 /* Reductions with the reference's accumulation types.                 */
 /* ps_cblas_dsdot (double), ps_cblas_snrm2 (float sum of squares).     */
 /* dip_cfuns.c:122-209.                                                */

@@ -41,6 +41,7 @@ _SIGS = {
     "sh_prof_report": (c_int, [c_void_p, ctypes.c_char_p, c_int]),
     "sh_dip": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_int, c_int,
                        c_float, c_int, c_int, c_int, c_void_p, ctypes.POINTER(DipReport)]),
+    "sh_smooth": (c_int, [c_void_p, c_void_p, c_void_p] + [c_int] * 14),
     "sh_pwspray3d": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_int,
                              c_int, c_float, c_void_p]),
     "sh_somean3d": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_int,

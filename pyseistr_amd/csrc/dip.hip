@@ -342,6 +342,70 @@ smooth_fused(const float *__restrict__ in, float *__restrict__ out, int n1, int 
     sm_partials<EPI>(acc, part, gridDim.x * gridDim.y, blockIdx.x + gridDim.x * blockIdx.y, sh);
 }
 
+// ---------------------------------------------------------------------------
+// Stand-alone smoothing (smoothcf, dip_cfuns.c:2006-2123): every line along
+// axis `ax` smoothed `repeat` times by ps_smooth (adj: fold -> doubint ->
+// triple) or ps_smooth2 (triple2 -> doubint2 -> fold2), with the box and
+// derivative options (dip_cfuns.c:403-625).  One thread per line runs the
+// reference's loops in its order of float (and, in triple, double)
+// operations, so the result is bit-identical to the reference.  The line's
+// work array is tmp[e * nl + line] (consecutive lines, consecutive words).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+smoothc_lines(float *__restrict__ x, float *__restrict__ tmp, int n1, int n2, int n3, int ax, int nb, int box,
+              int der, int adj, int repeat) {
+    const long n12 = (long)n1 * n2;
+    const long nl = ax == 0 ? n12 / n1 * n3 : (ax == 1 ? (long)n1 * n3 : n12);
+    const long line = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (line >= nl) return;
+    long o, d;
+    int nx;
+    if (ax == 0) { o = line * n1; d = 1; nx = n1; }
+    else if (ax == 1) { o = line % n1 + line / n1 * n12; d = n1; nx = n2; }
+    else { o = line; d = n12; nx = n3; }
+    const int np = nx + 2 * nb;
+    const float wt = box ? (float)(1.0 / (2 * nb - 1)) : (float)(1.0 / (nb * nb));
+    const bool one = box || der;
+    float *t = tmp + line;
+    auto T = [&](int e) -> float & { return t[(long)e * nl]; };
+    for (int r = 0; r < repeat; r++) {
+        if (adj) {
+            for (int e = 0; e < np; e++) T(e) = x[o + (long)fold_idx(e - nb, nx) * d];  // fold
+            float s = 0.0f;                                                          // doubint
+            for (int i = np - 1; i >= 0; i--) { s += T(i); T(i) = s; }
+            if (!one) {
+                s = 0.0f;
+                for (int i = 0; i < np; i++) { s += T(i); T(i) = s; }
+            }
+            for (int i = 0; i < nx; i++) {                                           // triple
+                if (box) x[o + i * d] = (T(i + 1) - T(i + 2 * nb)) * wt;
+                else x[o + i * d] = (float)((2. * T(i + nb) - T(i) - T(i + 2 * nb)) * wt);
+            }
+        } else {
+            for (int e = 0; e < np; e++) T(e) = 0.0f;                                // triple2
+            if (box) {
+                const float mw = -wt;
+                for (int i = 0; i < nx; i++) T(i + 1) += wt * x[o + i * d];
+                for (int i = 0; i < nx; i++) T(i + 2 * nb) += mw * x[o + i * d];
+            } else {
+                const float mw = -wt, w2 = (float)(2. * wt);
+                for (int i = 0; i < nx; i++) T(i) += mw * x[o + i * d];
+                for (int i = 0; i < nx; i++) T(i + nb) += w2 * x[o + i * d];
+                for (int i = 0; i < nx; i++) T(i + 2 * nb) += mw * x[o + i * d];
+            }
+            float s = 0.0f;                                                          // doubint2
+            for (int i = 0; i < np; i++) { s += T(i); T(i) = s; }
+            if (!one) {
+                s = 0.0f;
+                for (int i = np - 1; i >= 0; i--) { s += T(i); T(i) = s; }
+            }
+            for (int i = 0; i < nx; i++) x[o + i * d] = T(i + nb);                   // fold2
+            for (int e = nb + nx; e < np; e++) x[o + (long)fold_idx(e - nb, nx) * d] += T(e);
+            for (int e = nb - 1; e >= 0; e--) x[o + (long)fold_idx(e - nb, nx) * d] += T(e);
+        }
+    }
+}
+
 // mask32 (dip_cfuns.c:914-966, both = false)
 __global__ void mask_kernel(const float *__restrict__ md, unsigned char *__restrict__ m0,
                             unsigned char *__restrict__ m1, int n1, int n2, int n3, int nw) {
@@ -731,6 +795,29 @@ extern "C" int sh_dip(sh_ctx *ctx, const float *data, const float *mask, const f
                     rep->ls_step[d][it] = hs[d].rec[2 * it + 1];
                 }
         }
+    }
+    return SH_OK;
+}
+
+// Replaces dipcfun.smoothcf (pyseistr/src/dip_cfuns.c:2006-2123), the C
+// entry behind smoothc (pyseistr/smooth.py:115).
+extern "C" int sh_smooth(sh_ctx *ctx, const float *in, float *out, int n1, int n2, int n3, int repeat, int adj,
+                         int r1, int r2, int r3, int diff1, int diff2, int diff3, int box1, int box2, int box3) {
+    if (!ctx) return SH_ENULL;
+    if (!in || !out) return sh_fail(ctx, SH_ENULL, "sh_smooth: NULL pointer");
+    if (n1 < 1 || n2 < 1 || n3 < 1) return sh_fail(ctx, SH_EINVAL, "sh_smooth: bad shape %d x %d x %d", n1, n2, n3);
+    if (repeat < 0) return sh_fail(ctx, SH_EINVAL, "sh_smooth: repeat must be >= 0");
+    const long n = (long)n1 * n2 * n3;
+    if (out != in) SH_HIP(ctx, hipMemcpyAsync(out, in, (size_t)n * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
+    const int nn[3] = {n1, n2, n3}, rect[3] = {r1, r2, r3}, diff[3] = {diff1, diff2, diff3}, box[3] = {box1, box2, box3};
+    for (int ax = 0; ax < 3; ax++) {
+        if (rect[ax] <= 1 || repeat == 0) continue;
+        const long nl = n / nn[ax];
+        float *tmp = nullptr;
+        SH_TRY(sh_slot(ctx, SLOT_DIP0 + 5, (size_t)(nn[ax] + 2L * rect[ax]) * nl * sizeof(float), (void **)&tmp));
+        hipLaunchKernelGGL(smoothc_lines, dim3((unsigned)ceil_div(nl, 256)), dim3(256), 0, ctx->stream, out, tmp, n1, n2,
+                           n3, ax, rect[ax], box[ax] != 0, diff[ax] != 0, adj != 0, repeat);
+        SH_TRY(sh_launch_check(ctx, "smoothc_lines"));
     }
     return SH_OK;
 }
