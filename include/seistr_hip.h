@@ -82,6 +82,16 @@ int sh_dip(sh_ctx *ctx, const float *data, const float *mask, const float *init,
 int sh_smooth(sh_ctx *ctx, const float *in, float *out, int n1, int n2, int n3, int repeat, int adj,
               int r1, int r2, int r3, int diff1, int diff2, int diff3, int box1, int box2, int box3);
 
+/* Butterworth band-pass of every trace (n1 samples, sampling d1): low cut
+ * flo and high cut fhi (in the units of 1/d1), nplo / nphi poles, phase
+ * != 0 minimum phase, else zero phase (each filter again over the reversed
+ * trace, with half the poles).  Replaces bpcfun.cbp
+ * (pyseistr/src/bp_cfuns.c:524-645), the C entry behind bandpassc
+ * (pyseistr/bp.py:128), including its parameter handling; bit-identical to
+ * the reference.  out may equal in.  SH_EUNSUP above 127 poles. */
+int sh_bandpass(sh_ctx *ctx, const float *in, float *out, int n1, int n2, int n3, float d1, float flo,
+                float fhi, int nplo, int nphi, int phase);
+
 /* 3-D plane-wave spray, slots visited by distance from the source trace.
  * Replaces the spray loop of csomean3d/csomf3d (pyseistr/src/
  * sof3d_cfuns.c:1639-1705) and the pure pwspray3d (pyseistr/pwspray3d.py:2).

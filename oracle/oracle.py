@@ -95,6 +95,20 @@ def trianglen(x, shape, rect, dtype=np.float32):
     return x
 
 
+def bandpass(x, shape, d1, flo=0.0, fhi=0.5, nplo=6, nphi=6, phase=0, dtype=np.float32):
+    """cbp (bp_cfuns.c:524): flat F-order in and out (traces of n1)."""
+    x = _arr(x, dtype).copy()
+    n1 = int(shape[0])
+    ntr = int(np.prod(shape)) // n1
+    r = _real(dtype)
+    lib = _lib(dtype)
+    lib.or_bandpass.argtypes = [ctypes.c_int, ctypes.c_int, r, r, r, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                ctypes.c_void_p]
+    if lib.or_bandpass(n1, ntr, d1, flo, fhi, int(nplo), int(nphi), int(phase), _ptr(x)) != 0:
+        raise ValueError("too many poles")
+    return x
+
+
 def smoothc(x, shape, rect=(1, 1, 1), diff=(0, 0, 0), box=(0, 0, 0), repeat=1, adj=1, dtype=np.float32):
     """smoothcf (dip_cfuns.c:2006): flat F-order in and out."""
     x = _arr(x, dtype).copy()

@@ -58,6 +58,16 @@ def _f(x):
     return np.ascontiguousarray(np.asarray(x, dtype=np.float32).ravel(order="F"))
 
 
+def cbp(din, shape, d1, flo=0.0, fhi=0.5, nplo=6, nphi=6, phase=0):
+    """bpcfun.cbp, format "Oiiifffiiii" = (din, n1, n2, n3, d1, flo, fhi,
+    nplo, nphi, phase, verb) (bp_cfuns.c:540)."""
+    n1, n2, n3 = shape
+    with _quiet():
+        out = _mod("bpcfun").cbp(_f(din), n1, n2, n3, float(d1), float(flo), float(fhi), int(nplo), int(nphi),
+                                 int(phase), 0)
+    return np.asarray(out, np.float32).ravel()
+
+
 def smoothcf(din, shape, rect=(1, 1, 1), diff=(0, 0, 0), box=(0, 0, 0), repeat=1, adj=1):
     """dipcfun.smoothcf, format "Oiiiiiiiiiiiiii" = (din, n1, n2, n3, repeat,
     adj, r1, r2, r3, diff1, diff2, diff3, box1, box2, box3)

@@ -365,6 +365,113 @@ EXPORT void or_smoothc(int n1, int n2, int n3, int repeat, int adj, const int *r
 }
 
 /* ------------------------------------------------------------------ */
+/* Butterworth band-pass per trace (cbp, bp_cfuns.c:524-645):          */
+/* coefficients (sf_butter_init, 420-462), recursive first- and        */
+/* second-order sections (sf_butter_apply, 473-507); zero phase runs   */
+/* each filter again over the reversed trace (sf_reverse, 509-522),    */
+/* restated as a backward pass (the same operations in the same order).*/
+/* ------------------------------------------------------------------ */
+#define BP_PI (3.14159265358979323846264338328)
+#define BP_MAXSEC 64
+typedef struct {
+    int low, nn;
+    real den[BP_MAXSEC][2], mid;
+} butter_t;
+
+static real bp_sin(real a) { return sizeof(real) == 4 ? (real)sinf((float)a) : (real)sin((double)a); }
+static real bp_cos(real a) { return sizeof(real) == 4 ? (real)cosf((float)a) : (real)cos((double)a); }
+
+static void butter_init(butter_t *bw, int low, real cutoff, int nn) {
+    real arg, ss, sinw, cosw, fact;
+    arg = 2. * BP_PI * cutoff;
+    sinw = bp_sin(arg);
+    cosw = bp_cos(arg);
+    bw->nn = nn;
+    bw->low = low;
+    if (nn % 2) {
+        if (low) {
+            fact = (1. + cosw) / sinw;
+            bw->den[nn / 2][0] = 1. / (1. + fact);
+            bw->den[nn / 2][1] = 1. - fact;
+        } else {
+            fact = sinw / (1. + cosw);
+            bw->den[nn / 2][0] = 1. / (fact + 1.);
+            bw->den[nn / 2][1] = fact - 1.;
+        }
+    }
+    fact = low ? bp_sin(0.5 * arg) : bp_cos(0.5 * arg);
+    fact *= fact;
+    for (int j = 0; j < nn / 2; j++) {
+        ss = bp_sin(BP_PI * (2 * j + 1) / (2 * nn)) * sinw;
+        bw->den[j][0] = fact / (1. + ss);
+        bw->den[j][1] = (1. - ss) / fact;
+    }
+    bw->mid = -2. * cosw / fact;
+}
+
+/* one pass of every section over x[0..nx) (dir = +1) or backwards (-1) */
+static void butter_apply(const butter_t *bw, int nx, real *x, int dir) {
+    real d0, d1 = bw->mid, d2, x0, x1, x2, y0, y1, y2;
+    int nn = bw->nn;
+    if (nn % 2) {
+        d0 = bw->den[nn / 2][0];
+        d2 = bw->den[nn / 2][1];
+        x0 = y1 = 0.;
+        for (int k = 0; k < nx; k++) {
+            int ix = dir > 0 ? k : nx - 1 - k;
+            x1 = x0;
+            x0 = x[ix];
+            y0 = bw->low ? (x0 + x1 - d2 * y1) * d0 : (x0 - x1 - d2 * y1) * d0;
+            x[ix] = y1 = y0;
+        }
+    }
+    for (int j = 0; j < nn / 2; j++) {
+        d0 = bw->den[j][0];
+        d2 = bw->den[j][1];
+        x1 = x0 = y1 = y2 = 0.;
+        for (int k = 0; k < nx; k++) {
+            int ix = dir > 0 ? k : nx - 1 - k;
+            x2 = x1;
+            x1 = x0;
+            x0 = x[ix];
+            y0 = bw->low ? (x0 + 2 * x1 + x2 - d1 * y1 - d2 * y2) * d0 : (x0 - 2 * x1 + x2 - d1 * y1 - d2 * y2) * d0;
+            y2 = y1;
+            x[ix] = y1 = y0;
+        }
+    }
+}
+
+/* cbp's parameter handling (582-606) and trace loop (615-637); in place
+ * on ntr traces of n1 samples.  Returns 0, or -1 if a pole count exceeds
+ * what this restatement holds. */
+EXPORT int or_bandpass(int n1, int ntr, real d1, real flo, real fhi, int nplo, int nphi, int phase, real *x) {
+    const float eps = 0.0001;
+    butter_t blo, bhi;
+    if (!(flo < 0)) flo = flo * d1;
+    if (!(fhi < 0)) fhi = fhi * d1;
+    if (nplo < 1) nplo = 1;
+    if (nplo > 1 && !phase) nplo /= 2;
+    if (nphi < 1) nphi = 1;
+    if (nphi > 1 && !phase) nphi /= 2;
+    if ((nplo + 1) / 2 > BP_MAXSEC || (nphi + 1) / 2 > BP_MAXSEC) return -1;
+    int has_lo = flo > eps, has_hi = fhi < 0.5 - eps;
+    if (has_lo) butter_init(&blo, 0, flo, nplo);
+    if (has_hi) butter_init(&bhi, 1, fhi, nphi);
+    for (long i2 = 0; i2 < ntr; i2++) {
+        real *tr = x + i2 * (long)n1;
+        if (has_lo) {
+            butter_apply(&blo, n1, tr, 1);
+            if (!phase) butter_apply(&blo, n1, tr, -1);
+        }
+        if (has_hi) {
+            butter_apply(&bhi, n1, tr, 1);
+            if (!phase) butter_apply(&bhi, n1, tr, -1);
+        }
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
 /* Reductions with the reference's accumulation types.                 */
 /* ps_cblas_dsdot (double), ps_cblas_snrm2 (float sum of squares).     */
 /* dip_cfuns.c:122-209.                                                */

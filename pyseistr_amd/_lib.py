@@ -42,6 +42,8 @@ _SIGS = {
     "sh_dip": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_int, c_int,
                        c_float, c_int, c_int, c_int, c_void_p, ctypes.POINTER(DipReport)]),
     "sh_smooth": (c_int, [c_void_p, c_void_p, c_void_p] + [c_int] * 14),
+    "sh_bandpass": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_float, c_float, c_float, c_int,
+                            c_int, c_int]),
     "sh_pwspray3d": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_int,
                              c_int, c_float, c_void_p]),
     "sh_somean3d": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_int,

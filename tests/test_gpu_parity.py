@@ -244,6 +244,34 @@ def test_smoothc_bit_exact(ps, shape, rect):
     assert np.array_equal(out, exp)
 
 
+@pytest.mark.parametrize("shape", [(200, 3, 1), (64, 10, 2), (512, 256, 8)])
+def test_bandpassc_bit_exact(ps, shape):
+    """bandpassc (cbp) on the device against the FP32 oracle, which is pinned
+    bit for bit against the reference's own cbp: coefficients from the same
+    host expressions, one thread per trace in the reference's order, the
+    zero-phase pass run backwards."""
+    from oracle import oracle as O
+    from pyseistr_amd import bandpass
+    rng = np.random.default_rng(sum(shape))
+    x = rng.standard_normal(int(np.prod(shape))).astype(np.float32)
+    d = _dev(x)
+    for flo, fhi, nplo, nphi in [(0.0, 30.0, 6, 6), (5.0, 40.0, 3, 5), (2.5, 1000.0, 8, 2), (10.0, 60.0, 1, 1)]:
+        for phase in (0, 1):
+            got = bandpass.bandpass_device(d, shape, 0.004, flo, fhi, nplo, nphi, phase).cpu().numpy()
+            exp = O.bandpass(x, shape, 0.004, flo, fhi, nplo, nphi, phase)
+            assert np.array_equal(got.view(np.uint32), exp.view(np.uint32)), (flo, fhi, nplo, nphi, phase)
+    # the drop-in: 1-D and 2-D inputs keep their shapes
+    w = rng.standard_normal(300).astype(np.float32)
+    out = ps.bandpassc(w, 0.004, flo=0, fhi=20)
+    assert out.shape == w.shape
+    assert np.array_equal(out, O.bandpass(w, (300, 1, 1), 0.004, 0.0, 20.0))
+    g = rng.standard_normal((100, 7)).astype(np.float32)
+    out = ps.bandpassc(g, 0.002, flo=3, fhi=80, nplo=4, nphi=4, phase=1)
+    assert out.shape == g.shape
+    with pytest.raises(NotImplementedError):
+        bandpass.bandpass_device(d, shape, 0.004, 5.0, 40.0, 300, 6, 1)
+
+
 def test_dip_directions_on_two_streams_identical(ps, monkeypatch):
     """dip3d runs its two directions side by side (the crossline one on the
     context's second stream, with its own workspace); they are independent,
