@@ -59,6 +59,8 @@ def parse():
     ap.add_argument("--no-conv", action="store_true",
                     help="skip p_conv and the error checks (profiling runs: timing only)")
     ap.add_argument("--no-batch2d", action="store_true", help="skip the batched 2-D side measurement")
+    ap.add_argument("--no-neighbours", action="store_true",
+                    help="skip the side measurements of smoothc, bandpassc and patch3d / patch3d_inv")
     return ap.parse_args()
 
 
@@ -134,6 +136,76 @@ def batched_2d(torch, count=2048, sample=128):
     return {"patches": count, "samples": b.ntot, "tile": list(batch.tile_shapes()[tile]) if tile >= 0 else None,
             "batched_s": round(tb, 4), "batched_samples_per_s": round(b.ntot / tb, 1),
             "per_patch_s_est": round(tp, 3), "per_patch_sample": sample, "speedup": round(tp / tb, 2)}
+
+
+def neighbours(torch, shape, cpu=True, reps=5):
+    """Side measurements of the path's neighbours on the volume of the main
+    workload (device-resident, torch events on the library's stream): ms per
+    call, samples/s, and GB/s of the minimum traffic (each input and output
+    byte moved once).  CPU legs on a bounded sample, one core: the
+    reference's C entries (smoothcf, cbp) where oracle/_ref is built, the
+    numpy restatement of patch3d / patch3d_inv (the reference's are Python
+    loops of the same form)."""
+    import numpy as np
+
+    from pyseistr_amd import bandpass, patch, smooth
+    n = int(np.prod(shape))
+    g = torch.Generator(device="cuda").manual_seed(9)
+    x = torch.randn(n, generator=g, device="cuda")
+    x64 = x.double()
+    pl, ps = (64, 32, 32), (32, 16, 16)
+    npatch = patch.patch_count(shape, pl, ps)
+    L = int(np.prod(pl))
+    X = torch.empty((npatch, L), dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    y64 = torch.empty_like(x64)
+    ops = {
+        "smoothc(rect=5,5,5)": (lambda: smooth.smooth_device(x, shape, (5, 5, 5), out=y), 3 * 8.0 * n),
+        "bandpassc(dt=0.004, 5-60 Hz, 6+6 poles, zero phase)":
+            (lambda: bandpass.bandpass_device(x, shape, 0.004, 5.0, 60.0, 6, 6, 0, out=y), 8.0 * n),
+        f"patch3d(l={pl}, s={ps})": (lambda: patch.patch_device(x64, shape, pl, ps, out=X), 8.0 * (n + npatch * L)),
+        f"patch3d_inv(l={pl}, s={ps})": (lambda: patch.patch_inv_device(X, shape, pl, ps, out=y64),
+                                         8.0 * (n + npatch * L)),
+    }
+    out = {}
+    for name, (fn, by) in ops.items():
+        fn()
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(reps):
+            fn()
+        b.record()
+        torch.cuda.synchronize()
+        ms = a.elapsed_time(b) / reps
+        out[name] = {"ms": round(ms, 3), "samples_per_s": round(n / (ms / 1e3), 1),
+                     "min_traffic_GB/s": round(by / (ms / 1e3) / 1e9, 1)}
+    del X
+    if cpu:
+        cs = (512, 128, 128)
+        v = np.random.default_rng(9).standard_normal(int(np.prod(cs))).astype(np.float32)
+        legs = {}
+        try:
+            from oracle import ref
+            if ref.available():
+                legs["smoothc(rect=5,5,5)"] = (lambda: ref.smoothcf(v, cs, (5, 5, 5)), "reference")
+                legs["bandpassc(dt=0.004, 5-60 Hz, 6+6 poles, zero phase)"] = (
+                    lambda: ref.cbp(v, cs, 0.004, 5.0, 60.0, 6, 6, 0), "reference")
+        except ImportError:
+            pass
+        from oracle import patch_oracle as PO
+        v3 = v.reshape(cs, order="F")
+        Xc = PO.cut(v3, pl, ps)
+        legs[f"patch3d(l={pl}, s={ps})"] = (lambda: PO.cut(v3, pl, ps), "port")
+        legs[f"patch3d_inv(l={pl}, s={ps})"] = (lambda: PO.paste(Xc, cs, pl, ps), "port")
+        for name, (fn, kind) in legs.items():
+            t = time.perf_counter()
+            fn()
+            dt = time.perf_counter() - t
+            out[name]["cpu"] = {"samples_per_s": round(v.size / dt, 1), "kind": kind, "cores": 1,
+                                "sample": f"{cs[0]}x{cs[1]}x{cs[2]}, {dt:.2f} s"}
+        del Xc
+    return out
 
 
 def measured_traffic(name, path=None):
@@ -325,6 +397,7 @@ def main():
         info["cnn_frac_bf16_peak"] = round(info["cnn_tflops"] / BF16_PEAK_TFLOPS, 4)
 
     batch2d = None if args.no_batch2d else batched_2d(torch)
+    nbr = None if args.no_neighbours else neighbours(torch, shape, cpu=not args.no_cpu)
     cpu = None if args.no_cpu else cpu_baseline(tuple(int(s) for s in args.cpu_shape.split(",")))
     line = {
         "metric": METRIC, "value": round(value, 1), "unit": "samples/s", "n_gpus": 1, "steps": args.steps,
@@ -338,6 +411,7 @@ def main():
                         "shape": list(shape), "samples_per_step": n, "start": "warm" if use_warm else "cold",
                         "wall_s": round(wall, 3)}, **info),
         "roofline": roofline, "cpu_baseline": cpu, "breakdown": breakdown, "batched_2d": batch2d,
+        "neighbours": nbr,
     }
     print(json.dumps(line), flush=True)
     return 0

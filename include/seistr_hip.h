@@ -92,6 +92,24 @@ int sh_smooth(sh_ctx *ctx, const float *in, float *out, int n1, int n2, int n3, 
 int sh_bandpass(sh_ctx *ctx, const float *in, float *out, int n1, int n2, int n3, float d1, float flo,
                 float fhi, int nplo, int nphi, int phase);
 
+/* Patch extraction and reassembly (mode 1), for 2-D data pass n3 = l3 =
+ * s3 = 1.  Each axis is zero-padded so that (n - l) is a multiple of s;
+ * patches are numbered with axis 1 outermost and axis 3 innermost, each a
+ * row of l1*l2*l3 samples in Fortran order.  Replace the pure-Python
+ * patch2d / patch3d (pyseistr/patch.py:2-77, 86-201) and patch2d_inv /
+ * patch3d_inv (patch.py:208-300, 307-446); float64 throughout, as the
+ * reference computes once it pads.
+ * sh_patch_count: *npatch = number of patches; SH_EINVAL if none fits. */
+int sh_patch_count(int n1, int n2, int n3, int l1, int l2, int l3, int s1, int s2, int s3, long *npatch);
+/* X[npatch][l1*l2*l3] = the patches of in[n1*n2*n3] (Fortran order). */
+int sh_patch(sh_ctx *ctx, const double *in, double *X, int n1, int n2, int n3, int l1, int l2, int l3,
+             int s1, int s2, int s3);
+/* out[n1*n2*n3] = the patches X summed in patch order and divided by the
+ * number covering each sample (NaN where none does), identical to the
+ * reference's float64 reassembly. */
+int sh_patch_inv(sh_ctx *ctx, const double *X, double *out, int n1, int n2, int n3, int l1, int l2, int l3,
+                 int s1, int s2, int s3);
+
 /* 3-D plane-wave spray, slots visited by distance from the source trace.
  * Replaces the spray loop of csomean3d/csomf3d (pyseistr/src/
  * sof3d_cfuns.c:1639-1705) and the pure pwspray3d (pyseistr/pwspray3d.py:2).

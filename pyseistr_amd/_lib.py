@@ -44,6 +44,9 @@ _SIGS = {
     "sh_smooth": (c_int, [c_void_p, c_void_p, c_void_p] + [c_int] * 14),
     "sh_bandpass": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_float, c_float, c_float, c_int,
                             c_int, c_int]),
+    "sh_patch_count": (c_int, [c_int] * 9 + [ctypes.POINTER(ctypes.c_long)]),
+    "sh_patch": (c_int, [c_void_p, c_void_p, c_void_p] + [c_int] * 9),
+    "sh_patch_inv": (c_int, [c_void_p, c_void_p, c_void_p] + [c_int] * 9),
     "sh_pwspray3d": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_int,
                              c_int, c_float, c_void_p]),
     "sh_somean3d": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_int,

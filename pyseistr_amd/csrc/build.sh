@@ -11,6 +11,7 @@ SRCS="common dip"
 [ -f "$HERE/cnn.hip" ] && SRCS="$SRCS cnn"
 [ -f "$HERE/batch.hip" ] && SRCS="$SRCS batch"
 [ -f "$HERE/bandpass.hip" ] && SRCS="$SRCS bandpass"
+[ -f "$HERE/patch.hip" ] && SRCS="$SRCS patch"
 OBJDIR="$HERE/_build"
 mkdir -p "$OBJDIR"
 FLAGS="--offload-arch=gfx950 -O3 -fPIC -std=c++17 -ffp-contract=off -Xclang -target-feature -Xclang -packed-fp32-ops -Wno-unused-result"
