@@ -60,7 +60,7 @@ def parse():
                     help="skip p_conv and the error checks (profiling runs: timing only)")
     ap.add_argument("--no-batch2d", action="store_true", help="skip the batched 2-D side measurement")
     ap.add_argument("--no-neighbours", action="store_true",
-                    help="skip the side measurements of smoothc, bandpassc and patch3d / patch3d_inv")
+                    help="skip the side measurements of smoothc, bandpassc, patch3d / patch3d_inv and fkdip")
     return ap.parse_args()
 
 
@@ -139,16 +139,18 @@ def batched_2d(torch, count=2048, sample=128):
 
 
 def neighbours(torch, shape, cpu=True, reps=5):
-    """Side measurements of the path's neighbours on the volume of the main
-    workload (device-resident, torch events on the library's stream): ms per
-    call, samples/s, and GB/s of the minimum traffic (each input and output
-    byte moved once).  CPU legs on a bounded sample, one core: the
-    reference's C entries (smoothcf, cbp) where oracle/_ref is built, the
-    numpy restatement of patch3d / patch3d_inv (the reference's are Python
-    loops of the same form)."""
+    """Side measurements of the path's neighbours, device-resident, timed
+    with torch events on the library's stream: smoothc, bandpassc and patch
+    I/O on the volume of the main workload, fkdip on an 8192 x 4096 DAS
+    section.  Reported: ms per call, samples/s, and GB/s of the minimum
+    traffic (each input and output byte moved once).  CPU legs on a bounded
+    sample, one core: the reference's C entries (smoothcf, cbp) where
+    oracle/_ref is built, the numpy restatements of patch3d / patch3d_inv
+    (the reference's are Python loops of the same form) and of fkdip (the
+    same numpy FFT calls as the reference)."""
     import numpy as np
 
-    from pyseistr_amd import bandpass, patch, smooth
+    from pyseistr_amd import bandpass, fk, patch, smooth
     n = int(np.prod(shape))
     g = torch.Generator(device="cuda").manual_seed(9)
     x = torch.randn(n, generator=g, device="cuda")
@@ -167,6 +169,10 @@ def neighbours(torch, shape, cpu=True, reps=5):
         f"patch3d_inv(l={pl}, s={ps})": (lambda: patch.patch_inv_device(X, shape, pl, ps, out=y64),
                                          8.0 * (n + npatch * L)),
     }
+    fks = (8192, 4096)  # a DAS section: samples x channels
+    xf = torch.randn(fks[0] * fks[1], generator=g, device="cuda", dtype=torch.float64)
+    yf = torch.empty_like(xf)
+    ops[f"fkdip({fks[0]}x{fks[1]}, w=0.02)"] = (lambda: fk.fkdip_device(xf, fks, 0.02, out=yf), 16.0 * xf.numel())
     out = {}
     for name, (fn, by) in ops.items():
         fn()
@@ -178,9 +184,10 @@ def neighbours(torch, shape, cpu=True, reps=5):
         b.record()
         torch.cuda.synchronize()
         ms = a.elapsed_time(b) / reps
-        out[name] = {"ms": round(ms, 3), "samples_per_s": round(n / (ms / 1e3), 1),
+        ns = xf.numel() if name.startswith("fkdip") else n
+        out[name] = {"ms": round(ms, 3), "samples_per_s": round(ns / (ms / 1e3), 1),
                      "min_traffic_GB/s": round(by / (ms / 1e3) / 1e9, 1)}
-    del X
+    del X, xf, yf
     if cpu:
         cs = (512, 128, 128)
         v = np.random.default_rng(9).standard_normal(int(np.prod(cs))).astype(np.float32)
@@ -205,6 +212,14 @@ def neighbours(torch, shape, cpu=True, reps=5):
             out[name]["cpu"] = {"samples_per_s": round(v.size / dt, 1), "kind": kind, "cores": 1,
                                 "sample": f"{cs[0]}x{cs[1]}x{cs[2]}, {dt:.2f} s"}
         del Xc
+        from oracle import fk_oracle as FO
+        sec = np.random.default_rng(9).standard_normal((2048, 1024))
+        t = time.perf_counter()
+        FO.fkdip(sec, 0.02)
+        dt = time.perf_counter() - t
+        out[f"fkdip({fks[0]}x{fks[1]}, w=0.02)"]["cpu"] = {
+            "samples_per_s": round(sec.size / dt, 1), "kind": "port", "cores": 1,
+            "sample": f"2048x1024 through numpy's FFTs (the reference's own calls), {dt:.2f} s"}
     return out
 
 

@@ -110,6 +110,14 @@ int sh_patch(sh_ctx *ctx, const double *in, double *X, int n1, int n2, int n3, i
 int sh_patch_inv(sh_ctx *ctx, const double *X, double *out, int n1, int n2, int n3, int l1, int l2, int l3,
                  int s1, int s2, int s3);
 
+/* FK dip filter of a 2-D section in[n1*n2] (Fortran order, float64):
+ * padded to powers of two, transformed in time and space, a cone of half
+ * width w (fraction of the padded trace count) kept, and transformed back;
+ * out[n1*n2].  Replaces the pure-Python fkdip (pyseistr/fk.py:2-68), which
+ * computes with numpy's float64 FFTs; agreement is to float64 rounding.
+ * SH_EINVAL when w * nk == 0 (the reference divides by it). */
+int sh_fkdip(sh_ctx *ctx, const double *in, double *out, int n1, int n2, double w);
+
 /* 3-D plane-wave spray, slots visited by distance from the source trace.
  * Replaces the spray loop of csomean3d/csomf3d (pyseistr/src/
  * sof3d_cfuns.c:1639-1705) and the pure pwspray3d (pyseistr/pwspray3d.py:2).

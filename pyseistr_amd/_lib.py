@@ -47,6 +47,7 @@ _SIGS = {
     "sh_patch_count": (c_int, [c_int] * 9 + [ctypes.POINTER(ctypes.c_long)]),
     "sh_patch": (c_int, [c_void_p, c_void_p, c_void_p] + [c_int] * 9),
     "sh_patch_inv": (c_int, [c_void_p, c_void_p, c_void_p] + [c_int] * 9),
+    "sh_fkdip": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, ctypes.c_double]),
     "sh_pwspray3d": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_int,
                              c_int, c_float, c_void_p]),
     "sh_somean3d": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_int,

@@ -12,6 +12,7 @@ SRCS="common dip"
 [ -f "$HERE/batch.hip" ] && SRCS="$SRCS batch"
 [ -f "$HERE/bandpass.hip" ] && SRCS="$SRCS bandpass"
 [ -f "$HERE/patch.hip" ] && SRCS="$SRCS patch"
+[ -f "$HERE/fk.hip" ] && SRCS="$SRCS fk"
 OBJDIR="$HERE/_build"
 mkdir -p "$OBJDIR"
 FLAGS="--offload-arch=gfx950 -O3 -fPIC -std=c++17 -ffp-contract=off -Xclang -target-feature -Xclang -packed-fp32-ops -Wno-unused-result"
@@ -21,6 +22,7 @@ for s in $SRCS; do
   pids="$pids $!"
 done
 for p in $pids; do wait $p; done
-$HIPCC --offload-arch=gfx950 -shared -fPIC $(for s in $SRCS; do echo "$OBJDIR/$s.o"; done) -o "$OUT.tmp"
+$HIPCC --offload-arch=gfx950 -shared -fPIC $(for s in $SRCS; do echo "$OBJDIR/$s.o"; done) -L/opt/rocm/lib -lhipfft \
+  -Wl,-rpath,/opt/rocm/lib -o "$OUT.tmp"
 mv "$OUT.tmp" "$OUT"
 echo "built $OUT"

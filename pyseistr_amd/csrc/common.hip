@@ -132,6 +132,7 @@ int sh_ctx_trim(sh_ctx *ctx) {
     if (!ctx) return SH_ENULL;
     SH_HIP(ctx, hipStreamSynchronize(ctx->stream));
     if (ctx->aux) SH_HIP(ctx, hipStreamSynchronize(ctx->aux));
+    ctx->fk.reset();
     for (int i = 0; i < SLOT_COUNT; i++) {
         if (ctx->slot_ptr[i]) (void)hipFree(ctx->slot_ptr[i]);
         ctx->slot_ptr[i] = nullptr;

@@ -8,6 +8,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -40,6 +41,9 @@ struct sh_ctx {
     std::vector<sh_prof_rec> recs;
     std::vector<hipEvent_t> pool;
     size_t pool_used = 0;
+    // FFT plans of the FK dip filter for the last size it ran on (owned;
+    // the deleter destroys them)
+    std::shared_ptr<void> fk;
 };
 
 hipEvent_t sh_prof_event(sh_ctx *ctx);
@@ -82,7 +86,8 @@ enum {
     SLOT_SPRAY0 = 20,    // spray module: 20..39
     SLOT_INTERP0 = 40,   // interpolation module: 40..51, warm-start network: 52..59
     SLOT_BATCH0 = 64,    // batched 2-D module: 64..79
-    SLOT_COUNT = 80
+    SLOT_FK0 = 80,       // FK dip filter: 80..81
+    SLOT_COUNT = 84
 };
 
 // ---------------------------------------------------------------------------

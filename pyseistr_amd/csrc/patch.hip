@@ -15,15 +15,27 @@ struct PGeo {
     int n[3], l[3], s[3], P[3];  // sizes, patch sizes, shifts, patches per axis
 };
 
-__global__ void patch_cut(const double *__restrict__ in, double *__restrict__ X, PGeo g, long L, long total) {
-    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= total) return;
-    const long pid = idx / L, e = idx % L;
-    const int e1 = (int)(e % g.l[0]), e2 = (int)((e / g.l[0]) % g.l[1]), e3 = (int)(e / ((long)g.l[0] * g.l[1]));
-    const int p3 = (int)(pid % g.P[2]), p2 = (int)((pid / g.P[2]) % g.P[1]), p1 = (int)(pid / ((long)g.P[1] * g.P[2]));
-    const int i1 = p1 * g.s[0] + e1, i2 = p2 * g.s[1] + e2, i3 = p3 * g.s[2] + e3;
-    const bool in_ = i1 < g.n[0] && i2 < g.n[1] && i3 < g.n[2];
-    X[idx] = in_ ? in[i1 + (long)g.n[0] * (i2 + (long)g.n[1] * i3)] : 0.0;
+// One block per patch.  Threads cover the l1 x l2 plane of the patch (q =
+// e1 + l1*e2, so runs of l1 samples are contiguous in both the volume and
+// the patch row) and walk axis 3 with fixed strides; the only divisions are
+// one per plane position and the block's patch-number split.
+__global__ void patch_cut(const double *__restrict__ in, double *__restrict__ X, PGeo g, long L) {
+    const int pid = blockIdx.x;
+    const int p3 = pid % g.P[2], p2 = (pid / g.P[2]) % g.P[1], p1 = pid / (g.P[1] * g.P[2]);
+    const int b1 = p1 * g.s[0], b2 = p2 * g.s[1], b3 = p3 * g.s[2];
+    const int l1 = g.l[0], l12 = g.l[0] * g.l[1], l3 = g.l[2];
+    const long n12 = (long)g.n[0] * g.n[1];
+    const int m3 = min(l3, g.n[2] - b3);  // axis-3 samples inside the volume (may be <= 0)
+    double *row = X + (long)pid * L;
+    for (int q = threadIdx.x; q < l12; q += blockDim.x) {
+        const int i1 = b1 + q % l1, i2 = b2 + q / l1;
+        const bool ok = i1 < g.n[0] && i2 < g.n[1];
+        const double *src = in + i1 + (long)g.n[0] * i2 + n12 * b3;
+        const int m = ok ? m3 : 0;
+        int e3 = 0;
+        for (; e3 < m; e3++) row[q + (long)e3 * l12] = src[e3 * n12];
+        for (; e3 < l3; e3++) row[q + (long)e3 * l12] = 0.0;
+    }
 }
 
 // patches along one axis whose span covers sample i: [lo, hi]
@@ -33,24 +45,33 @@ __device__ inline void covering(int i, int l, int s, int P, int &lo, int &hi) {
     hi = min(i / s, P - 1);
 }
 
-__global__ void patch_paste(const double *__restrict__ X, double *__restrict__ out, PGeo g, long L, long nout) {
-    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= nout) return;
-    const int i1 = (int)(v % g.n[0]), i2 = (int)((v / g.n[0]) % g.n[1]), i3 = (int)(v / ((long)g.n[0] * g.n[1]));
-    int lo1, hi1, lo2, hi2, lo3, hi3;
-    covering(i1, g.l[0], g.s[0], g.P[0], lo1, hi1);
+// One block per (i2, i3) column of the output, threads along i1.  Every
+// sample adds its covering patches in patch order (p1, then p2, then p3),
+// the order in which the reference accumulates them.
+__global__ void patch_paste(const double *__restrict__ X, double *__restrict__ out, PGeo g, long L) {
+    const int c = blockIdx.x;
+    const int i2 = c % g.n[1], i3 = c / g.n[1];
+    int lo2, hi2, lo3, hi3;
     covering(i2, g.l[1], g.s[1], g.P[1], lo2, hi2);
     covering(i3, g.l[2], g.s[2], g.P[2], lo3, hi3);
-    double acc = 0.0, cnt = 0.0;
-    for (int p1 = lo1; p1 <= hi1; p1++)
-        for (int p2 = lo2; p2 <= hi2; p2++)
-            for (int p3 = lo3; p3 <= hi3; p3++) {
-                const long id = ((long)p1 * g.P[1] + p2) * g.P[2] + p3;
-                const long e = (i1 - p1 * g.s[0]) + (long)g.l[0] * ((i2 - p2 * g.s[1]) + (long)g.l[1] * (i3 - p3 * g.s[2]));
-                acc += X[id * L + e];
-                cnt += 1.0;
+    const long l1 = g.l[0], l12 = (long)g.l[0] * g.l[1];
+    for (int i1 = threadIdx.x; i1 < g.n[0]; i1 += blockDim.x) {
+        int lo1, hi1;
+        covering(i1, g.l[0], g.s[0], g.P[0], lo1, hi1);
+        double acc = 0.0, cnt = 0.0;
+        for (int p1 = lo1; p1 <= hi1; p1++) {
+            const long e1 = i1 - p1 * g.s[0];
+            for (int p2 = lo2; p2 <= hi2; p2++) {
+                const long id0 = ((long)p1 * g.P[1] + p2) * g.P[2];
+                const long e12 = e1 + l1 * (i2 - p2 * g.s[1]);
+                for (int p3 = lo3; p3 <= hi3; p3++) {
+                    acc += X[(id0 + p3) * L + e12 + l12 * (i3 - p3 * g.s[2])];
+                    cnt += 1.0;
+                }
             }
-    out[v] = acc / cnt;
+        }
+        out[(long)c * g.n[0] + i1] = acc / cnt;
+    }
 }
 
 // padded size N and patch count P per axis (patch.py:66-72, 174-184)
@@ -72,8 +93,8 @@ int geometry(sh_ctx *ctx, int n1, int n2, int n3, int l1, int l2, int l3, int s1
         np *= g.P[a];
     }
     L = (long)l1 * l2 * l3;
-    if (np * L / 256 >= (1L << 31) || (long)n1 * n2 * n3 / 256 >= (1L << 31))
-        return sh_fail(ctx, SH_EINVAL, "patch: %ld patches of %ld samples exceed one launch", np, L);
+    if (np >= (1L << 31) || (long)n2 * n3 >= (1L << 31) || (long)l1 * l2 >= (1L << 31))
+        return sh_fail(ctx, SH_EINVAL, "patch: %ld patches or %ld columns exceed one launch", np, (long)n2 * n3);
     return SH_OK;
 }
 
@@ -95,8 +116,8 @@ extern "C" int sh_patch(sh_ctx *ctx, const double *in, double *X, int n1, int n2
     PGeo g;
     long np, L;
     SH_TRY(geometry(ctx, n1, n2, n3, l1, l2, l3, s1, s2, s3, g, np, L));
-    const long total = np * L;
-    hipLaunchKernelGGL(patch_cut, dim3((unsigned)ceil_div(total, 256)), dim3(256), 0, ctx->stream, in, X, g, L, total);
+    const int tpb = (int)std::min<long>(256, ceil_div((long)l1 * l2, 64) * 64);
+    hipLaunchKernelGGL(patch_cut, dim3((unsigned)np), dim3(tpb), 0, ctx->stream, in, X, g, L);
     return sh_launch_check(ctx, "patch_cut");
 }
 
@@ -107,7 +128,7 @@ extern "C" int sh_patch_inv(sh_ctx *ctx, const double *X, double *out, int n1, i
     PGeo g;
     long np, L;
     SH_TRY(geometry(ctx, n1, n2, n3, l1, l2, l3, s1, s2, s3, g, np, L));
-    const long nout = (long)n1 * n2 * n3;
-    hipLaunchKernelGGL(patch_paste, dim3((unsigned)ceil_div(nout, 256)), dim3(256), 0, ctx->stream, X, out, g, L, nout);
+    const int tpb = (int)std::min<long>(256, ceil_div((long)n1, 64) * 64);
+    hipLaunchKernelGGL(patch_paste, dim3((unsigned)((long)n2 * n3)), dim3(tpb), 0, ctx->stream, X, out, g, L);
     return sh_launch_check(ctx, "patch_paste");
 }
