@@ -244,22 +244,32 @@ def test_smoothc_bit_exact(ps, shape, rect):
     assert np.array_equal(out, exp)
 
 
-@pytest.mark.parametrize("shape", [(200, 3, 1), (64, 10, 2), (512, 256, 8)])
-def test_bandpassc_bit_exact(ps, shape):
+@pytest.mark.parametrize("shape", [(200, 3, 1), (64, 10, 2), (512, 256, 8), (1, 70, 1), (130, 129, 1)])
+def test_bandpassc_bit_exact(ps, shape, monkeypatch):
     """bandpassc (cbp) on the device against the FP32 oracle, which is pinned
     bit for bit against the reference's own cbp: coefficients from the same
-    host expressions, one thread per trace in the reference's order, the
-    zero-phase pass run backwards."""
+    host expressions, every section in the reference's order of float
+    operations, the zero-phase pass run backwards.  Covers the fused LDS
+    kernel (sections cascaded per sample, 64-trace tiles with ragged sample
+    and trace tails), the fused minimum-phase pass of both filters, and the
+    one-thread-per-trace kernel that takes passes of more than 8 sections
+    (and runs everything under SH_BANDPASS_PLAIN=1)."""
     from oracle import oracle as O
     from pyseistr_amd import bandpass
     rng = np.random.default_rng(sum(shape))
     x = rng.standard_normal(int(np.prod(shape))).astype(np.float32)
     d = _dev(x)
-    for flo, fhi, nplo, nphi in [(0.0, 30.0, 6, 6), (5.0, 40.0, 3, 5), (2.5, 1000.0, 8, 2), (10.0, 60.0, 1, 1)]:
-        for phase in (0, 1):
-            got = bandpass.bandpass_device(d, shape, 0.004, flo, fhi, nplo, nphi, phase).cpu().numpy()
-            exp = O.bandpass(x, shape, 0.004, flo, fhi, nplo, nphi, phase)
-            assert np.array_equal(got.view(np.uint32), exp.view(np.uint32)), (flo, fhi, nplo, nphi, phase)
+    cases = [(0.0, 30.0, 6, 6), (5.0, 40.0, 3, 5), (2.5, 1000.0, 8, 2), (10.0, 60.0, 1, 1), (3.0, 70.0, 9, 9),
+             (20.0, 45.0, 20, 18)]
+    for plain in (False, True):
+        if plain:
+            monkeypatch.setenv("SH_BANDPASS_PLAIN", "1")
+        for flo, fhi, nplo, nphi in cases:
+            for phase in (0, 1):
+                got = bandpass.bandpass_device(d, shape, 0.004, flo, fhi, nplo, nphi, phase).cpu().numpy()
+                exp = O.bandpass(x, shape, 0.004, flo, fhi, nplo, nphi, phase)
+                assert np.array_equal(got.view(np.uint32), exp.view(np.uint32)), (flo, fhi, nplo, nphi, phase, plain)
+    monkeypatch.delenv("SH_BANDPASS_PLAIN")
     # the drop-in: 1-D and 2-D inputs keep their shapes
     w = rng.standard_normal(300).astype(np.float32)
     out = ps.bandpassc(w, 0.004, flo=0, fhi=20)
