@@ -406,6 +406,61 @@ smoothc_lines(float *__restrict__ x, float *__restrict__ tmp, int n1, int n2, in
     }
 }
 
+// smoothcf with adj != 0 (ps_smooth) in two sweeps per line: the fold
+// feeds the backward running sum directly (its sums are stored once in the
+// work plane, [sample][line]), and the forward sweep runs the second
+// running sum and forms each output from a ring of the last 2*nb integrated
+// values in LDS.  Every float operation of fold / doubint / triple happens
+// in the reference's order, so results are bit-identical to smoothc_lines;
+// the work plane is written and read once instead of five times.
+__global__ void __launch_bounds__(64)
+smoothc_sweep(float *__restrict__ x, float *__restrict__ tmp, int n1, int n2, int n3, int ax, int nb, int box,
+              int der, int repeat) {
+    extern __shared__ float ring[];  // [2 * nb][64]
+    const long n12 = (long)n1 * n2;
+    const long nl = ax == 0 ? n12 / n1 * n3 : (ax == 1 ? (long)n1 * n3 : n12);
+    const long line = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (line >= nl) return;
+    long o, d;
+    int nx;
+    if (ax == 0) { o = line * n1; d = 1; nx = n1; }
+    else if (ax == 1) { o = line % n1 + line / n1 * n12; d = n1; nx = n2; }
+    else { o = line; d = n12; nx = n3; }
+    const int np = nx + 2 * nb, R = 2 * nb;
+    const float wt = box ? (float)(1.0 / (2 * nb - 1)) : (float)(1.0 / (nb * nb));
+    const bool one = box || der;
+    float *t = tmp + line;
+    float *rg = ring + threadIdx.x;
+    for (int r = 0; r < repeat; r++) {
+        float s = 0.0f;
+        for (int e = np - 1; e >= 0; e--) {
+            s += x[o + (long)fold_idx(e - nb, nx) * d];
+            t[(long)e * nl] = s;
+        }
+        float s2 = 0.0f;
+        int slot = 0;  // e % R
+        for (int e = 0; e < np; e++) {
+            float v = t[(long)e * nl];
+            if (!one) {
+                s2 += v;
+                v = s2;
+            }
+            if (e >= R) {  // output i = e - 2 nb; the ring holds T(i) .. T(i + 2 nb - 1)
+                const long oi = o + (long)(e - R) * d;
+                if (box) {
+                    const int s1 = slot + 1 == R ? 0 : slot + 1;
+                    x[oi] = (rg[s1 * 64] - v) * wt;
+                } else {
+                    const int sm = slot + nb >= R ? slot + nb - R : slot + nb;
+                    x[oi] = (float)((2. * rg[sm * 64] - rg[slot * 64] - v) * wt);
+                }
+            }
+            rg[slot * 64] = v;
+            slot = slot + 1 == R ? 0 : slot + 1;
+        }
+    }
+}
+
 // mask32 (dip_cfuns.c:914-966, both = false)
 __global__ void mask_kernel(const float *__restrict__ md, unsigned char *__restrict__ m0,
                             unsigned char *__restrict__ m1, int n1, int n2, int n3, int nw) {
@@ -810,11 +865,20 @@ extern "C" int sh_smooth(sh_ctx *ctx, const float *in, float *out, int n1, int n
     const long n = (long)n1 * n2 * n3;
     if (out != in) SH_HIP(ctx, hipMemcpyAsync(out, in, (size_t)n * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
     const int nn[3] = {n1, n2, n3}, rect[3] = {r1, r2, r3}, diff[3] = {diff1, diff2, diff3}, box[3] = {box1, box2, box3};
+    // SH_SMOOTHC_LINES=1: the one-pass-per-step kernel for adj != 0 as well
+    const bool lines_only = getenv("SH_SMOOTHC_LINES") != nullptr;
     for (int ax = 0; ax < 3; ax++) {
         if (rect[ax] <= 1 || repeat == 0) continue;
         const long nl = n / nn[ax];
         float *tmp = nullptr;
         SH_TRY(sh_slot(ctx, SLOT_DIP0 + 5, (size_t)(nn[ax] + 2L * rect[ax]) * nl * sizeof(float), (void **)&tmp));
+        const size_t ring = (size_t)2 * rect[ax] * 64 * sizeof(float);
+        if (adj && ring <= 32768 && !lines_only) {
+            hipLaunchKernelGGL(smoothc_sweep, dim3((unsigned)ceil_div(nl, 64)), dim3(64), ring, ctx->stream, out, tmp, n1,
+                               n2, n3, ax, rect[ax], box[ax] != 0, diff[ax] != 0, repeat);
+            SH_TRY(sh_launch_check(ctx, "smoothc_sweep"));
+            continue;
+        }
         hipLaunchKernelGGL(smoothc_lines, dim3((unsigned)ceil_div(nl, 256)), dim3(256), 0, ctx->stream, out, tmp, n1, n2,
                            n3, ax, rect[ax], box[ax] != 0, diff[ax] != 0, adj != 0, repeat);
         SH_TRY(sh_launch_check(ctx, "smoothc_lines"));

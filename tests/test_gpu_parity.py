@@ -220,22 +220,28 @@ def test_fused_smoothing_matches_separable(ps, monkeypatch):
 @pytest.mark.parametrize("shape,rect", [((40, 1, 1), (3, 1, 1)), ((37, 23, 1), (1, 4, 1)),
                                         ((20, 11, 9), (5, 5, 5)), ((5, 4, 3), (12, 2, 7)),
                                         ((128, 64, 48), (4, 6, 3))])
-def test_smoothc_bit_exact(ps, shape, rect):
+def test_smoothc_bit_exact(ps, shape, rect, monkeypatch):
     """smoothc (smoothcf) on the device against the FP32 oracle, which is
-    pinned bit for bit against the reference's own smoothcf: one thread per
-    line in the reference's operation order gives identical bits, for both
-    forms, derivative, box, repeat and radii beyond the axis length."""
+    pinned bit for bit against the reference's own smoothcf: the reference's
+    operation order gives identical bits, for both forms, derivative, box,
+    repeat and radii beyond the axis length; adj=1 runs the two-sweep kernel
+    and, under SH_SMOOTHC_LINES=1, the one-thread-per-line kernel that adj=0
+    always uses."""
     from oracle import oracle as O
     from pyseistr_amd import smooth
     rng = np.random.default_rng(sum(shape))
     x = rng.standard_normal(int(np.prod(shape))).astype(np.float32)
     d = _dev(x)
-    for diff, box in [((0, 0, 0), (0, 0, 0)), ((1, 0, 1), (0, 0, 0)), ((0, 0, 0), (1, 1, 0))]:
-        for repeat in (1, 2):
-            for adj in (0, 1):
-                got = smooth.smooth_device(d, shape, rect, diff, box, repeat, adj).cpu().numpy()
-                exp = O.smoothc(x, shape, rect, diff, box, repeat, adj)
-                assert np.array_equal(got.view(np.uint32), exp.view(np.uint32)), (diff, box, repeat, adj)
+    for lines in (False, True):
+        if lines:
+            monkeypatch.setenv("SH_SMOOTHC_LINES", "1")
+        for diff, box in [((0, 0, 0), (0, 0, 0)), ((1, 0, 1), (0, 0, 0)), ((0, 0, 0), (1, 1, 0))]:
+            for repeat in (1, 2):
+                for adj in (0, 1):
+                    got = smooth.smooth_device(d, shape, rect, diff, box, repeat, adj).cpu().numpy()
+                    exp = O.smoothc(x, shape, rect, diff, box, repeat, adj)
+                    assert np.array_equal(got.view(np.uint32), exp.view(np.uint32)), (diff, box, repeat, adj, lines)
+    monkeypatch.delenv("SH_SMOOTHC_LINES")
     # the drop-in: 2-D input, squeezed output, reference defaults (adj=1)
     g = rng.standard_normal((50, 30)).astype(np.float32)
     out = ps.smoothc(g, rect=[1, 5, 1])
@@ -280,6 +286,24 @@ def test_bandpassc_bit_exact(ps, shape, monkeypatch):
     assert out.shape == g.shape
     with pytest.raises(NotImplementedError):
         bandpass.bandpass_device(d, shape, 0.004, 5.0, 40.0, 300, 6, 1)
+
+
+def test_smoothc_bandpassc_full_size_bit_exact(ps):
+    """At the benchmark's 512 x 256 x 256: smoothc (rect 5,5,5, and a
+    derivative / box mix) and bandpassc (zero and minimum phase) bit for bit
+    against the FP32 oracle."""
+    from oracle import oracle as O
+    from pyseistr_amd import bandpass, smooth
+    shape = (512, 256, 256)
+    x = np.random.default_rng(31).standard_normal(int(np.prod(shape))).astype(np.float32)
+    d = _dev(x)
+    for rect, diff, box in [((5, 5, 5), (0, 0, 0), (0, 0, 0)), ((3, 9, 2), (0, 1, 0), (1, 0, 0))]:
+        got = smooth.smooth_device(d, shape, rect, diff, box).cpu().numpy()
+        assert np.array_equal(got.view(np.uint32), O.smoothc(x, shape, rect, diff, box).view(np.uint32)), rect
+    for phase in (0, 1):
+        got = bandpass.bandpass_device(d, shape, 0.004, 5.0, 60.0, 6, 6, phase).cpu().numpy()
+        exp = O.bandpass(x, shape, 0.004, 5.0, 60.0, 6, 6, phase)
+        assert np.array_equal(got.view(np.uint32), exp.view(np.uint32)), phase
 
 
 def test_dip_directions_on_two_streams_identical(ps, monkeypatch):
