@@ -15,15 +15,17 @@
 
 #include <hipfft/hipfft.h>
 
+#include <cstdlib>
+
 namespace {
 
 struct FkPlans {
-    int n1 = 0, n2 = 0;
+    int n1 = 0, n2 = 0, two_d = 0;
     hipfftHandle t_fwd = nullptr, x = nullptr, t_inv = nullptr;  // time R2C, space C2C, time C2R
+    hipfftHandle f2 = nullptr, i2 = nullptr;                     // both axes at once: R2C, C2R
     ~FkPlans() {
-        if (t_fwd) hipfftDestroy(t_fwd);
-        if (x) hipfftDestroy(x);
-        if (t_inv) hipfftDestroy(t_inv);
+        for (hipfftHandle h : {t_fwd, x, t_inv, f2, i2})
+            if (h) hipfftDestroy(h);
     }
 };
 
@@ -33,12 +35,13 @@ int pow2_at_least(int n) {
     return p;
 }
 
-// R[c][i] (nf per trace) = in[c][i] for i < n1, zero beyond
-__global__ void fk_stage(const double *__restrict__ in, double *__restrict__ R, int n1, int nf, int n2) {
+// R[c][i] (nf per trace, nc traces) = in[c][i] for i < n1 and c < n2, zero
+// beyond
+__global__ void fk_stage(const double *__restrict__ in, double *__restrict__ R, int n1, int nf, int n2, int nc) {
     const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= (long)nf * n2) return;
+    if (k >= (long)nf * nc) return;
     const int i = (int)(k % nf), c = (int)(k / nf);
-    R[k] = i < n1 ? in[(long)c * n1 + i] : 0.0;
+    R[k] = i < n1 && c < n2 ? in[(long)c * n1 + i] : 0.0;
 }
 
 // n1 == 1: the length-1 time transform is the identity, so the spectrum
@@ -95,7 +98,8 @@ int fft_fail(sh_ctx *ctx, const char *what, hipfftResult r) {
 // plans for (n1, n2), cached on the context
 int fk_plans(sh_ctx *ctx, int n1, int n2, FkPlans *&out) {
     auto *p = static_cast<FkPlans *>(ctx->fk.get());
-    if (p && p->n1 == n1 && p->n2 == n2) {
+    const int want_2d = pow2_at_least(n1) > 1 && pow2_at_least(n2) > 1 && !getenv("SH_FK_1D");
+    if (p && p->n1 == n1 && p->n2 == n2 && p->two_d == want_2d) {
         out = p;
         return SH_OK;
     }
@@ -104,6 +108,19 @@ int fk_plans(sh_ctx *ctx, int n1, int n2, FkPlans *&out) {
     if (!np_) return sh_fail(ctx, SH_EALLOC, "sh_fkdip: out of host memory");
     ctx->fk = std::shared_ptr<void>(np_, [](void *q) { delete static_cast<FkPlans *>(q); });
     const int nf = pow2_at_least(n1), nk = pow2_at_least(n2), nh = nf / 2 + 1;
+    if (nf > 1 && nk > 1 && !getenv("SH_FK_1D")) {
+        // the time transform of every padded trace followed by the space
+        // transform of every frequency is the 2-D real transform of the
+        // nk x nf array (time contiguous); the library then picks its own
+        // column kernels for the space axis
+        FK_FFT(ctx, "2-D plan", hipfftPlan2d(&np_->f2, nk, nf, HIPFFT_D2Z));
+        FK_FFT(ctx, "2-D plan", hipfftPlan2d(&np_->i2, nk, nf, HIPFFT_Z2D));
+        np_->two_d = 1;
+        np_->n1 = n1;
+        np_->n2 = n2;
+        out = np_;
+        return SH_OK;
+    }
     if (nf > 1) {
         int n[1] = {nf}, re[1] = {nf}, ce[1] = {nh};
         FK_FFT(ctx, "time plan", hipfftPlanMany(&np_->t_fwd, 1, n, re, 1, nf, ce, 1, nh, HIPFFT_D2Z, n2));
@@ -129,22 +146,34 @@ extern "C" int sh_fkdip(sh_ctx *ctx, const double *in, double *out, int n1, int 
     const int nf = pow2_at_least(n1), nk = pow2_at_least(n2), nh = nf / 2 + 1;
     const double nw = w * nk;
     if (nw == 0.0) return sh_fail(ctx, SH_EINVAL, "sh_fkdip: w * nk is zero (the cone slope divides by it)");
-    if ((long)nh * nk > (1L << 31) || (long)nf * n2 > (1L << 31))
+    if ((long)nh * nk > (1L << 31) || (long)nf * nk > (1L << 31))
         return sh_fail(ctx, SH_EINVAL, "sh_fkdip: %d x %d spectrum exceeds one transform batch", nh, nk);
     const double slope = (double)nh / nw;
     FkPlans *pl;
     SH_TRY(fk_plans(ctx, n1, n2, pl));
     void *rb = nullptr, *cb = nullptr;
-    SH_TRY(sh_slot(ctx, SLOT_FK0, sizeof(double) * (size_t)nf * n2, &rb));
+    const int nc = pl->two_d ? nk : n2;  // traces held in the real buffer
+    SH_TRY(sh_slot(ctx, SLOT_FK0, sizeof(double) * (size_t)nf * nc, &rb));
     SH_TRY(sh_slot(ctx, SLOT_FK0 + 1, sizeof(double2) * (size_t)nh * nk, &cb));
     double *R = (double *)rb;
     double2 *C = (double2 *)cb;
     hipStream_t st = ctx->stream;
-    for (hipfftHandle h : {pl->t_fwd, pl->x, pl->t_inv})
+    for (hipfftHandle h : {pl->t_fwd, pl->x, pl->t_inv, pl->f2, pl->i2})
         if (h) FK_FFT(ctx, "set stream", hipfftSetStream(h, st));
     const int T = 256;
+    const double scale = 1.0 / ((double)nf * (double)nk);
+    if (pl->two_d) {
+        hipLaunchKernelGGL(fk_stage, dim3(ceil_div((long)nf * nk, T)), dim3(T), 0, st, in, R, n1, nf, n2, nk);
+        SH_TRY(sh_launch_check(ctx, "fk_stage"));
+        FK_FFT(ctx, "2-D transform", hipfftExecD2Z(pl->f2, R, (hipfftDoubleComplex *)C));
+        hipLaunchKernelGGL(fk_cone, dim3(ceil_div((long)nh * nk, T)), dim3(T), 0, st, C, nh, nk, slope);
+        SH_TRY(sh_launch_check(ctx, "fk_cone"));
+        FK_FFT(ctx, "2-D transform", hipfftExecZ2D(pl->i2, (hipfftDoubleComplex *)C, R));
+        hipLaunchKernelGGL(fk_crop, dim3(ceil_div((long)n1 * n2, T)), dim3(T), 0, st, R, out, n1, nf, n2, scale);
+        return sh_launch_check(ctx, "fk_crop");
+    }
     if (nf > 1) {
-        hipLaunchKernelGGL(fk_stage, dim3(ceil_div((long)nf * n2, T)), dim3(T), 0, st, in, R, n1, nf, n2);
+        hipLaunchKernelGGL(fk_stage, dim3(ceil_div((long)nf * n2, T)), dim3(T), 0, st, in, R, n1, nf, n2, n2);
         SH_TRY(sh_launch_check(ctx, "fk_stage"));
         if (nk > n2) {
             hipLaunchKernelGGL(fk_zero, dim3(ceil_div((long)nh * (nk - n2), T)), dim3(T), 0, st, C, nh, n2, nk);
@@ -163,7 +192,6 @@ extern "C" int sh_fkdip(sh_ctx *ctx, const double *in, double *out, int n1, int 
     if (nk > 1)
         FK_FFT(ctx, "space transform",
                hipfftExecZ2Z(pl->x, (hipfftDoubleComplex *)C, (hipfftDoubleComplex *)C, HIPFFT_BACKWARD));
-    const double scale = 1.0 / ((double)nf * (double)nk);
     if (nf > 1) {
         FK_FFT(ctx, "time transform", hipfftExecZ2D(pl->t_inv, (hipfftDoubleComplex *)C, R));
         hipLaunchKernelGGL(fk_crop, dim3(ceil_div((long)n1 * n2, T)), dim3(T), 0, st, R, out, n1, nf, n2, scale);

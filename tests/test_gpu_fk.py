@@ -42,6 +42,20 @@ def test_fkdip_matches_oracle(fk, shape, w, dt):
         assert _rel(got, F.fkdip(d, w)) <= TOL_F32_INPUT
 
 
+@pytest.mark.parametrize("shape", [(1000, 300), (513, 65), (64, 64)])
+def test_one_dimensional_transforms_agree(fk, monkeypatch, shape):
+    """SH_FK_1D=1 runs the time and space transforms as separate batched 1-D
+    transforms (the path taken anyway when an axis has one sample) instead
+    of one 2-D real transform; both meet the float64 bar."""
+    d = np.random.default_rng(shape[1]).standard_normal(shape)
+    ref = F.fkdip(d, 0.03)
+    two = fk.fkdip(d, 0.03)
+    monkeypatch.setenv("SH_FK_1D", "1")
+    one = fk.fkdip(d, 0.03)
+    monkeypatch.delenv("SH_FK_1D")
+    assert _rel(one, ref) <= TOL and _rel(two, ref) <= TOL
+
+
 def test_repeated_and_resized_calls(fk):
     # the plans are cached per size: alternate sizes and repeat
     rng = np.random.default_rng(3)
