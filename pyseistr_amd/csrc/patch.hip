@@ -19,8 +19,13 @@ struct PGeo {
 // e1 + l1*e2, so runs of l1 samples are contiguous in both the volume and
 // the patch row) and walk axis 3 with fixed strides; the only divisions are
 // one per plane position and the block's patch-number split.
-__global__ void patch_cut(const double *__restrict__ in, double *__restrict__ X, PGeo g, long L) {
-    const int pid = blockIdx.x;
+// Blocks are dealt to the eight XCDs in turn; block b takes patch
+// (b % 8) * per + b / 8, so each XCD works through a contiguous run of patch
+// numbers and the overlapping neighbours share its L2.
+__global__ void patch_cut(const double *__restrict__ in, double *__restrict__ X, PGeo g, long L, int npatch,
+                          int per) {
+    const int pid = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
+    if (pid >= npatch) return;
     const int p3 = pid % g.P[2], p2 = (pid / g.P[2]) % g.P[1], p1 = pid / (g.P[1] * g.P[2]);
     const int b1 = p1 * g.s[0], b2 = p2 * g.s[1], b3 = p3 * g.s[2];
     const int l1 = g.l[0], l12 = g.l[0] * g.l[1], l3 = g.l[2];
@@ -33,6 +38,13 @@ __global__ void patch_cut(const double *__restrict__ in, double *__restrict__ X,
         const double *src = in + i1 + (long)g.n[0] * i2 + n12 * b3;
         const int m = ok ? m3 : 0;
         int e3 = 0;
+        for (; e3 + 8 <= m; e3 += 8) {  // eight loads in flight before their stores
+            double v[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) v[u] = src[(e3 + u) * n12];
+#pragma unroll
+            for (int u = 0; u < 8; u++) row[q + (long)(e3 + u) * l12] = v[u];
+        }
         for (; e3 < m; e3++) row[q + (long)e3 * l12] = src[e3 * n12];
         for (; e3 < l3; e3++) row[q + (long)e3 * l12] = 0.0;
     }
@@ -93,7 +105,7 @@ int geometry(sh_ctx *ctx, int n1, int n2, int n3, int l1, int l2, int l3, int s1
         np *= g.P[a];
     }
     L = (long)l1 * l2 * l3;
-    if (np >= (1L << 31) || (long)n2 * n3 >= (1L << 31) || (long)l1 * l2 >= (1L << 31))
+    if (np >= (1L << 31) - 16 || (long)n2 * n3 >= (1L << 31) || (long)l1 * l2 >= (1L << 31))
         return sh_fail(ctx, SH_EINVAL, "patch: %ld patches or %ld columns exceed one launch", np, (long)n2 * n3);
     return SH_OK;
 }
@@ -117,7 +129,8 @@ extern "C" int sh_patch(sh_ctx *ctx, const double *in, double *X, int n1, int n2
     long np, L;
     SH_TRY(geometry(ctx, n1, n2, n3, l1, l2, l3, s1, s2, s3, g, np, L));
     const int tpb = (int)std::min<long>(256, ceil_div((long)l1 * l2, 64) * 64);
-    hipLaunchKernelGGL(patch_cut, dim3((unsigned)np), dim3(tpb), 0, ctx->stream, in, X, g, L);
+    const int per = (int)ceil_div(np, 8);
+    hipLaunchKernelGGL(patch_cut, dim3((unsigned)(8L * per)), dim3(tpb), 0, ctx->stream, in, X, g, L, (int)np, per);
     return sh_launch_check(ctx, "patch_cut");
 }
 

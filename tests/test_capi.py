@@ -35,3 +35,30 @@ def test_no_cpu_fallback():
     import pyseistr_amd as ps
     with pytest.raises(ps.SeistrError):
         ps.dip2dc(np.zeros((16, 8), np.float32))
+
+
+def test_patch_count_host_call():
+    """sh_patch_count is host arithmetic only: it agrees with the patch
+    oracle's counts and rejects a patch that fits nowhere."""
+    import numpy as np
+    import pytest
+    from oracle import patch_oracle as P
+    from pyseistr_amd import patch
+    for n, l, s in [((100, 60, 1), (8, 8, 1), (4, 4, 1)), ((101, 37, 9), (16, 10, 3), (8, 3, 2)),
+                    ((20, 9, 3), (3, 12, 2), (5, 4, 3)), ((3, 5, 4), (5, 5, 2), (3, 3, 1))]:
+        assert patch.patch_count(n, l, s) == int(np.prod(P.counts(n, l, s)))
+    with pytest.raises(ValueError):
+        patch.patch_count((3, 4, 1), (5, 2, 1), (2, 1, 1))
+
+
+def test_neighbours_have_no_cpu_fallback():
+    import torch
+    if torch.cuda.is_available():
+        return
+    import numpy as np
+    import pytest
+    import pyseistr_amd as ps
+    for call in (lambda: ps.patch2d(np.zeros((16, 16))), lambda: ps.fkdip(np.zeros((16, 8)), 0.1),
+                 lambda: ps.smoothc(np.zeros((16, 8)), rect=[3, 1, 1]), lambda: ps.bandpassc(np.zeros(64), 0.004)):
+        with pytest.raises(ps.SeistrError):
+            call()
