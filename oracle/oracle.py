@@ -245,9 +245,16 @@ def soint3d(din, mask, dipi, dipx, shape, order=1, niter=100, njs=(1, 1), drift=
 
 
 def soint2d(din, mask, dip, shape, order=1, niter=100, njs=(1, 1), drift=0, hasmask=1, twoplane=0,
-            dtype=np.float32):
+            prec=0, dip2=None, dtype=np.float32):
+    """csoint2d; prec=1 is the preconditioned solve (dip2 = the second plane's
+    slopes when twoplane is set)."""
     n1, n2 = (int(v) for v in shape)
     a = _arr(din, dtype); m = _arr(mask, dtype); p = _arr(dip, dtype); o = np.zeros_like(a)
+    if prec:
+        q = _arr(dip2 if dip2 is not None else dip, dtype) if twoplane else None
+        _lib(dtype).or_soint2d_prec(_ptr(a), _ptr(m), _ptr(p), _ptr(q) if q is not None else None, n1, n2, order,
+                                    niter, hasmask, _ptr(o))
+        return o
     _lib(dtype).or_soint2d(_ptr(a), _ptr(m), _ptr(p), n1, n2, order, njs[0], niter, drift, hasmask,
                            twoplane, _ptr(o))
     return o

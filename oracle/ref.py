@@ -141,11 +141,12 @@ def sint3d(din, mask, dipi, dipx, shape, niter=100, eps=0.01, ns1=1, ns2=1, orde
 
 
 def soint2d(din, mask, dip, shape, order=1, niter=100, njs=(1, 1), drift=0, hasmask=1, twoplane=0,
-            prec=0):
+            prec=0, dip2=None):
     """soint2dcfun.csoint2d, format "OOOOiiiiiiiiiii" (soint2d_cfuns.c:2277)."""
     n1, n2 = shape
     with _quiet():
-        return _mod("soint2dcfun").csoint2d(_f(din), _f(mask), _f(dip), _f(dip), n1, n2, order, njs[0],
+        return _mod("soint2dcfun").csoint2d(_f(din), _f(mask), _f(dip), _f(dip if dip2 is None else dip2), n1, n2,
+                                            order, njs[0],
                                             njs[1], niter, drift, hasmask, twoplane, prec, 0)
 
 

@@ -1458,6 +1458,100 @@ EXPORT int or_soint2d(const real *din, const real *mask, const real *dip, int n1
     return it;
 }
 
+/* predict_lop (soint2d_cfuns.c:1537-1574) without add: yy = P xx, or
+ * xx = P^T yy.  A running trace tt walks the traces; each step is the
+ * forward-direction prediction (predict_step with forw) through slope
+ * trace i2 of dip. */
+static void prec_lop(predictor *w, int adj, int n2, const real *dip, real *xx, real *yy, real *tt) {
+    int n1 = w->n1;
+    size_t n = (size_t)n1 * n2;
+    for (size_t i = 0; i < n; i++) { if (adj) xx[i] = 0; else yy[i] = 0; }
+    for (int i1 = 0; i1 < n1; i1++) tt[i1] = 0;
+    if (adj) {
+        for (int i2 = n2 - 1; i2 >= 0; i2--) {
+            pred_step(w, 1, 1, tt, dip + (size_t)i2 * n1);
+            for (int i1 = 0; i1 < n1; i1++) tt[i1] += yy[i1 + (size_t)i2 * n1];
+            for (int i1 = 0; i1 < n1; i1++) xx[i1 + (size_t)i2 * n1] += tt[i1];
+        }
+    } else {
+        for (int i2 = 0; i2 < n2; i2++) {
+            for (int i1 = 0; i1 < n1; i1++) tt[i1] += xx[i1 + (size_t)i2 * n1];
+            for (int i1 = 0; i1 < n1; i1++) yy[i1 + (size_t)i2 * n1] += tt[i1];
+            pred_step(w, 0, 1, tt, dip + (size_t)i2 * n1);
+        }
+    }
+}
+
+/* The preconditioner: predict_lop through dip, or predict2_lop
+ * (soint2d_cfuns.c:1597-1616) = P(qq) P(pp) when qq is set. */
+typedef struct { predictor w; int n2; const real *pp, *qq; real *tt, *tmp; } prec_ctx;
+static void prec_apply(prec_ctx *c, int adj, real *x, real *y) {
+    if (!c->qq) { prec_lop(&c->w, adj, c->n2, c->pp, x, y, c->tt); return; }
+    if (adj) {
+        prec_lop(&c->w, 1, c->n2, c->qq, c->tmp, y, c->tt);
+        prec_lop(&c->w, 1, c->n2, c->pp, x, c->tmp, c->tt);
+    } else {
+        prec_lop(&c->w, 0, c->n2, c->pp, x, c->tmp, c->tt);
+        prec_lop(&c->w, 0, c->n2, c->qq, c->tmp, y, c->tt);
+    }
+}
+
+/* csoint2d with prec (soint2d_cfuns.c:2352-2396): ps_solver_prec
+ * (1704-1990) with the masked data operator, this preconditioner, eps = 0
+ * and no initial model, so the model is x = P p with p minimising
+ * |M P p - d|.  The solver's model temporary is the output itself (the
+ * reference aliases x and dat); d is read only at the start.  Vectors p and
+ * g have 2n entries (the eps-scaled data part stays zero). */
+EXPORT int or_soint2d_prec(const real *din, const real *mask, const real *dip, const real *dip2, int n1,
+                           int n2, int nw, int niter, int hasmask, real *out) {
+    const real TOLERANCE = (real)1.e-12f;
+    const real eps = 0;
+    size_t n = (size_t)n1 * n2;
+    unsigned char *known = xcalloc(n, 1);
+    for (size_t i = 0; i < n; i++) {
+        out[i] = din[i];
+        known[i] = hasmask == 1 ? (mask[i] != 0.) : (din[i] != 0.);
+    }
+    prec_ctx c;
+    pred_init(&c.w, n1, nw, (real)0.0001f);
+    c.n2 = n2; c.pp = dip; c.qq = dip2;
+    c.tt = xcalloc(n1, sizeof(real));
+    c.tmp = dip2 ? xcalloc(n, sizeof(real)) : NULL;
+    real *p = xcalloc(2 * n, sizeof(real)), *g = xcalloc(2 * n, sizeof(real));
+    real *rr = xcalloc(n, sizeof(real)), *gg = xcalloc(n, sizeof(real)), *x = xcalloc(n, sizeof(real));
+    cgstep_state st = {xcalloc(2 * n, sizeof(real)), xcalloc(n, sizeof(real)), 0};
+    for (size_t i = 0; i < n; i++) rr[i] = -din[i];
+    double dprr0 = 1., dpgm0 = 1.;
+    int iter;
+    for (iter = 0; iter < niter; iter++) {
+        for (size_t i = 0; i < n; i++) x[i] = known[i] ? (real)0 + rr[i] : (real)0;
+        prec_apply(&c, 1, g, x);
+        for (size_t i = 0; i < n; i++) g[i + n] = eps * rr[i];
+        prec_apply(&c, 0, g, x);
+        for (size_t i = 0; i < n; i++) gg[i] = known[i] ? (real)0 + x[i] : (real)0;
+        for (size_t i = 0; i < n; i++) gg[i] += eps * g[i + n];
+        double dprr, dpgm;
+        if (iter == 0) {
+            dprr0 = snrm2(n, rr);
+            dpgm0 = snrm2(n, g);
+            dprr = 1.;
+            dpgm = 1.;
+        } else {
+            dprr = snrm2(n, rr) / dprr0;
+            dpgm = snrm2(n, g) / dpgm0;
+        }
+        if (dprr < TOLERANCE || dpgm < TOLERANCE) {
+            prec_apply(&c, 0, p, out);
+            break;
+        }
+        cgstep(&st, 0, 2 * n, n, p, g, rr, gg);
+        if (iter == niter - 1) prec_apply(&c, 0, p, out);
+    }
+    pred_free(&c.w);
+    free(c.tt); free(c.tmp); free(known); free(p); free(g); free(rr); free(gg); free(x); free(st.S); free(st.Ss);
+    return iter;
+}
+
 /* Masked data operator (ps_mask_lop, soint3d_cfuns.c:1359-1372). */
 typedef struct { const unsigned char *m; size_t n; } mask_ctx;
 static void mask_op(void *c, int adj, int add, real *x, real *y) {

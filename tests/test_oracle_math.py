@@ -432,3 +432,61 @@ def test_bandpass_frequency_response(nplo):
     y = O.bandpass(pulse, (n, 1, 1), dt, flo=5.0, fhi=40.0, nplo=6, nphi=6, phase=0, dtype=np.float64)
     m = 300
     assert np.abs(y[n // 2 - m:n // 2] - y[n // 2 + m:n // 2:-1]).max() < 1e-3 * np.abs(y).max()
+
+
+SOINT_PREC_CASES = [((40, 24), 1, 10, 1, 0), ((60, 30), 2, 25, 1, 0), ((33, 17), 2, 8, 0, 0),
+                    ((40, 24), 2, 12, 1, 1), ((50, 20), 1, 3, 1, 1)]
+
+
+def _soint_prec_case(shape, hasmask, seed):
+    rng = np.random.default_rng(seed)
+    d = rng.standard_normal(shape).astype(np.float32)
+    mask = (rng.random(shape) > 0.5).astype(np.float32)
+    if hasmask == 0:
+        d = d * mask
+    return d, mask, (0.7 * rng.standard_normal(shape)).astype(np.float32), \
+        (0.5 * rng.standard_normal(shape)).astype(np.float32)
+
+
+def test_soint2d_prec_matches_reference():
+    """The FP32 restatement of csoint2d with prec (predict_lop or
+    predict2_lop preconditioner, ps_solver_prec) against the reference's
+    own csoint2d (oracle/_ref), bit for bit."""
+    from oracle import ref as R
+    if not R.available():
+        pytest.skip("reference modules not built")
+    for k, (shape, order, niter, hasmask, twoplane) in enumerate(SOINT_PREC_CASES):
+        d, mask, p, q = _soint_prec_case(shape, hasmask, k)
+        f = lambda a: a.ravel(order="F")  # noqa: E731
+        r = np.asarray(R.soint2d(f(d), f(mask), f(p), shape, order=order, niter=niter, hasmask=hasmask,
+                                 twoplane=twoplane, prec=1, dip2=f(q) if twoplane else None), np.float32).ravel()
+        o = O.soint2d(f(d), f(mask), f(p), shape, order=order, niter=niter, hasmask=hasmask, twoplane=twoplane,
+                      prec=1, dip2=f(q))
+        assert np.array_equal(r.view(np.uint32), o.view(np.uint32)), (shape, order, niter, hasmask, twoplane)
+
+
+@pytest.mark.parametrize("twoplane", [0, 1])
+def test_soint2d_prec_residual_decreases(twoplane):
+    """In float64 the preconditioned CG fits the known samples ever better:
+    |M x_k - d| does not grow with the iteration count k (a wrong adjoint or
+    a transposed preconditioner breaks this), and with every sample known
+    the one-plane solve approaches the data."""
+    shape = (36, 14)
+    d, mask, _, _ = _soint_prec_case(shape, 1, 7)
+    # gently varying slopes keep the preconditioned system well conditioned
+    p = np.repeat((0.5 * np.sin(np.arange(36) / 6.0))[:, None], 14, axis=1).astype(np.float32)
+    q = np.full(shape, 0.3, np.float32)
+    f = lambda a: a.ravel(order="F")  # noqa: E731
+    m = f(mask) != 0
+    res = []
+    for k in range(1, 41):
+        x = O.soint2d(f(d), f(mask), f(p), shape, order=2, niter=k, hasmask=1, twoplane=twoplane, prec=1, dip2=f(q),
+                      dtype=np.float64)
+        res.append(np.linalg.norm((x - f(d).astype(np.float64))[m]))
+    assert all(b <= a * (1 + 1e-9) for a, b in zip(res, res[1:])), res
+    # a product of two predictions (twoplane) converges more slowly
+    assert res[-1] < (0.6 if twoplane else 0.1) * res[0]
+    if not twoplane:
+        full = np.ones(shape, np.float32)
+        x = O.soint2d(f(d), f(full), f(p), shape, order=2, niter=60, hasmask=1, prec=1, dtype=np.float64)
+        assert np.linalg.norm(x - f(d)) < 0.05 * np.linalg.norm(f(d))
