@@ -168,11 +168,24 @@ int sh_soint3d(sh_ctx *ctx, const float *din, const float *mask, const float *di
                int niter, int drift, int seed, int hasmask, float var, float *out, int *iters);
 
 /* 2-D analogue without preconditioning.  Replaces soint2dcfun.csoint2d
- * (pyseistr/src/soint2d_cfuns.c:2260-2419) for prec == 0; twoplane != 0
- * returns the data unchanged, as the reference does. */
+ * (pyseistr/src/soint2d_cfuns.c:2260-2419) for prec == 0 (sh_soint2d_prec
+ * below for prec != 0); twoplane != 0 returns the data unchanged, as the
+ * reference does. */
 int sh_soint2d(sh_ctx *ctx, const float *din, const float *mask, const float *dip,
                int n1, int n2, int nw, int nj1, int niter, int drift, int hasmask,
                int twoplane, float *out, int *iters);
+
+/* The preconditioned form (prec != 0) of soint2dcfun.csoint2d
+ * (pyseistr/src/soint2d_cfuns.c:2352-2396): out = P p with p minimising
+ * |M P p - din| by ps_solver_prec (1704-1990; eps = 0), P = predict_lop
+ * (1537-1574) through dip, or predict2_lop (1597-1616) = P(dip2) P(dip)
+ * when dip2 is non-NULL (twoplane).  M keeps the known samples (mask != 0,
+ * or din != 0 when hasmask != 1).  The trace-to-trace prediction chain is
+ * sequential and runs in one thread; its banded factors are built in
+ * parallel.  *iters (nullable) receives the iterations run. */
+int sh_soint2d_prec(sh_ctx *ctx, const float *din, const float *mask, const float *dip,
+                    const float *dip2, int n1, int n2, int nw, int niter, int hasmask,
+                    float *out, int *iters);
 
 /* Masked shaping-CG interpolation with structure-oriented smoothing as the
  * shaping operator.  Replaces soint3dcfun.csint3d (pyseistr/src/

@@ -17,6 +17,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
+#include <cstring>
 #include <vector>
 
 void sh_mt_randn(unsigned seed, size_t n, float *out);
@@ -449,6 +451,416 @@ int sint_cg(sh_ctx *ctx, Shape3 &Sh, const unsigned char *known, float *p, float
     return SH_OK;
 }
 
+// ---------------------------------------------------------------------------
+// soint2dc with prec: the preconditioned solve of csoint2d
+// (soint2d_cfuns.c:2352-2396) with predict_lop / predict2_lop (1537-1616)
+// as the preconditioner and ps_solver_prec (1704-1990).  The model is
+// x = P p; P runs a trace along i2 and predicts it trace to trace.  Each
+// prediction's banded factor depends only on its slope trace, so all
+// factors are built in parallel (prec_factor); the trace-to-trace chain is
+// inherently sequential (every step needs the whole previous trace, and
+// each solve is itself a recurrence along i1), so it runs in one thread
+// with the cached factors (prec_chain).  Arithmetic per step is the
+// spray's predictor, operation for operation.
+// ---------------------------------------------------------------------------
+#include "predict.hpp"
+
+// factor of the forward-direction prediction through slope trace i2
+// (F layout [i2][NB + 1][n1])
+template <int NW>
+__global__ void prec_factor(const float *__restrict__ dip, float *__restrict__ F, int n1, int n2, float e) {
+    const int i2 = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i2 >= n2) return;
+    constexpr int NB = 2 * NW;
+    Acc q[1] = {Acc{dip + (long)i2 * n1, 1}};
+    bool rev[1] = {true};
+    factor_sweep<NW, 1>(q, rev, n1, e, Sink{F + (long)i2 * (NB + 1) * n1, 1}, n1);
+}
+
+// predict_lop without add: y = P x (adj = 0) or x = P^T y (adj = 1); tt and
+// ys are one trace each of scratch.  One thread.
+template <int NW>
+__global__ void prec_chain(int adj, float *__restrict__ x, float *__restrict__ y, const float *__restrict__ dip,
+                           const float *__restrict__ F, float *tt, float *ys, int n1, int n2, float e) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    constexpr int NB = 2 * NW;
+    for (int i1 = 0; i1 < n1; i1++) tt[i1] = 0.0f;
+    bool rev[1] = {true};
+    if (adj) {
+        for (int i2 = n2 - 1; i2 >= 0; i2--) {
+            const long o = (long)i2 * n1;
+            predict_adj_factored<NW>(Acc{tt, 1}, Acc{dip + o, 1}, true, F + (long)i2 * (NB + 1) * n1, 1, n1,
+                                     Sink{ys, 1}, n1, e, Sink{tt, 1});
+            for (int i1 = 0; i1 < n1; i1++) tt[i1] += y[o + i1];
+            for (int i1 = 0; i1 < n1; i1++) x[o + i1] = 0.0f + tt[i1];
+        }
+    } else {
+        for (int i2 = 0; i2 < n2; i2++) {
+            const long o = (long)i2 * n1;
+            for (int i1 = 0; i1 < n1; i1++) tt[i1] += x[o + i1];
+            for (int i1 = 0; i1 < n1; i1++) y[o + i1] = 0.0f + tt[i1];
+            Acc in[1] = {Acc{tt, 1}}, q[1] = {Acc{dip + o, 1}};
+            solve_factored_pf<NW, 1, 8>(in, q, rev, F + (long)i2 * (NB + 1) * n1, 1, n1, Sink{ys, 1}, n1, e,
+                                        Sink{tt, 1});
+        }
+    }
+}
+
+// prec_chain with the current trace's factor and slopes, the running trace
+// and the solve scratch in LDS ((NB + 4) * n1 floats): the block stages
+// them and does the element-wise steps, thread 0 runs the substitutions
+// from LDS.  Same operations in the same order as prec_chain.
+template <int NW>
+__global__ void __launch_bounds__(256)
+prec_chain_lds(int adj, float *__restrict__ x, float *__restrict__ y, const float *__restrict__ dip,
+               const float *__restrict__ F, int n1, int n2, float e) {
+    extern __shared__ float sm[];
+    constexpr int NB = 2 * NW;
+    float *Fs = sm, *ds = Fs + (NB + 1) * n1, *tt = ds + n1, *ys = tt + n1;
+    const int t = threadIdx.x, T = blockDim.x;
+    for (int i = t; i < n1; i += T) tt[i] = 0.0f;
+    bool rev[1] = {true};
+    for (int s = 0; s < n2; s++) {
+        const int i2 = adj ? n2 - 1 - s : s;
+        const long o = (long)i2 * n1;
+        const float *Fi = F + (long)i2 * (NB + 1) * n1;
+        for (int i = t; i < (NB + 1) * n1; i += T) Fs[i] = Fi[i];
+        for (int i = t; i < n1; i += T) ds[i] = dip[o + i];
+        if (!adj)
+            for (int i = t; i < n1; i += T) {
+                const float v = tt[i] + x[o + i];
+                tt[i] = v;
+                y[o + i] = 0.0f + v;
+            }
+        __syncthreads();
+        if (t == 0) {
+            if (adj) {
+                predict_adj_factored<NW>(Acc{tt, 1}, Acc{ds, 1}, true, Fs, 1, n1, Sink{ys, 1}, n1, e, Sink{tt, 1});
+            } else {
+                Acc in[1] = {Acc{tt, 1}}, q[1] = {Acc{ds, 1}};
+                solve_factored<NW, 1>(in, q, rev, Fs, 1, n1, Sink{ys, 1}, n1, e, Sink{tt, 1});
+            }
+        }
+        __syncthreads();
+        if (adj)
+            for (int i = t; i < n1; i += T) {
+                const float v = tt[i] + y[o + i];
+                tt[i] = v;
+                x[o + i] = 0.0f + v;
+            }
+    }
+}
+
+// Substitutions of one prediction with a stored factor Fs (plane 0 = d,
+// plane 1 + m = o[m]; n1 per plane): forward b[k] = src[k] - ...,
+// y[k] = b[k] / d[k], then backward into out (which may be y).  The loops of
+// solve_factored / predict_adj_factored, operation for operation.  (Loads
+// issued ahead into register rings measured slower here: the operands are
+// in LDS.)
+template <int NW>
+__device__ void subst_factored(const float *src, const float *Fs, float *y, int n1, float *out) {
+    constexpr int NB = 2 * NW;
+    float Lo[NB][NB], Bw[NB];
+#pragma unroll
+    for (int r = 0; r < NB; r++) {
+        Bw[r] = 0.0f;
+#pragma unroll
+        for (int m = 0; m < NB; m++) Lo[r][m] = 0.0f;
+    }
+    for (int k = 0; k < n1; k++) {
+        const int m1 = k < NB ? k : NB;
+        float b = src[k];
+#pragma unroll
+        for (int m = 0; m < NB; m++)
+            if (m < m1) b -= (Lo[m][m]) * Bw[m];
+        y[k] = b / Fs[k];
+        float ok[NB];
+#pragma unroll
+        for (int m = 0; m < NB; m++) ok[m] = Fs[(1 + m) * n1 + k];
+#pragma unroll
+        for (int r = NB - 1; r > 0; r--) {
+            Bw[r] = Bw[r - 1];
+#pragma unroll
+            for (int m = 0; m < NB; m++) Lo[r][m] = Lo[r - 1][m];
+        }
+        Bw[0] = b;
+#pragma unroll
+        for (int m = 0; m < NB; m++) Lo[0][m] = ok[m];
+    }
+    float nx[NB];
+#pragma unroll
+    for (int m = 0; m < NB; m++) nx[m] = 0.0f;
+    for (int k = n1 - 1; k >= 0; k--) {
+        float t = y[k];
+        const int m1 = (n1 - k - 1) < NB ? (n1 - k - 1) : NB;
+#pragma unroll
+        for (int m = 0; m < NB; m++)
+            if (m < m1) t -= Fs[(1 + m) * n1 + k] * nx[m];
+#pragma unroll
+        for (int m = NB - 1; m > 0; m--) nx[m] = nx[m - 1];
+        nx[0] = t;
+        out[k] = t;
+    }
+}
+
+// prec_chain with the per-sample work spread over the block: the filter
+// taps, the right-hand side A^T A tt + e * boundary (forward) and the
+// adjoint product A^T A s + e * boundary(s) are computed for all samples in
+// parallel, each sample's sums in the reference's order; only the
+// substitutions run in thread 0.  LDS: factor (NB + 1) n1, taps (NB + 1) n1,
+// tt, y, tmp, rhs (n1 each).
+template <int NW>
+__global__ void __launch_bounds__(256)
+prec_chain_par(int adj, float *__restrict__ x, float *__restrict__ y, const float *__restrict__ dip,
+               const float *__restrict__ F, int n1, int n2, float e) {
+    extern __shared__ float sm[];
+    constexpr int NB = 2 * NW, NA = 2 * NW + 1;
+    float *Fs = sm, *tp = Fs + (NB + 1) * n1, *tt = tp + NA * n1, *ys = tt + n1, *tm = ys + n1, *rh = tm + n1;
+    const int t = threadIdx.x, T = blockDim.x;
+    for (int i = t; i < n1; i += T) tt[i] = 0.0f;
+    for (int s = 0; s < n2; s++) {
+        const int i2 = adj ? n2 - 1 - s : s;
+        const long o = (long)i2 * n1;
+        const float *Fi = F + (long)i2 * (NB + 1) * n1;
+        for (int i = t; i < (NB + 1) * n1; i += T) Fs[i] = Fi[i];
+        const Acc q{dip + o, 1};
+        for (int i = t; i < n1; i += T) {
+            float a[NA];
+            taps_at<NW>(q, i, n1, true, a);
+#pragma unroll
+            for (int j = 0; j < NA; j++) tp[j * n1 + i] = a[j];
+        }
+        if (!adj) {
+            for (int i = t; i < n1; i += T) {
+                const float v = tt[i] + x[o + i];
+                tt[i] = v;
+                y[o + i] = 0.0f + v;
+            }
+            __syncthreads();
+            // tmp = A tt (pwd_set forward, first half)
+            for (int i = t; i < n1; i += T) {
+                float v = 0.0f;
+                if (i >= NW && i < n1 - NW) {
+#pragma unroll
+                    for (int j = 0; j < NA; j++) v += tp[j * n1 + i] * tt[i + j - NW];
+                }
+                tm[i] = v;
+            }
+            __syncthreads();
+            // rhs = A^T tmp + e * boundary of tt
+            for (int k = t; k < n1; k += T) {
+                float v = 0.0f;
+#pragma unroll
+                for (int j = 0; j < NA; j++) {
+                    const int kk = k + j - NW;
+                    if (kk >= NW && kk < n1 - NW) v += tp[j * n1 + kk] * tm[kk];
+                }
+                if (k == 0) v += e * tt[0];
+                if (k == 1) v += e * tt[1];
+                if (k == n1 - 2) v += e * tt[n1 - 2];
+                if (k == n1 - 1) v += e * tt[n1 - 1];
+                rh[k] = v;
+            }
+            __syncthreads();
+            if (t == 0) subst_factored<NW>(rh, Fs, ys, n1, tt);
+            __syncthreads();
+        } else {
+            __syncthreads();
+            if (t == 0) subst_factored<NW>(tt, Fs, ys, n1, ys);
+            __syncthreads();
+            // tmp = A s on the solved trace (pwd_set adjoint, gather form)
+            for (int i = t; i < n1; i += T) {
+                float v = 0.0f;
+                if (i >= NW && i < n1 - NW) {
+#pragma unroll
+                    for (int j = NA - 1; j >= 0; j--) v += tp[j * n1 + i] * ys[i - j + NW];
+                }
+                tm[i] = v;
+            }
+            __syncthreads();
+            for (int k = t; k < n1; k += T) {
+                float v = 0.0f;
+#pragma unroll
+                for (int w = 0; w < NA; w++) {
+                    const int i = k - NW + w;
+                    if (i >= NW && i < n1 - NW) v += tp[(2 * NW - w) * n1 + i] * tm[i];
+                }
+                if (k == 0) v += e * ys[0];
+                if (k == 1) v += e * ys[1];
+                if (k == n1 - 2) v += e * ys[n1 - 2];
+                if (k == n1 - 1) v += e * ys[n1 - 1];
+                v += y[o + k];
+                tt[k] = v;
+                x[o + k] = 0.0f + v;
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// xt = M^T rr (ps_mask_lop adjoint without add)
+__global__ void mask_adj(const unsigned char *__restrict__ known, const float *__restrict__ rr, float *__restrict__ xt,
+                         long n) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) xt[i] = known[i] ? 0.0f + rr[i] : 0.0f;
+}
+
+// gg = M xt, then gg += eps * (eps * rr) with eps = 0 (the solver's
+// regularisation rows, which only touch the sign of zero)
+__global__ void mask_fwd_gg(const unsigned char *__restrict__ known, const float *__restrict__ xt,
+                            const float *__restrict__ rr, float *__restrict__ gg, long n) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float v = known[i] ? 0.0f + xt[i] : 0.0f;
+    const float g2 = 0.0f * rr[i];
+    gg[i] = v + 0.0f * g2;
+}
+
+// P (adj = 0: y = P x) or P^T (adj = 1: x = P^T y) for one or two slope
+// planes (predict2_lop: P = P(q) P(p))
+template <int NW>
+int prec_apply(sh_ctx *ctx, int adj, float *x, float *y, const float *pp, const float *qq, const float *Fp,
+                      const float *Fq, float *tmp, float *tt, float *ys, int n1, int n2, float e) {
+    constexpr int NB = 2 * NW;
+    const size_t ldp = (size_t)(2 * NB + 6) * n1 * sizeof(float);
+    const char *mode = getenv("SH_SOINT_PREC_CHAIN");  // "lds" or "global": the slower kernels
+    if (ldp <= 65536 && !mode) {
+        auto run = [&](int a, float *xx, float *yy, const float *dp, const float *Fd) {
+            hipLaunchKernelGGL((prec_chain_par<NW>), dim3(1), dim3(256), ldp, ctx->stream, a, xx, yy, dp, Fd, n1, n2, e);
+        };
+        if (!qq) run(adj, x, y, pp, Fp);
+        else if (adj) {
+            run(1, tmp, y, qq, Fq);
+            run(1, x, tmp, pp, Fp);
+        } else {
+            run(0, x, tmp, pp, Fp);
+            run(0, tmp, y, qq, Fq);
+        }
+        return sh_launch_check(ctx, "prec_chain_par");
+    }
+    const size_t lds = (size_t)(NB + 4) * n1 * sizeof(float);
+    if (lds <= 65536 && !(mode && !strcmp(mode, "global"))) {
+        auto run = [&](int a, float *xx, float *yy, const float *dp, const float *Fd) {
+            hipLaunchKernelGGL((prec_chain_lds<NW>), dim3(1), dim3(256), lds, ctx->stream, a, xx, yy, dp, Fd, n1, n2, e);
+        };
+        if (!qq) run(adj, x, y, pp, Fp);
+        else if (adj) {
+            run(1, tmp, y, qq, Fq);
+            run(1, x, tmp, pp, Fp);
+        } else {
+            run(0, x, tmp, pp, Fp);
+            run(0, tmp, y, qq, Fq);
+        }
+        return sh_launch_check(ctx, "prec_chain_lds");
+    }
+    if (!qq) {
+        hipLaunchKernelGGL((prec_chain<NW>), dim3(1), dim3(64), 0, ctx->stream, adj, x, y, pp, Fp, tt, ys, n1, n2, e);
+        return sh_launch_check(ctx, "prec_chain");
+    }
+    if (adj) {
+        hipLaunchKernelGGL((prec_chain<NW>), dim3(1), dim3(64), 0, ctx->stream, 1, tmp, y, qq, Fq, tt, ys, n1, n2, e);
+        hipLaunchKernelGGL((prec_chain<NW>), dim3(1), dim3(64), 0, ctx->stream, 1, x, tmp, pp, Fp, tt, ys, n1, n2, e);
+    } else {
+        hipLaunchKernelGGL((prec_chain<NW>), dim3(1), dim3(64), 0, ctx->stream, 0, x, tmp, pp, Fp, tt, ys, n1, n2, e);
+        hipLaunchKernelGGL((prec_chain<NW>), dim3(1), dim3(64), 0, ctx->stream, 0, tmp, y, qq, Fq, tt, ys, n1, n2, e);
+    }
+    return sh_launch_check(ctx, "prec_chain");
+}
+
+// ps_solver_prec with the masked data operator, eps = 0, no initial model
+// (soint2d_cfuns.c:1704-1990 as csoint2d calls it)
+template <int NW>
+int soint2d_prec_solve(sh_ctx *ctx, const float *din, const unsigned char *known, const float *pp,
+                              const float *qq, int n1, int n2, int niter, float *out, int *iters_out) {
+    const long n = (long)n1 * n2;
+    constexpr int NB = 2 * NW;
+    const float e = 0.0001f;
+    const int nb = ceil_div(n, TB);
+    // F for pp (and qq), p, g, S, rr, gg, Ss, xt, tmp, tt, ys
+    const size_t nF = (size_t)(NB + 1) * n;
+    const size_t total = nF * (qq ? 2 : 1) + 8 * (size_t)n + 2 * (size_t)n1;
+    float *buf = nullptr;
+    SH_TRY(sh_slot(ctx, SLOT_INTERP0 + 10, total * sizeof(float), (void **)&buf));
+    float *Fp = buf, *Fq = qq ? Fp + nF : nullptr;
+    float *p = buf + nF * (qq ? 2 : 1), *gv = p + n, *S = gv + n, *rr = S + n, *gg = rr + n, *Ss = gg + n;
+    float *xt = Ss + n, *tmp = xt + n, *tt = tmp + n, *ys = tt + n1;
+    double *part = nullptr, *dsum = nullptr;
+    SH_TRY(sh_slot(ctx, SLOT_RED, (size_t)nb * 8 * sizeof(double), (void **)&part));
+    SH_TRY(sh_slot(ctx, SLOT_SCAL, 16 * sizeof(double), (void **)&dsum));
+    hipLaunchKernelGGL((prec_factor<NW>), dim3(ceil_div(n2, 64)), dim3(64), 0, ctx->stream, pp, Fp, n1, n2, e);
+    if (qq) hipLaunchKernelGGL((prec_factor<NW>), dim3(ceil_div(n2, 64)), dim3(64), 0, ctx->stream, qq, Fq, n1, n2, e);
+    hipLaunchKernelGGL(zero_f, dim3(nb), dim3(TB), 0, ctx->stream, p, n);
+    hipLaunchKernelGGL(zero_f, dim3(nb), dim3(TB), 0, ctx->stream, S, n);
+    hipLaunchKernelGGL(zero_f, dim3(nb), dim3(TB), 0, ctx->stream, Ss, n);
+    hipLaunchKernelGGL(neg_kernel, dim3(nb), dim3(TB), 0, ctx->stream, din, rr, n);
+    SH_TRY(sh_launch_check(ctx, "soint2d prec setup"));
+    const double TOL = (double)1.e-12f, EPSILON = (double)1.e-12f;
+    double dprr0 = 1.0, dpgm0 = 1.0, h[8];
+    bool started = false;
+    int it;
+    for (it = 0; it < niter; it++) {
+        hipLaunchKernelGGL(mask_adj, dim3(nb), dim3(TB), 0, ctx->stream, known, rr, xt, n);
+        SH_TRY(prec_apply<NW>(ctx, 1, gv, xt, pp, qq, Fp, Fq, tmp, tt, ys, n1, n2, e));
+        SH_TRY(prec_apply<NW>(ctx, 0, gv, xt, pp, qq, Fp, Fq, tmp, tt, ys, n1, n2, e));
+        hipLaunchKernelGGL(mask_fwd_gg, dim3(nb), dim3(TB), 0, ctx->stream, known, xt, rr, gg, n);
+        hipLaunchKernelGGL(dot_kernel, dim3(nb), dim3(TB), 0, ctx->stream, gg, gg, n, part, 0, nb);
+        hipLaunchKernelGGL(dot_kernel, dim3(nb), dim3(TB), 0, ctx->stream, gg, Ss, n, part, 1, nb);
+        hipLaunchKernelGGL(dot_kernel, dim3(nb), dim3(TB), 0, ctx->stream, gg, rr, n, part, 2, nb);
+        hipLaunchKernelGGL(dot_kernel, dim3(nb), dim3(TB), 0, ctx->stream, Ss, Ss, n, part, 3, nb);
+        hipLaunchKernelGGL(dot_kernel, dim3(nb), dim3(TB), 0, ctx->stream, Ss, rr, n, part, 4, nb);
+        hipLaunchKernelGGL(dot_kernel, dim3(nb), dim3(TB), 0, ctx->stream, rr, rr, n, part, 5, nb);
+        hipLaunchKernelGGL(dot_kernel, dim3(nb), dim3(TB), 0, ctx->stream, gv, gv, n, part, 6, nb);
+        SH_TRY(sh_launch_check(ctx, "soint2d prec dots"));
+        SH_TRY(sum_partials(ctx, part, nb, 7, dsum, h));
+        const double gdg = h[0], gds = h[1], gdr0 = h[2], sds = h[3], sdr0 = h[4], rnorm = h[5], gnorm = h[6];
+        // the reference sums these two norms in float; they only decide the
+        // early stop against 1e-12
+        double dprr, dpgm;
+        if (it == 0) {
+            dprr0 = (float)rnorm;
+            dpgm0 = (float)gnorm;
+            dprr = 1.0;
+            dpgm = 1.0;
+        } else {
+            dprr = (float)rnorm / dprr0;
+            dpgm = (float)gnorm / dpgm0;
+        }
+        if (dprr < TOL || dpgm < TOL) {
+            SH_TRY(prec_apply<NW>(ctx, 0, p, out, pp, qq, Fp, Fq, tmp, tt, ys, n1, n2, e));
+            break;
+        }
+        // ps_cgstep (soint2d_cfuns.c:493-545); its first call restarts
+        const bool forget = !started;
+        started = true;
+        bool step = true;
+        double alfa = 0.0, beta = 0.0;
+        if (forget) {
+            hipLaunchKernelGGL(zero_f, dim3(nb), dim3(TB), 0, ctx->stream, S, n);
+            hipLaunchKernelGGL(zero_f, dim3(nb), dim3(TB), 0, ctx->stream, Ss, n);
+            alfa = gdg;
+            if (alfa <= 0.) step = false;
+            else alfa = -gdr0 / alfa;
+        } else if (gdg == 0. || sds == 0.) {
+            step = false;
+        } else {
+            double determ = 1.0 - (gds / gdg) * (gds / sds);
+            if (determ > EPSILON) determ *= gdg * sds;
+            else determ = gdg * sds * EPSILON;
+            const double gdr = -gdr0, sdr = -sdr0;
+            alfa = (sds * gdr - gds * sdr) / determ;
+            beta = (-gds * gdr + gdg * sdr) / determ;
+        }
+        if (step) {
+            hipLaunchKernelGGL(cgstep_update, dim3(nb), dim3(TB), 0, ctx->stream, S, gv, p, n, Ss, gg, rr, n,
+                               (float)alfa, (float)beta);
+            SH_TRY(sh_launch_check(ctx, "cgstep_update"));
+        }
+        if (it == niter - 1) SH_TRY(prec_apply<NW>(ctx, 0, p, out, pp, qq, Fp, Fq, tmp, tt, ys, n1, n2, e));
+    }
+    if (iters_out) *iters_out = it;
+    return SH_OK;
+}
+
 __global__ void count_kernel(const unsigned char *__restrict__ m, long n, double *part) {
     __shared__ double sh[16];
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -516,6 +928,24 @@ int sh_soint2d(sh_ctx *ctx, const float *din, const float *mask, const float *di
     hipLaunchKernelGGL(zero_f, dim3(nb), dim3(TB), 0, ctx->stream, dd, n);
     int it = 0;
     SH_DISPATCH_NW(ctx, nw, NW, SH_TRY(soint_solve<NW>(ctx, g, dip, dip, known, out, dd, niter, &it)));
+    if (iters) *iters = it;
+    return SH_OK;
+}
+
+int sh_soint2d_prec(sh_ctx *ctx, const float *din, const float *mask, const float *dip, const float *dip2, int n1,
+                    int n2, int nw, int niter, int hasmask, float *out, int *iters) {
+    if (!ctx) return SH_ENULL;
+    if (!din || !dip || !out || (hasmask == 1 && !mask)) return sh_fail(ctx, SH_ENULL, "soint2d: NULL pointer");
+    if (n1 < 8 || n2 < 1 || niter < 0) return sh_fail(ctx, SH_EINVAL, "soint2d: bad arguments");
+    const long n = (long)n1 * n2;
+    SH_HIP(ctx, hipMemcpyAsync(out, din, sizeof(float) * n, hipMemcpyDeviceToDevice, ctx->stream));
+    unsigned char *known = nullptr;
+    SH_TRY(sh_slot(ctx, SLOT_INTERP0 + 11, (size_t)n + 256, (void **)&known));
+    hipLaunchKernelGGL(known_kernel, dim3(ceil_div(n, TB)), dim3(TB), 0, ctx->stream, hasmask == 1 ? mask : din, known,
+                       n);
+    SH_TRY(sh_launch_check(ctx, "known_kernel"));
+    int it = 0;
+    SH_DISPATCH_NW(ctx, nw, NW, SH_TRY(soint2d_prec_solve<NW>(ctx, din, known, dip, dip2, n1, n2, niter, out, &it)));
     if (iters) *iters = it;
     return SH_OK;
 }

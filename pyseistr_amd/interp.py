@@ -5,8 +5,10 @@ sint3dc / sint2dc (pyseistr/sint.py:97, 61): same signatures, defaults and
 output shapes; the arithmetic is csoint3d / csoint2d / csint3d / csint2d of
 the C extension (soint3d_cfuns.c, soint2d_cfuns.c), run by the gfx950
 library.  Extensions: ``report=True`` also returns the solver iteration count.
-Not implemented on the GPU (raise NotImplementedError): drift=1, and
-soint2dc(prec=1) (the reference's preconditioned predictive solver).
+soint2dc(prec=1), the reference's preconditioned predictive solver, runs its
+trace-to-trace prediction chain in one GPU thread (the chain is sequential)
+with the banded factors built in parallel.  Not implemented on the GPU
+(raise NotImplementedError): drift=1.
 """
 import ctypes
 
@@ -41,7 +43,7 @@ def soint2dc(din, mask, dip, order=1, niter=100, njs=[1, 1], drift=0, hasmask=1,
     din = np.asarray(din)
     n1, n2 = din.shape
     if prec:
-        raise NotImplementedError("soint2dc(prec=1) is not implemented on the GPU")
+        return _soint2d_prec(din, mask, dip, order, niter, hasmask, twoplane, report)
     if twoplane:
         dip = np.asarray(dip)[:, :, 0]
     ctx = _lib.context()
@@ -50,6 +52,24 @@ def soint2dc(din, mask, dip, order=1, niter=100, njs=[1, 1], drift=0, hasmask=1,
     it = _iters()
     rc = _lib.load().sh_soint2d(ctx, _lib.ptr(d), _lib.ptr(m), _lib.ptr(p), n1, n2, int(order), int(njs[0]),
                                 int(niter), int(drift), int(hasmask), int(twoplane), _lib.ptr(out), ctypes.byref(it))
+    _lib.check(ctx, rc, "soint2dc")
+    res = _lib.to_host(out).reshape(n1, n2, order="F")
+    return (res, it.value) if report else res
+
+
+def _soint2d_prec(din, mask, dip, order, niter, hasmask, twoplane, report):
+    """soint2dc(prec=1): the preconditioned solve (sh_soint2d_prec); with
+    twoplane, dip[:, :, 0] and dip[:, :, 1] are the two slope planes."""
+    n1, n2 = din.shape
+    dip = np.asarray(dip)
+    ctx = _lib.context()
+    d, m = _lib.to_device(din), _lib.to_device(mask)
+    p = _lib.to_device(dip[:, :, 0] if twoplane else dip)
+    q = _lib.to_device(dip[:, :, 1]) if twoplane else None
+    out = _lib.empty(n1 * n2)
+    it = _iters()
+    rc = _lib.load().sh_soint2d_prec(ctx, _lib.ptr(d), _lib.ptr(m), _lib.ptr(p), _lib.ptr(q), n1, n2, int(order),
+                                     int(niter), int(hasmask), _lib.ptr(out), ctypes.byref(it))
     _lib.check(ctx, rc, "soint2dc")
     res = _lib.to_host(out).reshape(n1, n2, order="F")
     return (res, it.value) if report else res

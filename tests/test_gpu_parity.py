@@ -361,3 +361,47 @@ def test_dip3d_long_run_matches_oracle(ps):
     d = np.abs(g - o32)
     assert d.max() <= 2e-3 and np.sqrt(np.mean(d ** 2)) <= 2e-4
     assert np.abs(g - o64).max() <= np.abs(o32 - o64).max() + 1e-4
+
+
+@pytest.mark.parametrize("n1,n2,order,niter,hasmask,twoplane", [(40, 24, 1, 10, 1, 0), (60, 30, 2, 25, 1, 0),
+                                                                  (33, 17, 2, 8, 0, 0), (40, 24, 2, 12, 1, 1),
+                                                                  (50, 20, 1, 3, 1, 1), (64, 9, 3, 6, 1, 0)])
+def test_soint2d_prec_bit_exact(ps, n1, n2, order, niter, hasmask, twoplane):
+    """soint2dc(prec=1): the preconditioned solve (predict_lop / predict2_lop
+    preconditioner, ps_solver_prec) against the FP32 oracle, which equals the
+    reference's csoint2d bit for bit; iteration counts agree too."""
+    from oracle import oracle as O
+    rng = np.random.default_rng(n1 * 100 + n2 + niter)
+    d = rng.standard_normal((n1, n2)).astype(np.float32)
+    mask = (rng.random((n1, n2)) > 0.5).astype(np.float32)
+    if hasmask == 0:
+        d = d * mask
+    dip = (0.7 * rng.standard_normal((n1, n2, 2))).astype(np.float32)
+    dd = dip if twoplane else dip[:, :, 0]
+    got, it = ps.soint2dc(d, mask, dd, order=order, niter=niter, hasmask=hasmask, twoplane=twoplane, prec=1,
+                          report=True)
+    exp = O.soint2d(d.ravel(order="F"), mask.ravel(order="F"), dip[:, :, 0].ravel(order="F"), (n1, n2), order=order,
+                    niter=niter, hasmask=hasmask, twoplane=twoplane, prec=1, dip2=dip[:, :, 1].ravel(order="F"))
+    same(got, exp)
+    assert it == niter
+    # niter = 0 returns the data
+    same(ps.soint2dc(d, mask, dd, niter=0, hasmask=hasmask, twoplane=twoplane, prec=1), d)
+
+
+def test_soint2d_prec_chain_kernels_identical(ps, monkeypatch):
+    """The three chain kernels give the same bits: the default one (per-sample
+    work spread over the block) and the ones taken when its LDS staging would
+    exceed 64 KB (SH_SOINT_PREC_CHAIN=lds: one thread from LDS;
+    SH_SOINT_PREC_CHAIN=global: one thread from HBM)."""
+    rng = np.random.default_rng(5)
+    d = rng.standard_normal((48, 20)).astype(np.float32)
+    mask = (rng.random((48, 20)) > 0.4).astype(np.float32)
+    dip = (0.6 * rng.standard_normal((48, 20, 2))).astype(np.float32)
+    for twoplane in (0, 1):
+        dd = dip if twoplane else dip[:, :, 0]
+        a = ps.soint2dc(d, mask, dd, order=2, niter=7, twoplane=twoplane, prec=1)
+        for mode in ("lds", "global"):
+            monkeypatch.setenv("SH_SOINT_PREC_CHAIN", mode)
+            b = ps.soint2dc(d, mask, dd, order=2, niter=7, twoplane=twoplane, prec=1)
+            monkeypatch.delenv("SH_SOINT_PREC_CHAIN")
+            same(a, b)
