@@ -28,6 +28,8 @@ def fkdip(d, w):
     d = np.asarray(d)
     if d.ndim != 2:
         raise ValueError("fkdip: d must be 2-D")
+    if np.iscomplexobj(d):
+        raise TypeError("fkdip: complex data are not supported (the device path takes real float64)")
     n1, n2 = d.shape
     torch = _lib._torch()
     dev = torch.from_numpy(np.ascontiguousarray(d.astype(np.float64, copy=False).ravel(order="F"))).cuda()

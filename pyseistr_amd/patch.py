@@ -74,6 +74,8 @@ def _cut(A, l, s, mode):
     if mode != 1:
         raise NotImplementedError("patch: only mode 1 exists")
     A = np.asarray(A)
+    if np.iscomplexobj(A):
+        raise TypeError("patch: complex data are not supported (the device path is real float64)")
     shape = A.shape + (1,) * (3 - A.ndim)
     # the reference pads with float64 zeros, which promotes the whole array
     dtype = np.result_type(A.dtype, np.float64) if _padded(A.shape, l, s) else A.dtype
@@ -84,8 +86,10 @@ def _cut(A, l, s, mode):
 def _paste(X, n, l, s, mode):
     if mode != 1:
         raise NotImplementedError("patch_inv: only mode 1 exists")
-    torch = _lib._torch()
     X = np.asarray(X)
+    if np.iscomplexobj(X):
+        raise TypeError("patch_inv: complex patches are not supported (the device path is real float64)")
+    torch = _lib._torch()
     shape = tuple(int(v) for v in n) + (1,) * (3 - len(n))
     l3, s3 = l + (1,) * (3 - len(l)), s + (1,) * (3 - len(s))
     Xd = torch.from_numpy(np.ascontiguousarray(X.reshape(X.shape[0], -1), np.float64)).cuda()

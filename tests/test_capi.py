@@ -62,3 +62,14 @@ def test_neighbours_have_no_cpu_fallback():
                  lambda: ps.smoothc(np.zeros((16, 8)), rect=[3, 1, 1]), lambda: ps.bandpassc(np.zeros(64), 0.004)):
         with pytest.raises(ps.SeistrError):
             call()
+
+
+def test_complex_input_rejected_before_any_device_call():
+    import numpy as np
+    import pytest
+    import pyseistr_amd as ps
+    z = np.zeros((16, 16), np.complex64)
+    for call in (lambda: ps.patch2d(z), lambda: ps.patch2d_inv(np.zeros((9, 64), complex), 16, 16),
+                 lambda: ps.fkdip(z, 0.1)):
+        with pytest.raises(TypeError):
+            call()
