@@ -555,8 +555,8 @@ prec_chain_lds(int adj, float *__restrict__ x, float *__restrict__ y, const floa
 // plane 1 + m = o[m]; n1 per plane): forward b[k] = src[k] - ...,
 // y[k] = b[k] / d[k], then backward into out (which may be y).  The loops of
 // solve_factored / predict_adj_factored, operation for operation.  (Loads
-// issued ahead into register rings measured slower here: the operands are
-// in LDS.)
+// issued ahead into register rings, or the loops unrolled by four, measured
+// slower here: the operands are in LDS.)
 template <int NW>
 __device__ void subst_factored(const float *src, const float *Fs, float *y, int n1, float *out) {
     constexpr int NB = 2 * NW;
