@@ -161,7 +161,9 @@ int sh_somf2d(sh_ctx *ctx, const float *dn, const float *dip, int n1, int n2, in
 
 /* PWD-regularised interpolation of unknown samples.  Replaces
  * soint3dcfun.csoint3d (pyseistr/src/soint3d_cfuns.c:2405-2508) behind
- * soint3dc (pyseistr/soint3d.py:65).  iters (nullable, host): solver
+ * soint3dc (pyseistr/soint3d.py:65).  drift != 0 shifts each stencil by
+ * NINT(p) along i1 and takes the filter at p - NINT(p)
+ * (allpass3_lop, soint3d_cfuns.c:646-664).  iters (nullable, host): solver
  * iterations run; filling it synchronises. */
 int sh_soint3d(sh_ctx *ctx, const float *din, const float *mask, const float *dipi,
                const float *dipx, int n1, int n2, int n3, int nw, int nj1, int nj2,

@@ -16,6 +16,7 @@
 #include "common.hpp"
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -30,15 +31,30 @@ constexpr int TB = 256;
 struct Geo {
     int n1, n2, n3, nw, nj1, nj2, nparts;
     long n;
+    int drift = 0;   // shift each stencil by NINT(p) along i1 (allpass "drift")
+    int maxid = 0;   // max |NINT(p)| over both slope fields (adjoint search window)
+    const int *ids = nullptr;  // NINT(p) per part and sample when drift
 };
 
+// PS_NINT (soint3d_cfuns.c:11): round half away from zero, in double
+__device__ inline int ps_nint(float p) { return (int)(p > 0.0 ? p + 0.5 : p - 0.5); }
+
 // taps[part][iw][i] = a_iw(p_part[i]) (apfilt, soint3d_cfuns.c:159-250)
+// with drift, the taps are those of p - NINT(p) and ids[i] = NINT(p)
+// (allpass3_lop, soint3d_cfuns.c:646-652)
 template <int NW>
-__global__ void taps_kernel(const float *__restrict__ pp, float *__restrict__ taps, long n) {
+__global__ void taps_kernel(const float *__restrict__ pp, float *__restrict__ taps, long n, int drift,
+                            int *__restrict__ ids) {
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     float a[2 * NW + 1];
-    apf_pass<NW>(pp[i], a);
+    float p = pp[i];
+    if (drift) {
+        const int id = ps_nint(p);
+        ids[i] = id;
+        p = p - id;
+    }
+    apf_pass<NW>(p, a);
 #pragma unroll
     for (int iw = 0; iw <= 2 * NW; iw++) taps[(long)iw * n + i] = a[iw];
 }
@@ -63,13 +79,18 @@ pwd_fwd(const float *__restrict__ x, const float *__restrict__ taps, float *__re
         int nj = part_id ? g.nj2 : g.nj1;
         long ip = part_id ? (long)g.n1 * g.n2 : g.n1;
         bool valid = ix >= NW * nj && ix < g.n1 - NW * nj && (part_id ? iz < g.n3 - 1 : iy < g.n2 - 1);
+        int id = 0;
+        if (valid && g.drift) {
+            id = g.ids[(long)part_id * g.n + k];
+            valid = ix - NW * nj - id >= 0 && ix + NW * nj - id < g.n1;
+        }
         float v = base ? base[i] : 0.0f;
         if (valid) {
             const float *tp = taps + (long)part_id * (2 * NW + 1) * g.n;
 #pragma unroll
             for (int iw = 0; iw <= 2 * NW; iw++) {
                 int is = (iw - NW) * nj;
-                v += (x[k + ip + is] - x[k - is]) * tp[(long)iw * g.n + k];
+                v += (x[k + ip + is] - x[k - is - id]) * tp[(long)iw * g.n + k];
             }
         }
         y[i] = v;
@@ -107,6 +128,14 @@ pwd_adj(const float *__restrict__ rr, const float *__restrict__ taps, const unsi
             long ip = part_id ? (long)g.n1 * g.n2 : g.n1;
             const float *tp = taps + (long)part_id * (2 * NW + 1) * g.n;
             const float *yy = rr + (long)part_id * g.n;
+            const int *idp = g.drift ? g.ids + (long)part_id * g.n : nullptr;
+            // with drift a source sample i contributes only when its shifted
+            // stencil stays inside the trace
+            auto live = [&](long i, int ix) {
+                if (!idp) return true;
+                const int id = idp[i];
+                return ix - h - id >= 0 && ix + h - id < g.n1;
+            };
             // "+" terms: i = k - ip - is, source row valid if it has a successor
             bool prow = part_id ? (kz >= 1) : (ky >= 1);
             if (prow) {
@@ -116,12 +145,12 @@ pwd_adj(const float *__restrict__ rr, const float *__restrict__ taps, const unsi
                     int ix = kx - is;
                     if (ix >= h && ix < g.n1 - h) {
                         long i = k - ip - is;
-                        v += yy[i] * tp[(long)iw * g.n + i];
+                        if (live(i, ix)) v += yy[i] * tp[(long)iw * g.n + i];
                     }
                 }
             }
             bool mrow = part_id ? (kz < g.n3 - 1) : (ky < g.n2 - 1);
-            if (mrow) {
+            if (mrow && !idp) {
 #pragma unroll
                 for (int iw = 0; iw <= 2 * NW; iw++) {
                     int is = (iw - NW) * nj;
@@ -129,6 +158,22 @@ pwd_adj(const float *__restrict__ rr, const float *__restrict__ taps, const unsi
                     if (ix >= h && ix < g.n1 - h) {
                         long i = k + is;
                         v -= yy[i] * tp[(long)iw * g.n + i];
+                    }
+                }
+            } else if (mrow) {
+                // "-" terms with drift: sources ix of this row with
+                // ix - is - NINT(p(ix)) = kx, in the scatter's order (ix,
+                // then iw, ascending); |is + NINT| <= h + maxid
+                const int W = h + min(g.maxid, g.n1);
+                const int lo = max(h, kx - W), hi = min(g.n1 - h - 1, kx + W);
+                for (int ix = lo; ix <= hi; ix++) {
+                    const long i = k - kx + ix;
+                    if (!live(i, ix)) continue;
+                    const int id = idp[i];
+#pragma unroll
+                    for (int iw = 0; iw <= 2 * NW; iw++) {
+                        int is = (iw - NW) * nj;
+                        if (ix - is - id == kx) v -= yy[i] * tp[(long)iw * g.n + i];
                     }
                 }
             }
@@ -193,10 +238,31 @@ int sum_partials(sh_ctx *ctx, const double *part, int nblk, int K, double *dsum,
     return SH_OK;
 }
 
+// out = max |a[i]| over n ints (block maximum, one atomic per block; out
+// must start at 0)
+__global__ void __launch_bounds__(TB) maxabs_kernel(const int *__restrict__ a, long n, int *out) {
+    __shared__ int sh[TB];
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int v = i < n ? a[i] : 0;
+    sh[threadIdx.x] = v >= 0 ? v : (v == INT_MIN ? INT_MAX : -v);
+    __syncthreads();
+    for (int w = TB / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] = max(sh[threadIdx.x], sh[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicMax(out, sh[0]);
+}
+
+__global__ void fill_i(int *__restrict__ a, int v, long n) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
 // ps_solver + ps_cgstep with the "known" and "x0" options
 template <int NW>
-int soint_solve(sh_ctx *ctx, const Geo &g, const float *pp, const float *qq, const unsigned char *known, float *x,
+int soint_solve(sh_ctx *ctx, const Geo &g0, const float *pp, const float *qq, const unsigned char *known, float *x,
                 const float *dd, int niter, int *iters_out) {
+    Geo g = g0;
     long n = g.n, ny = g.n * g.nparts;
     int NA = 2 * NW + 1;
     float *buf = nullptr;
@@ -208,10 +274,28 @@ int soint_solve(sh_ctx *ctx, const Geo &g, const float *pp, const float *qq, con
     double *part = nullptr, *dsum = nullptr;
     SH_TRY(sh_slot(ctx, SLOT_RED, (size_t)nby * 8 * sizeof(double), (void **)&part));
     SH_TRY(sh_slot(ctx, SLOT_SCAL, 16 * sizeof(double), (void **)&dsum));
-    hipLaunchKernelGGL((taps_kernel<NW>), dim3(nbn), dim3(TB), 0, ctx->stream, pp, taps, n);
+    int *ids = nullptr;
+    if (g.drift) {
+        // NINT(p) per part and sample, then the largest shift (one sync)
+        SH_TRY(sh_slot(ctx, SLOT_INTERP0 + 9, ((size_t)g.nparts * n + 64) * sizeof(int), (void **)&ids));
+    }
+    int *dmax = ids ? ids + (size_t)g.nparts * n : nullptr;
+    hipLaunchKernelGGL((taps_kernel<NW>), dim3(nbn), dim3(TB), 0, ctx->stream, pp, taps, n, g.drift, ids);
     if (g.nparts == 2)
-        hipLaunchKernelGGL((taps_kernel<NW>), dim3(nbn), dim3(TB), 0, ctx->stream, qq, taps + (size_t)NA * n, n);
+        hipLaunchKernelGGL((taps_kernel<NW>), dim3(nbn), dim3(TB), 0, ctx->stream, qq, taps + (size_t)NA * n, n,
+                           g.drift, ids ? ids + n : nullptr);
     SH_TRY(sh_launch_check(ctx, "taps"));
+    if (g.drift) {
+        const long ni = (long)g.nparts * n;
+        hipLaunchKernelGGL(fill_i, dim3(1), dim3(TB), 0, ctx->stream, dmax, 0, 1L);
+        hipLaunchKernelGGL(maxabs_kernel, dim3(ceil_div(ni, TB)), dim3(TB), 0, ctx->stream, ids, ni, dmax);
+        SH_TRY(sh_launch_check(ctx, "maxabs"));
+        int h = 0;
+        SH_HIP(ctx, hipMemcpyAsync(&h, dmax, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        SH_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        g.ids = ids;
+        g.maxid = h;
+    }
     hipLaunchKernelGGL(zero_f, dim3(nbn), dim3(TB), 0, ctx->stream, S, n);
     hipLaunchKernelGGL(zero_f, dim3(nby), dim3(TB), 0, ctx->stream, Ss, ny);
     // rr = -dd, then rr += A x0 tap by tap (ps_solver 1018-1026)
@@ -880,9 +964,9 @@ int sh_soint3d(sh_ctx *ctx, const float *din, const float *mask, const float *di
     if (!din || !dipi || !dipx || !out || (hasmask == 1 && !mask))
         return sh_fail(ctx, SH_ENULL, "soint3d: NULL pointer");
     if (n1 < 8 || n2 < 2 || n3 < 2 || nj1 < 1 || nj2 < 1) return sh_fail(ctx, SH_EINVAL, "soint3d: bad arguments");
-    if (drift) return sh_fail(ctx, SH_EUNSUP, "soint3d: drift=1 is not implemented on the GPU");
     long n = (long)n1 * n2 * n3;
     Geo g{n1, n2, n3, nw, nj1, nj2, 2, n};
+    g.drift = drift != 0;
     unsigned char *known = nullptr;
     float *dd = nullptr;
     SH_TRY(sh_slot(ctx, SLOT_INTERP0 + 1, (size_t)n + (size_t)2 * n * sizeof(float) + 256, (void **)&dd));
@@ -918,8 +1002,8 @@ int sh_soint2d(sh_ctx *ctx, const float *din, const float *mask, const float *di
         if (iters) *iters = 0;
         return SH_OK;
     }
-    if (drift) return sh_fail(ctx, SH_EUNSUP, "soint2d: drift=1 is not implemented on the GPU");
     Geo g{n1, n2, 1, nw, nj1, nj1, 1, n};
+    g.drift = drift != 0;
     float *dd = nullptr;
     SH_TRY(sh_slot(ctx, SLOT_INTERP0 + 1, (size_t)n + (size_t)n * sizeof(float) + 256, (void **)&dd));
     unsigned char *known = (unsigned char *)(dd + n);

@@ -7,8 +7,8 @@ the C extension (soint3d_cfuns.c, soint2d_cfuns.c), run by the gfx950
 library.  Extensions: ``report=True`` also returns the solver iteration count.
 soint2dc(prec=1), the reference's preconditioned predictive solver, runs its
 trace-to-trace prediction chain in one GPU thread (the chain is sequential)
-with the banded factors built in parallel.  Not implemented on the GPU
-(raise NotImplementedError): drift=1.
+with the banded factors built in parallel.  drift=1 shifts each PWD stencil
+by NINT(p) as the reference does.
 """
 import ctypes
 

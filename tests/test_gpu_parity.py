@@ -405,3 +405,32 @@ def test_soint2d_prec_chain_kernels_identical(ps, monkeypatch):
             b = ps.soint2dc(d, mask, dd, order=2, niter=7, twoplane=twoplane, prec=1)
             monkeypatch.delenv("SH_SOINT_PREC_CHAIN")
             same(a, b)
+
+
+@pytest.mark.parametrize("shape,nj,order,niter", [((40, 12, 9), (1, 1), 2, 8), ((33, 10, 7), (2, 1), 1, 6),
+                                                   ((50, 9, 8), (1, 2), 2, 5)])
+def test_soint3d_drift_bit_exact(ps, shape, nj, order, niter):
+    """soint3dc(drift=1): stencils shifted by NINT(p) (slopes up to ~8
+    samples/trace, so shifts and skipped samples occur) against the FP32
+    oracle, which equals the reference bit for bit."""
+    from oracle import oracle as O
+    rng = np.random.default_rng(sum(shape) + niter)
+    d = rng.standard_normal(shape).astype(np.float32)
+    mask = (rng.random(shape) > 0.4).astype(np.float32)
+    pi = (2.5 * rng.standard_normal(shape)).astype(np.float32)
+    px = (2.0 * rng.standard_normal(shape)).astype(np.float32)
+    got = ps.soint3dc(d, mask, pi, px, order=order, niter=niter, njs=list(nj), drift=1)
+    f = lambda a: a.ravel(order="F")  # noqa: E731
+    same(got, O.soint3d(f(d), f(mask), f(pi), f(px), shape, order=order, niter=niter, njs=nj, drift=1))
+
+
+@pytest.mark.parametrize("n1,n2,nj,order,niter", [(40, 20, 1, 2, 10), (37, 15, 2, 1, 6), (64, 30, 1, 3, 7)])
+def test_soint2d_drift_bit_exact(ps, n1, n2, nj, order, niter):
+    from oracle import oracle as O
+    rng = np.random.default_rng(n1 + n2 + niter)
+    d = rng.standard_normal((n1, n2)).astype(np.float32)
+    mask = (rng.random((n1, n2)) > 0.4).astype(np.float32)
+    p = (2.5 * rng.standard_normal((n1, n2))).astype(np.float32)
+    got = ps.soint2dc(d, mask, p, order=order, niter=niter, njs=[nj, 1], drift=1)
+    f = lambda a: a.ravel(order="F")  # noqa: E731
+    same(got, O.soint2d(f(d), f(mask), f(p), (n1, n2), order=order, niter=niter, njs=(nj, 1), drift=1))

@@ -490,3 +490,32 @@ def test_soint2d_prec_residual_decreases(twoplane):
         full = np.ones(shape, np.float32)
         x = O.soint2d(f(d), f(full), f(p), shape, order=2, niter=60, hasmask=1, prec=1, dtype=np.float64)
         assert np.linalg.norm(x - f(d)) < 0.05 * np.linalg.norm(f(d))
+
+
+def test_soint_drift_matches_reference():
+    """soint3d / soint2d with drift=1 (each PWD stencil shifted by NINT(p),
+    the filter taken at p - NINT(p)): the FP32 restatement against the
+    reference's own csoint3d / csoint2d, bit for bit, with slopes large
+    enough that shifts and skipped samples occur."""
+    from oracle import ref as R
+    if not R.available():
+        pytest.skip("reference modules not built")
+    rng = np.random.default_rng(0)
+    f = lambda a: a.ravel(order="F")  # noqa: E731
+    for shape, nj, order, niter in [((40, 12, 9), (1, 1), 2, 8), ((33, 10, 7), (2, 1), 1, 6)]:
+        d = rng.standard_normal(shape).astype(np.float32)
+        mask = (rng.random(shape) > 0.4).astype(np.float32)
+        pi = (2.5 * rng.standard_normal(shape)).astype(np.float32)
+        px = (2.0 * rng.standard_normal(shape)).astype(np.float32)
+        r = np.asarray(R.soint3d(f(d), f(mask), f(pi), f(px), shape, order=order, niter=niter, njs=nj, drift=1),
+                       np.float32).ravel()
+        o = O.soint3d(f(d), f(mask), f(pi), f(px), shape, order=order, niter=niter, njs=nj, drift=1)
+        assert np.array_equal(r.view(np.uint32), o.view(np.uint32)), shape
+    for (n1, n2), nj, order, niter in [((40, 20), 1, 2, 10), ((37, 15), 2, 1, 6)]:
+        d = rng.standard_normal((n1, n2)).astype(np.float32)
+        mask = (rng.random((n1, n2)) > 0.4).astype(np.float32)
+        p = (2.5 * rng.standard_normal((n1, n2))).astype(np.float32)
+        r = np.asarray(R.soint2d(f(d), f(mask), f(p), (n1, n2), order=order, niter=niter, njs=(nj, 1), drift=1),
+                       np.float32).ravel()
+        o = O.soint2d(f(d), f(mask), f(p), (n1, n2), order=order, niter=niter, njs=(nj, 1), drift=1)
+        assert np.array_equal(r.view(np.uint32), o.view(np.uint32)), (n1, n2)
