@@ -651,10 +651,12 @@ int spray3_run(sh_ctx *ctx, int order, int n1, int n2, int n3, int ns2, int ns3,
             if (nf == 1) {
                 if (pf == 8) { SL(1, 8); PROF_B(ctx, "spray3_level<1,8>", by); }
                 else if (pf == 4) { SL(1, 4); PROF_B(ctx, "spray3_level<1,4>", by); }
+                else if (pf == 2) { SL(1, 2); PROF_B(ctx, "spray3_level<1,2>", by); }
                 else { SL(1, 0); PROF_B(ctx, "spray3_level<1,0>", by); }
             } else {
                 if (pf == 8) { SL(2, 8); PROF_B(ctx, "spray3_level<2,8>", by); }
                 else if (pf == 4) { SL(2, 4); PROF_B(ctx, "spray3_level<2,4>", by); }
+                else if (pf == 2) { SL(2, 2); PROF_B(ctx, "spray3_level<2,2>", by); }
                 else { SL(2, 0); PROF_B(ctx, "spray3_level<2,0>", by); }
             }
 #undef SL
