@@ -200,7 +200,7 @@ def test_bad_arguments_raise(ps):
     with pytest.raises(NotImplementedError):
         ps.somf3dc(w, w, w, 2, 2, 0.01, 2, option=2)
     with pytest.raises(NotImplementedError):
-        ps.soint3dc(w, w, w, w, drift=1)
+        ps.soint3dc(w, w, w, w, order=5)  # PWD orders 1..4
 
 
 def test_fused_smoothing_matches_separable(ps, monkeypatch):
