@@ -133,9 +133,11 @@ int sh_somean3d(sh_ctx *ctx, const float *din, const float *dipi, const float *d
                 int n1, int n2, int n3, int r1, int r2, float eps, int order, float *out);
 
 /* Structure-oriented median: spray, then the median of the nmf linear slot
- * indices centred at (np-1)/2 (option 1) or the space-varying median
- * (option != 1).  Replaces sof3dcfun.csomf3d (pyseistr/src/
- * sof3d_cfuns.c:1554-1753); eps forced to 0.01 as in the reference (1586). */
+ * indices centred at (np-1)/2 (option 1) or the space-varying median over
+ * all slots (option 2, svmf, sof3d_cfuns.c:1254-1352; needs nmf >= 5; where
+ * the reference reads one row past its extended panel the edge slot is
+ * used).  Replaces sof3dcfun.csomf3d (pyseistr/src/sof3d_cfuns.c:1554-1753);
+ * eps forced to 0.01 as in the reference (1586). */
 int sh_somf3d(sh_ctx *ctx, const float *din, const float *dipi, const float *dipx,
               int n1, int n2, int n3, int r1, int r2, int nmf, int option, int order,
               float eps, float *out);
@@ -153,9 +155,10 @@ int sh_pwspray2d(sh_ctx *ctx, int adj, const float *dip, int n1, int n2, int ns,
 int sh_somean2d(sh_ctx *ctx, const float *dn, const float *dip, int n1, int n2, int n3,
                 int ns, int order, int adj, float eps, float *out);
 
-/* Structure-oriented median (option 1) or space-varying median per n3
- * slice.  Replaces sofcfun.csomf2d (pyseistr/src/sof_cfuns.c:1534-1672)
- * behind somf2dc (pyseistr/somf2d.py:60). */
+/* Structure-oriented median (option 1) or space-varying median (option 2,
+ * svmf, sof_cfuns.c:1336-1431; needs nmf >= 5) per n3 slice.  Replaces
+ * sofcfun.csomf2d (pyseistr/src/sof_cfuns.c:1534-1672) behind somf2dc
+ * (pyseistr/somf2d.py:60). */
 int sh_somf2d(sh_ctx *ctx, const float *dn, const float *dip, int n1, int n2, int n3,
               int ns, int nmf, int option, int order, float eps, float *out);
 

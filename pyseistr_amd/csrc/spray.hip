@@ -815,6 +815,79 @@ int sh_somean3d(sh_ctx *ctx, const float *din, const float *dipi, const float *d
     return transpose(ctx, dT, out, n1, ntr, false);
 }
 
+}  // extern "C"
+
+namespace {
+
+// q-th smallest of w[0..len) by counting ranks (the value is that of the
+// reference's selection; a zero may come back with the other sign)
+__device__ inline float select_rank(const float *w, int len, int q) {
+    for (int a = 0; a < len; a++) {
+        const float v = w[a];
+        int less = 0, leq = 0;
+        for (int b = 0; b < len; b++) {
+            less += w[b] < v;
+            leq += w[b] <= v;
+        }
+        if (less <= q && q < leq) return v;
+    }
+    return w[0];
+}
+
+// Space-varying median of one trace's slot panel, centre slot only (svmf,
+// sof_cfuns.c:1336-1431): a first pass of medians over nfw + 2 slots around
+// every slot and sample (columns clamped into the panel; where the
+// reference reads one row past its extended panel the edge column is used),
+// their mean absolute value in float summed slot by slot and sample by
+// sample as there, then per centre sample a window of nfw + 2, nfw, nfw - 2
+// or nfw - 4 slots chosen by |x| against that mean.  One thread per trace;
+// element (slot s, sample k) at U[s * n + base + k * stride].
+__global__ void svmf_traces(const float *__restrict__ U, float *__restrict__ out, long n, int np, int n1, int nfw,
+                            long ntraces, int n2, int two_d) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntraces) return;
+    long base, stride;
+    if (two_d) {
+        const long i3 = t / n2, i2 = t % n2;
+        base = i3 * (long)n1 * n2 + i2;
+        stride = n2;
+    } else {
+        base = t;
+        stride = ntraces;
+    }
+    float w[128];
+    const int nf = nfw + 2, m = (nf - 1) / 2, m2 = (nfw - 1) / 2, c = (np - 1) / 2;
+    float sum = 0.0f;
+    for (int i = 0; i < np; i++)
+        for (int j = 0; j < n1; j++) {
+            for (int k = 0; k < nf; k++) {
+                int col = i + k - m2;
+                col = col < 0 ? 0 : (col > np - 1 ? np - 1 : col);
+                w[k] = U[(long)col * n + base + (long)j * stride];
+            }
+            sum = sum + fabsf(select_rank(w, nf, m));
+        }
+    const float avg = sum / (float)(n1 * np);
+    for (int j = 0; j < n1; j++) {
+        const float v = U[(long)c * n + base + (long)j * stride];
+        const float av = fabsf(v);
+        int len;
+        if (av < avg) len = (av < avg / 2) ? nfw + 2 : nfw;
+        else len = (av > avg * 2) ? nfw - 4 : nfw - 2;
+        const int h = (len - 1) / 2;
+        for (int k = 0; k < len; k++) {
+            int col = c + k - h;
+            col = col < 0 ? 0 : (col > np - 1 ? np - 1 : col);
+            w[k] = U[(long)col * n + base + (long)j * stride];
+        }
+        out[base + (long)j * stride] = select_rank(w, len, h);
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
 int sh_somf3d(sh_ctx *ctx, const float *din, const float *dipi, const float *dipx, int n1, int n2, int n3, int r1,
               int r2, int nmf, int option, int order, float eps, float *out) {
     (void)eps;
@@ -824,12 +897,21 @@ int sh_somf3d(sh_ctx *ctx, const float *din, const float *dipi, const float *dip
     int np = (2 * r1 + 1) * (2 * r2 + 1);
     if (nmf < 1 || nmf % 2 == 0 || nmf > np)
         return sh_fail(ctx, SH_EINVAL, "somf3d: median length %d must be odd and <= %d", nmf, np);
-    if (option != 1) return sh_fail(ctx, SH_EUNSUP, "somf3d: only option=1 (MF) is implemented on the GPU");
+    if (option != 1 && option != 2) return sh_fail(ctx, SH_EINVAL, "somf3d: option must be 1 or 2");
+    if (option == 2 && (nmf < 5 || nmf + 2 > 128))
+        return sh_fail(ctx, SH_EINVAL, "somf3d: the space-varying median needs 5 <= nmf <= 125");
     long ntr = (long)n2 * n3, n = (long)n1 * ntr;
     float *dT, *piT, *pxT, *U;
     SH_TRY(stage3(ctx, din, dipi, dipx, n1, ntr, &dT, &piT, &pxT));
     SH_TRY(sh_slot(ctx, SLOT_SPRAY0 + 2, (size_t)n * np * sizeof(float), (void **)&U));
     const float e = 0.01f * 0.01f;  // eps forced to 0.01 (sof3d_cfuns.c:1586)
+    if (option == 2) {  // its first pass reads every slot
+        SH_TRY(spray3_run(ctx, order, n1, n2, n3, r1, r2, e, dT, piT, pxT, U, nullptr, nullptr));
+        hipLaunchKernelGGL(svmf_traces, dim3(ceil_div(ntr, 64)), dim3(64), 0, ctx->stream, U, dT, n, np, n1, nmf, ntr,
+                           n2, 0);
+        SH_TRY(sh_launch_check(ctx, "svmf_traces"));
+        return transpose(ctx, dT, out, n1, ntr, false);
+    }
     int c = (np - 1) / 2, m = (nmf - 1) / 2;
     // only the slots the median window reads and their predecessors (37 of
     // 81 for r1 = r2 = 4, nmf = 33); the result is the same
@@ -918,7 +1000,9 @@ int sh_somf2d(sh_ctx *ctx, const float *dn, const float *dip, int n1, int n2, in
     int np = 2 * ns + 1;
     if (nmf < 1 || nmf % 2 == 0 || nmf > np)
         return sh_fail(ctx, SH_EINVAL, "somf2d: median length %d must be odd and <= %d", nmf, np);
-    if (option != 1) return sh_fail(ctx, SH_EUNSUP, "somf2d: only option=1 (MF) is implemented on the GPU");
+    if (option != 1 && option != 2) return sh_fail(ctx, SH_EINVAL, "somf2d: option must be 1 or 2");
+    if (option == 2 && (nmf < 5 || nmf + 2 > 128))
+        return sh_fail(ctx, SH_EINVAL, "somf2d: the space-varying median needs 5 <= nmf <= 125");
     long n = (long)n1 * n2 * n3;
     float *b = nullptr;
     SH_TRY(sh_slot(ctx, SLOT_SPRAY0 + 1, (size_t)n * 3 * sizeof(float), (void **)&b));
@@ -933,7 +1017,14 @@ int sh_somf2d(sh_ctx *ctx, const float *dn, const float *dip, int n1, int n2, in
     hipLaunchKernelGGL(zero_kernel, dim3(ceil_div(n * np, TB)), dim3(TB), 0, ctx->stream, P.U, n * np);
     SH_TRY(pws2_spray(ctx, P, xT));
     int m = (nmf - 1) / 2;
-    SH_TRY(launch_median(ctx, P.U, yT, n, ns - m, nmf));
+    if (option == 2) {
+        const long ntraces = (long)n2 * n3;
+        hipLaunchKernelGGL(svmf_traces, dim3(ceil_div(ntraces, 64)), dim3(64), 0, ctx->stream, P.U, yT, n, np, n1, nmf,
+                           ntraces, n2, 1);
+        SH_TRY(sh_launch_check(ctx, "svmf_traces"));
+    } else {
+        SH_TRY(launch_median(ctx, P.U, yT, n, ns - m, nmf));
+    }
     for (int i3 = 0; i3 < n3; i3++)
         SH_TRY(transpose(ctx, yT + (long)i3 * n1 * n2, out + (long)i3 * n1 * n2, n1, n2, false));
     return SH_OK;
