@@ -222,7 +222,7 @@ int sh_dip2d_batched(sh_ctx *ctx, const float *data, const long *offs, const int
                      const int *n2s, int npatch, int niter, int liter, int order, int r1,
                      int r2, int tile, const float *init, float *dip, int *rec);
 /* somf2dc per patch: csomf2d (pyseistr/src/sof_cfuns.c:1534-1672, behind
- * somf2dc, pyseistr/somf2d.py:60) with option 1 (n3 = 1 per patch). */
+ * somf2dc, pyseistr/somf2d.py:60), option 1 or 2 (n3 = 1 per patch). */
 int sh_somf2d_batched(sh_ctx *ctx, const float *data, const float *dip, const long *offs,
                       const int *n1s, const int *n2s, int npatch, int ns, int nmf, int option,
                       int order, float eps, float *out);

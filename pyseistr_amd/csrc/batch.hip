@@ -545,6 +545,24 @@ int sh_dip2d_batched(sh_ctx *ctx, const float *data, const long *offs, const int
     return SH_OK;
 }
 
+}  // extern "C"
+
+namespace {
+#include "svmf.hpp"
+
+// space-varying median of every trace of every patch (chain-block grid of
+// bspray2: n2 threads of each patch's blocks)
+__global__ void bsvmf(const float *__restrict__ U, float *__restrict__ out, const BPatch *__restrict__ P,
+                      const int *__restrict__ cblk_patch, long ntot, int np, int nfw) {
+    const BPatch q = P[cblk_patch[blockIdx.x]];
+    const int gid = (blockIdx.x - q.cblk0) * TB + threadIdx.x;
+    if (gid >= q.n2) return;
+    svmf_one(U, out, ntot, np, q.n1, nfw, q.off + gid, q.n2);
+}
+}  // namespace
+
+extern "C" {
+
 int sh_somf2d_batched(sh_ctx *ctx, const float *data, const float *dip, const long *offs, const int *n1s,
                       const int *n2s, int npatch, int ns, int nmf, int option, int order, float eps, float *out) {
     if (!ctx) return SH_ENULL;
@@ -554,7 +572,9 @@ int sh_somf2d_batched(sh_ctx *ctx, const float *data, const float *dip, const lo
     int np = 2 * ns + 1;
     if (nmf < 1 || nmf % 2 == 0 || nmf > np)
         return sh_fail(ctx, SH_EINVAL, "somf2d_batched: median length %d must be odd and <= %d", nmf, np);
-    if (option != 1) return sh_fail(ctx, SH_EUNSUP, "somf2d_batched: only option=1 (MF) is implemented");
+    if (option != 1 && option != 2) return sh_fail(ctx, SH_EINVAL, "somf2d_batched: option must be 1 or 2");
+    if (option == 2 && (nmf < 5 || nmf + 2 > 128))
+        return sh_fail(ctx, SH_EINVAL, "somf2d_batched: the space-varying median needs 5 <= nmf <= 125");
     if (order < 1 || order > 4) return sh_fail(ctx, SH_EUNSUP, "somf2d_batched: PWD order %d not supported", order);
     BatchTables T;
     SH_TRY(build_tables(ctx, offs, n1s, n2s, npatch, TILE_CFG[DEFAULT_TILE][0], TILE_CFG[DEFAULT_TILE][1], T));
@@ -589,7 +609,12 @@ int sh_somf2d_batched(sh_ctx *ctx, const float *data, const float *dip, const lo
         SH_TRY(sh_launch_check(ctx, "bspray2"));
     }
     int m = (nmf - 1) / 2;
-    SH_TRY(sh_median_internal(ctx, U, yT, n, ns - m, nmf));
+    if (option == 2) {
+        hipLaunchKernelGGL(bsvmf, cg, dim3(TB), 0, ctx->stream, U, yT, T.P, T.cblk_patch, n, np, nmf);
+        SH_TRY(sh_launch_check(ctx, "bsvmf"));
+    } else {
+        SH_TRY(sh_median_internal(ctx, U, yT, n, ns - m, nmf));
+    }
     return btrans(ctx, T, yT, out, false);
 }
 

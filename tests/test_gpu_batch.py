@@ -56,6 +56,11 @@ def test_somf2d_batched_bit_exact(env):
     for p, d, g in zip(pats[:3], dips[:3], got[:3]):
         ref = ps.somf2dc(p, d, 3, 1, 0.01, option=1, verb=0)
         assert np.array_equal(_bits(g), _bits(ref))
+    # the space-varying median (option 2) equals the per-patch filter
+    got = b.split(batch.somf2d_batched(b, db.data, ns=4, order=2, eps=0.01, option=2))
+    for p, d, g in zip(pats[:4], dips[:4], got[:4]):
+        ref = ps.somf2dc(p, d, 4, 2, 0.01, option=2, verb=0)
+        assert np.array_equal(_bits(g), _bits(ref))
 
 
 @pytest.mark.parametrize("tile", range(6))
@@ -118,5 +123,5 @@ def test_batched_errors(env):
     assert rc == -1
     with pytest.raises(ValueError):
         batch.dip2d_batched(b, niter=1, tile=99)
-    with pytest.raises(NotImplementedError):
-        batch.somf2d_batched(b, b.data, ns=2, option=2)
+    with pytest.raises(ValueError):
+        batch.somf2d_batched(b, b.data, ns=2, option=3)
