@@ -31,6 +31,8 @@ typedef struct sh_ctx sh_ctx;
 /* Create a context bound to a HIP stream (NULL = default stream). */
 sh_ctx *sh_ctx_create(void *hip_stream);
 void sh_ctx_destroy(sh_ctx *ctx);
+/* Rebind the context to another stream; the new stream first waits for the
+ * work already queued on the old one (the workspace is shared). */
 int sh_ctx_set_stream(sh_ctx *ctx, void *hip_stream);
 /* Release cached device workspace. */
 int sh_ctx_trim(sh_ctx *ctx);

@@ -131,7 +131,7 @@ def context():
             raise SeistrError("sh_ctx_create failed")
         ctxs[dev] = ctx
     stream = torch.cuda.current_stream().cuda_stream
-    lib.sh_ctx_set_stream(ctx, c_void_p(stream))
+    check(ctx, lib.sh_ctx_set_stream(ctx, c_void_p(stream)), "context")
     return ctx
 
 
@@ -162,6 +162,36 @@ def profile_report():
 
 def ptr(t):
     return c_void_p(t.data_ptr()) if t is not None else None
+
+
+def nsamples(a):
+    """Number of samples of a numpy array, a sequence or a tensor."""
+    return int(a.numel()) if hasattr(a, "numel") else int(np.size(a))
+
+
+def check_sizes(what, n, **arrays):
+    """ValueError unless every given array (numpy or tensor; None skipped)
+    holds exactly n samples.  The kernels index each input and output by the
+    dimensions alone, so a short buffer would be read or written past its
+    end; this runs before the library is called."""
+    for name, a in arrays.items():
+        if a is None:
+            continue
+        k = nsamples(a)
+        if k != n:
+            raise ValueError(f"{what}: {name} has {k} samples, {n} expected")
+
+
+def check_out(out, n, what):
+    """A caller-supplied output must be a flat contiguous float32 device
+    tensor of exactly n samples."""
+    if out is None:
+        return None
+    check_sizes(what, n, out=out)
+    torch = _torch()
+    if not (isinstance(out, torch.Tensor) and out.is_cuda and out.dtype == torch.float32 and out.is_contiguous()):
+        raise TypeError(f"{what}: out must be a contiguous float32 device tensor")
+    return out
 
 
 def to_device(a):

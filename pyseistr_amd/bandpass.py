@@ -8,6 +8,9 @@ from . import _lib
 def bandpass_device(din, shape, dt, flo=0.0, fhi=0.5, nplo=6, nphi=6, phase=0, out=None):
     """Device-level call: flat F-order float32 CUDA tensors in and out."""
     n1, n2, n3 = (int(s) for s in shape)
+    _lib.check_sizes("bandpass", n1 * n2 * n3, data=din)
+    _lib.check_out(out, n1 * n2 * n3, "bandpass")
+    din = _lib.to_device(din)
     ctx = _lib.context()
     lib = _lib.load()
     if out is None:

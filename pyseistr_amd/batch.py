@@ -71,6 +71,8 @@ class Batch:
 def dip2d_batched(batch, niter=5, liter=20, order=2, rect=(10, 10, 1), tile=None, init=None, out=None,
                   report=False):
     """Device call: dips of every patch, flat like batch.data."""
+    _lib.check_sizes("dip2d_batched", batch.ntot, data=batch.data, init=init)
+    _lib.check_out(out, batch.ntot, "dip2d_batched")
     ctx = _lib.context()
     if out is None:
         out = _lib.empty(batch.ntot)
@@ -93,6 +95,8 @@ def dip2d_batched(batch, niter=5, liter=20, order=2, rect=(10, 10, 1), tile=None
 def somf2d_batched(batch, dip, ns=6, order=2, eps=0.01, option=1, nmf=None, out=None):
     """Device call: structure-oriented median of every patch (nmf = 2ns+1,
     as somf2dc passes it, pyseistr/somf2d.py:96)."""
+    _lib.check_sizes("somf2d_batched", batch.ntot, data=batch.data, dip=dip)
+    _lib.check_out(out, batch.ntot, "somf2d_batched")
     ctx = _lib.context()
     if out is None:
         out = _lib.empty(batch.ntot)

@@ -145,6 +145,7 @@ void sh_ctx_destroy(sh_ctx *ctx) {
     if (!ctx) return;
     sh_ctx_trim(ctx);
     for (auto e : ctx->pool) (void)hipEventDestroy(e);
+    if (ctx->ev_switch) (void)hipEventDestroy(ctx->ev_switch);
     if (ctx->aux) {
         (void)hipStreamDestroy(ctx->aux);
         (void)hipEventDestroy(ctx->ev_fork);
@@ -155,7 +156,16 @@ void sh_ctx_destroy(sh_ctx *ctx) {
 
 int sh_ctx_set_stream(sh_ctx *ctx, void *hip_stream) {
     if (!ctx) return SH_ENULL;
-    ctx->stream = (hipStream_t)hip_stream;
+    hipStream_t next = (hipStream_t)hip_stream;
+    if (next != ctx->stream) {
+        // The workspace slots are shared by every call on this context: work
+        // still queued on the old stream may read or write them, so the new
+        // stream waits for it before anything is launched there.
+        if (!ctx->ev_switch) SH_HIP(ctx, hipEventCreateWithFlags(&ctx->ev_switch, hipEventDisableTiming));
+        SH_HIP(ctx, hipEventRecord(ctx->ev_switch, ctx->stream));
+        SH_HIP(ctx, hipStreamWaitEvent(next, ctx->ev_switch, 0));
+        ctx->stream = next;
+    }
     return SH_OK;
 }
 

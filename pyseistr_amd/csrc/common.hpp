@@ -32,6 +32,8 @@ struct sh_ctx {
     // stream inside one call, joined back before the call returns
     hipStream_t aux = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    // orders the previous stream before the next one in sh_ctx_set_stream
+    hipEvent_t ev_switch = nullptr;
     std::string err;
     std::vector<void *> slot_ptr;
     std::vector<size_t> slot_size;

@@ -26,6 +26,9 @@ def dip_device(din, shape, niter=5, liter=20, order=2, eps_dv=0.01, rect=(1, 1, 
     n1, n2, n3 = (int(s) for s in shape)
     n = n1 * n2 * n3
     nd = 1 if n3 == 1 else 2
+    _lib.check_sizes("dip", n, data=din, mask=mask)
+    _lib.check_sizes("dip", nd * n, init=init)
+    _lib.check_out(out, nd * n, "dip")
     ctx = _lib.context()
     lib = _lib.load()
     if out is None:
@@ -45,14 +48,13 @@ def dip_device(din, shape, niter=5, liter=20, order=2, eps_dv=0.01, rect=(1, 1, 
 
 def _run(din, shape, niter, liter, order, eps_dv, rect, mask, init, report):
     n = int(np.prod(shape))
+    nd = 1 if shape[2] == 1 else 2
+    if mask is not None and _lib.nsamples(mask) != n:
+        raise ValueError("Mask and Data should have the same dimension")
+    _lib.check_sizes("dip", n, data=din)
+    _lib.check_sizes("dip", nd * n, init=init)
     d = _lib.to_device(din)
-    if d.numel() != n:
-        raise ValueError(f"data has {d.numel()} samples, shape {shape} needs {n}")
-    m = None
-    if mask is not None:
-        m = _lib.to_device(mask)
-        if m.numel() != n:
-            raise ValueError("Mask and Data should have the same dimension")
+    m = _lib.to_device(mask) if mask is not None else None
     ini = _lib.to_device(init) if init is not None else None
     res = dip_device(d, shape, niter, liter, order, eps_dv, rect, m, ini, report=report)
     if report:

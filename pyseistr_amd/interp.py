@@ -25,6 +25,7 @@ def soint3dc(din, mask, dipi, dipx, order=1, niter=100, njs=[1, 1], drift=0, see
              verb=1, report=False):
     din = np.asarray(din)
     n1, n2, n3 = din.shape
+    _lib.check_sizes("soint3dc", n1 * n2 * n3, mask=mask, dipi=dipi, dipx=dipx)
     ctx = _lib.context()
     d, m = _lib.to_device(din), _lib.to_device(mask)
     p, q = _lib.to_device(dipi), _lib.to_device(dipx)
@@ -46,6 +47,7 @@ def soint2dc(din, mask, dip, order=1, niter=100, njs=[1, 1], drift=0, hasmask=1,
         return _soint2d_prec(din, mask, dip, order, niter, hasmask, twoplane, report)
     if twoplane:
         dip = np.asarray(dip)[:, :, 0]
+    _lib.check_sizes("soint2dc", n1 * n2, mask=mask, dip=dip)
     ctx = _lib.context()
     d, m, p = _lib.to_device(din), _lib.to_device(mask), _lib.to_device(dip)
     out = _lib.empty(n1 * n2)
@@ -62,6 +64,8 @@ def _soint2d_prec(din, mask, dip, order, niter, hasmask, twoplane, report):
     twoplane, dip[:, :, 0] and dip[:, :, 1] are the two slope planes."""
     n1, n2 = din.shape
     dip = np.asarray(dip)
+    _lib.check_sizes("soint2dc", n1 * n2, mask=mask)
+    _lib.check_sizes("soint2dc", n1 * n2 * (2 if twoplane else 1), dip=dip)
     ctx = _lib.context()
     d, m = _lib.to_device(din), _lib.to_device(mask)
     p = _lib.to_device(dip[:, :, 0] if twoplane else dip)
@@ -78,6 +82,7 @@ def _soint2d_prec(din, mask, dip, order, niter, hasmask, twoplane, report):
 def sint3dc(din, mask, dipi, dipx, niter=100, eps=0.01, ns1=1, ns2=1, order1=1, order2=1, verb=1, report=False):
     din = np.asarray(din)
     n1, n2, n3 = din.shape
+    _lib.check_sizes("sint3dc", n1 * n2 * n3, mask=mask, dipi=dipi, dipx=dipx)
     ctx = _lib.context()
     d, m = _lib.to_device(din), _lib.to_device(mask)
     p, q = _lib.to_device(dipi), _lib.to_device(dipx)
@@ -94,6 +99,7 @@ def sint3dc(din, mask, dipi, dipx, niter=100, eps=0.01, ns1=1, ns2=1, order1=1, 
 def sint2dc(din, mask, dip, niter=100, eps=0.01, ns=1, order=1, verb=1, report=False):
     din = np.asarray(din)
     n1, n2 = din.shape
+    _lib.check_sizes("sint2dc", n1 * n2, mask=mask, dip=dip)
     ctx = _lib.context()
     d, m, p = _lib.to_device(din), _lib.to_device(mask), _lib.to_device(dip)
     out = _lib.empty(n1 * n2)

@@ -8,6 +8,9 @@ from . import _lib
 def smooth_device(din, shape, rect=(1, 1, 1), diff=(0, 0, 0), box=(0, 0, 0), repeat=1, adj=1, out=None):
     """Device-level call: flat F-order float32 CUDA tensors in and out."""
     n1, n2, n3 = (int(s) for s in shape)
+    _lib.check_sizes("smooth", n1 * n2 * n3, data=din)
+    _lib.check_out(out, n1 * n2 * n3, "smooth")
+    din = _lib.to_device(din)
     ctx = _lib.context()
     lib = _lib.load()
     if out is None:

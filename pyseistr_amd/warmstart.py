@@ -60,6 +60,9 @@ def cnn_forward(params, d, shape, xscale, out=None, dil=None):
     dil: per-layer dilations (None = all 1)."""
     n1, n2, n3 = (int(s) for s in shape)
     n = n1 * n2 * n3
+    _lib.check_sizes("cnn_forward", n, data=d)
+    _lib.check_sizes("cnn_forward", NPARAM, params=params)
+    _lib.check_out(out, 2 * n, "cnn_forward")
     ctx = _lib.context()
     if out is None:
         out = _lib.empty(2 * n)
@@ -72,6 +75,11 @@ def cnn_forward(params, d, shape, xscale, out=None, dil=None):
 def cnn_grad(params, d, shape, xscale, target, wt=None, grad=None, want_loss=True, dil=None):
     """Device call: (loss or None, grad [NPARAM])."""
     n1, n2, n3 = (int(s) for s in shape)
+    n = n1 * n2 * n3
+    _lib.check_sizes("cnn_grad", n, data=d, wt=wt)
+    _lib.check_sizes("cnn_grad", 2 * n, target=target)
+    _lib.check_sizes("cnn_grad", NPARAM, params=params)
+    _lib.check_out(grad, NPARAM, "cnn_grad")
     ctx = _lib.context()
     if grad is None:
         grad = _lib.empty(NPARAM)
