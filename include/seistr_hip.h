@@ -15,6 +15,8 @@
 #ifndef SEISTR_HIP_H
 #define SEISTR_HIP_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -163,6 +165,23 @@ int sh_somean2d(sh_ctx *ctx, const float *dn, const float *dip, int n1, int n2, 
  * (pyseistr/somf2d.py:60). */
 int sh_somf2d(sh_ctx *ctx, const float *dn, const float *dip, int n1, int n2, int n3,
               int ns, int nmf, int option, int order, float eps, float *out);
+
+/* sh_somf2d without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments and results as sh_somf2d (bit-identical);
+ * the workspace is a fixed device array of SH_SOMF2D_NOALLOC_MAX_BYTES
+ * inside the library, one per GPU, held until the process ends.  A call
+ * whose workspace (sh_somf2d_noalloc_bytes) exceeds it fails with
+ * SH_EINVAL before anything is launched; a 512 x 256 gather with ns = 8
+ * needs about 18 MB.  Every call on one GPU shares that array, whatever
+ * its context: calls that overlap on one GPU (two streams or threads)
+ * race and must be ordered by the caller.  Launch timing is off inside
+ * the call (it would create events). */
+#define SH_SOMF2D_NOALLOC_MAX_BYTES ((size_t)64 << 20)
+size_t sh_somf2d_noalloc_bytes(int n1, int n2, int n3, int ns, int order);
+int sh_somf2d_noalloc(sh_ctx *ctx, const float *dn, const float *dip, int n1, int n2, int n3,
+                      int ns, int nmf, int option, int order, float eps, float *out);
 
 /* PWD-regularised interpolation of unknown samples.  Replaces
  * soint3dcfun.csoint3d (pyseistr/src/soint3d_cfuns.c:2405-2508) behind

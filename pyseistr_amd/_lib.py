@@ -60,6 +60,9 @@ _SIGS = {
                             c_float, c_void_p]),
     "sh_somf2d": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_int, c_int, c_int,
                           c_float, c_void_p]),
+    "sh_somf2d_noalloc_bytes": (ctypes.c_size_t, [c_int, c_int, c_int, c_int, c_int]),
+    "sh_somf2d_noalloc": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_int, c_int, c_int,
+                                  c_float, c_void_p]),
     "sh_soint3d": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_int,
                            c_int, c_int, c_int, c_int, c_int, c_int, c_float, c_void_p, c_int_p]),
     "sh_soint2d_prec": (c_int, [c_void_p] * 5 + [c_int] * 5 + [c_void_p, c_int_p]),

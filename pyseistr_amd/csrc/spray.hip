@@ -23,6 +23,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <mutex>
 
 namespace {
 
@@ -693,13 +694,17 @@ int stage3(sh_ctx *ctx, const float *din, const float *dipi, const float *dipx, 
 // Internal API (declared in common.hpp): 2-D structure-oriented smoothing.
 
 // Scratch (solver + running trace planes) and the factor cache of P.
-int pws2_prepare(sh_ctx *ctx, Pwsmooth2 &P, int slot_scr, int slot_F) {
+// Threads of the 2-D spray chains (two directions per trace and slice).
+long pws2_nthr(int n2, int n3) { return (long)ceil_div(2L * n2, TB) * TB * n3; }
+
+// Factor cache of P into caller-provided buffers: scr holds 2*nthr*n1
+// floats (solver scratch, then the running trace planes), F 2*(2*order+1)*n.
+int pws2_prepare_bufs(sh_ctx *ctx, Pwsmooth2 &P, float *scr, float *F) {
     long n = (long)P.n1 * P.n2 * P.n3;
-    int NB1 = 2 * P.order + 1;
-    P.nthr = (long)ceil_div(2L * P.n2, TB) * TB * P.n3;
-    SH_TRY(sh_slot(ctx, slot_scr, (size_t)P.nthr * P.n1 * 2 * sizeof(float), (void **)&P.scr));
+    P.nthr = pws2_nthr(P.n2, P.n3);
+    P.scr = scr;
     P.tr = P.scr + (long)P.nthr * P.n1;
-    SH_TRY(sh_slot(ctx, slot_F, (size_t)2 * NB1 * n * sizeof(float), (void **)&P.F));
+    P.F = F;
     dim3 grid(ceil_div(2L * P.n2, TB), P.n3);
     PROF_A(ctx);
     SH_DISPATCH_NW(ctx, P.order, NW,
@@ -707,6 +712,16 @@ int pws2_prepare(sh_ctx *ctx, Pwsmooth2 &P, int slot_scr, int slot_F) {
                                       P.n3, P.e));
     PROF_B(ctx, "factor2_kernel", 4.0 * (double)n);
     return sh_launch_check(ctx, "factor2_kernel");
+}
+
+int pws2_prepare(sh_ctx *ctx, Pwsmooth2 &P, int slot_scr, int slot_F) {
+    long n = (long)P.n1 * P.n2 * P.n3;
+    int NB1 = 2 * P.order + 1;
+    const long nthr = pws2_nthr(P.n2, P.n3);
+    float *scr = nullptr, *F = nullptr;
+    SH_TRY(sh_slot(ctx, slot_scr, (size_t)nthr * P.n1 * 2 * sizeof(float), (void **)&scr));
+    SH_TRY(sh_slot(ctx, slot_F, (size_t)2 * NB1 * n * sizeof(float), (void **)&F));
+    return pws2_prepare_bufs(ctx, P, scr, F);
 }
 
 int pws2_spray(sh_ctx *ctx, Pwsmooth2 &P, const float *x) {
@@ -947,9 +962,12 @@ int sh_somean2d(sh_ctx *ctx, const float *dn, const float *dip, int n1, int n2, 
     return SH_OK;
 }
 
-int sh_somf2d(sh_ctx *ctx, const float *dn, const float *dip, int n1, int n2, int n3, int ns, int nmf, int option,
-              int order, float eps, float *out) {
-    if (!ctx) return SH_ENULL;
+}  // extern "C"
+
+namespace {
+
+int somf2d_args(sh_ctx *ctx, const float *dn, const float *dip, const float *out, int n1, int n2, int n3, int ns,
+                int nmf, int option) {
     if (!dn || !dip || !out) return sh_fail(ctx, SH_ENULL, "somf2d: NULL pointer");
     if (n1 < 8 || n2 < 1 || n3 < 1 || ns < 0 || ns > 40) return sh_fail(ctx, SH_EINVAL, "somf2d: bad arguments");
     int np = 2 * ns + 1;
@@ -958,17 +976,54 @@ int sh_somf2d(sh_ctx *ctx, const float *dn, const float *dip, int n1, int n2, in
     if (option != 1 && option != 2) return sh_fail(ctx, SH_EINVAL, "somf2d: option must be 1 or 2");
     if (option == 2 && (nmf < 5 || nmf + 2 > 128))
         return sh_fail(ctx, SH_EINVAL, "somf2d: the space-varying median needs 5 <= nmf <= 125");
+    return SH_OK;
+}
+
+// Workspace of one somf2d call, in floats, each part a multiple of 64
+// floats (256 bytes): staged data, dip and result (3n), the spray (n*np),
+// the chains' solver scratch (2*nthr*n1) and the factor cache
+// (2*(2*order+1)*n).
+struct Somf2dWs {
+    size_t b, U, scr, F, total;
+};
+Somf2dWs somf2d_ws(int n1, int n2, int n3, int ns, int order) {
+    auto up = [](size_t v) { return (v + 63) & ~(size_t)63; };
+    const size_t n = (size_t)n1 * n2 * n3;
+    Somf2dWs w;
+    w.b = up(3 * n);
+    w.U = up(n * (2 * ns + 1));
+    w.scr = up((size_t)pws2_nthr(n2, n3) * n1 * 2);
+    w.F = up((size_t)2 * (2 * order + 1) * n);
+    w.total = w.b + w.U + w.scr + w.F;
+    return w;
+}
+
+// csomf2d on device buffers; ws (nullable) is the caller's workspace of
+// somf2d_ws(...).total floats, otherwise the context's cached slots.
+int somf2d_core(sh_ctx *ctx, const float *dn, const float *dip, int n1, int n2, int n3, int ns, int nmf, int option,
+                int order, float eps, float *out, float *ws) {
+    int np = 2 * ns + 1;
     long n = (long)n1 * n2 * n3;
     float *b = nullptr;
-    SH_TRY(sh_slot(ctx, SLOT_SPRAY0 + 1, (size_t)n * 3 * sizeof(float), (void **)&b));
+    Pwsmooth2 P{n1, n2, n3, ns, order, eps * eps, nullptr};
+    const Somf2dWs w = somf2d_ws(n1, n2, n3, ns, order);
+    if (ws) {
+        b = ws;
+        P.U = ws + w.b;
+    } else {
+        SH_TRY(sh_slot(ctx, SLOT_SPRAY0 + 1, (size_t)n * 3 * sizeof(float), (void **)&b));
+        SH_TRY(sh_slot(ctx, SLOT_SPRAY0 + 6, (size_t)n * np * sizeof(float), (void **)&P.U));
+    }
     float *xT = b, *dT = b + n, *yT = b + 2 * n;
+    P.dip = dT;
     for (int i3 = 0; i3 < n3; i3++) {
         SH_TRY(transpose(ctx, dip + (long)i3 * n1 * n2, dT + (long)i3 * n1 * n2, n1, n2, true));
         SH_TRY(transpose(ctx, dn + (long)i3 * n1 * n2, xT + (long)i3 * n1 * n2, n1, n2, true));
     }
-    Pwsmooth2 P{n1, n2, n3, ns, order, eps * eps, dT};
-    SH_TRY(sh_slot(ctx, SLOT_SPRAY0 + 6, (size_t)n * np * sizeof(float), (void **)&P.U));
-    SH_TRY(pws2_prepare(ctx, P, SLOT_SPRAY0 + 7, SLOT_SPRAY0 + 9));
+    if (ws)
+        SH_TRY(pws2_prepare_bufs(ctx, P, ws + w.b + w.U, ws + w.b + w.U + w.scr));
+    else
+        SH_TRY(pws2_prepare(ctx, P, SLOT_SPRAY0 + 7, SLOT_SPRAY0 + 9));
     hipLaunchKernelGGL(zero_kernel, dim3(ceil_div(n * np, TB)), dim3(TB), 0, ctx->stream, P.U, n * np);
     SH_TRY(pws2_spray(ctx, P, xT));
     int m = (nmf - 1) / 2;
@@ -983,6 +1038,74 @@ int sh_somf2d(sh_ctx *ctx, const float *dn, const float *dip, int n1, int n2, in
     for (int i3 = 0; i3 < n3; i3++)
         SH_TRY(transpose(ctx, yT + (long)i3 * n1 * n2, out + (long)i3 * n1 * n2, n1, n2, false));
     return SH_OK;
+}
+
+// Fixed workspace of sh_somf2d_noalloc: every device that runs it holds one
+// copy until the process ends.  Host code never uses this name as an
+// address (on the host it is only a stand-in): the device address comes
+// from hipGetSymbolAddress, once per device.
+constexpr size_t NOALLOC_FLOATS = (size_t)SH_SOMF2D_NOALLOC_MAX_BYTES / sizeof(float);
+static __device__ __align__(256) float scratch[NOALLOC_FLOATS];
+
+constexpr int NOALLOC_MAX_DEV = 64;
+std::mutex noalloc_mu;
+float *noalloc_ptr[NOALLOC_MAX_DEV] = {};
+
+// Device address of `scratch` on the context's device (looked up on the
+// first call there, with that device current).
+int noalloc_scratch(sh_ctx *ctx, float **p) {
+    const int dev = ctx->device;
+    if (dev < 0 || dev >= NOALLOC_MAX_DEV) return sh_fail(ctx, SH_EUNSUP, "somf2d_noalloc: device %d", dev);
+    std::lock_guard<std::mutex> lock(noalloc_mu);
+    if (!noalloc_ptr[dev]) {
+        int cur = 0;
+        SH_HIP(ctx, hipGetDevice(&cur));
+        if (cur != dev) SH_HIP(ctx, hipSetDevice(dev));
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (cur != dev) (void)hipSetDevice(cur);
+        if (e != hipSuccess || !ptr)
+            return sh_fail(ctx, SH_EHIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+        noalloc_ptr[dev] = (float *)ptr;
+    }
+    *p = noalloc_ptr[dev];
+    return SH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sh_somf2d(sh_ctx *ctx, const float *dn, const float *dip, int n1, int n2, int n3, int ns, int nmf, int option,
+              int order, float eps, float *out) {
+    if (!ctx) return SH_ENULL;
+    SH_TRY(somf2d_args(ctx, dn, dip, out, n1, n2, n3, ns, nmf, option));
+    return somf2d_core(ctx, dn, dip, n1, n2, n3, ns, nmf, option, order, eps, out, nullptr);
+}
+
+size_t sh_somf2d_noalloc_bytes(int n1, int n2, int n3, int ns, int order) {
+    if (n1 < 1 || n2 < 1 || n3 < 1 || ns < 0 || order < 1) return 0;
+    return somf2d_ws(n1, n2, n3, ns, order).total * sizeof(float);
+}
+
+int sh_somf2d_noalloc(sh_ctx *ctx, const float *dn, const float *dip, int n1, int n2, int n3, int ns, int nmf,
+                      int option, int order, float eps, float *out) {
+    if (!ctx) return SH_ENULL;
+    SH_TRY(somf2d_args(ctx, dn, dip, out, n1, n2, n3, ns, nmf, option));
+    if (order < 1 || order > 4) return sh_fail(ctx, SH_EUNSUP, "PWD order %d not supported (1..4)", order);
+    const size_t need = somf2d_ws(n1, n2, n3, ns, order).total;
+    if (need > NOALLOC_FLOATS)
+        return sh_fail(ctx, SH_EINVAL, "somf2d_noalloc: %d x %d x %d with ns = %d, order = %d needs %zu bytes of "
+                       "workspace, above the fixed %zu", n1, n2, n3, ns, order, need * sizeof(float),
+                       NOALLOC_FLOATS * sizeof(float));
+    float *ws = nullptr;
+    SH_TRY(noalloc_scratch(ctx, &ws));
+    // launch timers would create events: off for this call
+    const int prof = ctx->prof;
+    ctx->prof = 0;
+    const int rc = somf2d_core(ctx, dn, dip, n1, n2, n3, ns, nmf, option, order, eps, out, ws);
+    ctx->prof = prof;
+    return rc;
 }
 
 }  // extern "C"
