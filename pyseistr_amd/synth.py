@@ -13,7 +13,9 @@ def _ricker(tau, f, dt):
     return (1.0 - 2.0 * b) * np.exp(-b)
 
 
-def _events(shape, n_events, freqs, smax, rng, amp=(0.5, 1.0), dt=0.004, curved=True):
+def _events(shape, n_events, freqs, smax, rng, amp=(0.5, 1.0), dt=0.004, curved=True, rec=None):
+    """Sum of curved Ricker events; rec (a list) receives each event's
+    (t0, a, b, c, d, xk, yk, f, amplitude)."""
     n1 = shape[0]
     n2 = shape[1]
     n3 = shape[2] if len(shape) > 2 else 1
@@ -32,7 +34,10 @@ def _events(shape, n_events, freqs, smax, rng, amp=(0.5, 1.0), dt=0.004, curved=
             d = rng.uniform(-1, 1) * smax / (4.0 * max(n3, 2)) if n3 > 1 else 0.0
         t0 = rng.uniform(0.15, 0.85) * n1
         tt = t0 + a * (x - n2 / 2) + b * (y - n3 / 2) + c * (x - xk) ** 2 + d * (y - yk) ** 2
-        out += rng.uniform(*amp) * _ricker(t - tt, f, dt)
+        am = rng.uniform(*amp)
+        out += am * _ricker(t - tt, f, dt)
+        if rec is not None:
+            rec.append((t0, a, b, c, d, xk, yk, f, am))
     return out
 
 
@@ -64,6 +69,36 @@ def volume3d(shape=(256, 128, 128), seed=2, n_events=4, freqs=(25.0, 28.0, 32.0,
         m = rng.random(clean.shape) < spike_frac
         noisy[m] += spike_amp * rng.choice([-1.0, 1.0], size=int(m.sum()))
     return noisy.astype(np.float32), clean.astype(np.float32)
+
+
+def true_slopes3d(shape=(256, 128, 128), seed=2, n_events=4, freqs=(25.0, 28.0, 32.0, 35.0), smax=2.0, lobe=1.5,
+                  dt=0.004):
+    """The generator's own slopes of a volume3d call with the same
+    arguments: (inline, crossline, support).  Slopes are dt_k/dx and
+    dt_k/dy of the event whose main lobe covers the sample (samples per
+    trace, the sign convention of dip3d); support marks samples within
+    `lobe` samples of exactly one event's traveltime surface (crossings and
+    event-free samples, where the slope is not defined by the data, are
+    left out)."""
+    rng = np.random.default_rng(seed)
+    rec = []
+    _events(shape, n_events, list(freqs), smax, rng, dt=dt, rec=rec)
+    n1, n2, n3 = shape
+    t = np.arange(n1, dtype=np.float64)[:, None, None]
+    x = np.arange(n2, dtype=np.float64)[None, :, None]
+    y = np.arange(n3, dtype=np.float64)[None, None, :]
+    si = np.zeros(shape, np.float32)
+    sx = np.zeros(shape, np.float32)
+    hits = np.zeros(shape, np.int8)
+    for (t0, a, b, c, d, xk, yk, f, am) in rec:
+        tt = t0 + a * (x - n2 / 2) + b * (y - n3 / 2) + c * (x - xk) ** 2 + d * (y - yk) ** 2
+        near = np.abs(t - tt) <= lobe
+        gi = np.broadcast_to(a + 2 * c * (x - xk), shape)
+        gx = np.broadcast_to(b + 2 * d * (y - yk), shape)
+        si = np.where(near, gi, si).astype(np.float32)
+        sx = np.where(near, gx, sx).astype(np.float32)
+        hits += near.astype(np.int8)
+    return si, sx, hits == 1
 
 
 def interp3d(shape=(300, 100, 100), seed=3, mask_seed=33, keep=0.3, n_events=3, smax=1.5):
