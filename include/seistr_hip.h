@@ -27,12 +27,32 @@ extern "C" {
 #define SH_EALLOC (-3)   /* device allocation failed */
 #define SH_EHIP (-4)     /* HIP runtime error */
 #define SH_EUNSUP (-5)   /* option the library does not implement */
+#define SH_ENONFINITE (-6) /* an input holds NaN or Inf (with SH_FLAG_CHECK_FINITE) */
+
+/* Context flags of the path's entry points (sh_dip, sh_pwspray2d/3d,
+ * sh_somean2d/3d, sh_somf2d/3d, sh_soint2d/3d, sh_soint2d_prec,
+ * sh_sint2d/3d); the reference checks neither, so both are off by default.
+ * SH_FLAG_CHECK_FINITE: every float input is scanned first and the call
+ *   returns SH_ENONFINITE, naming the input, if one holds NaN or Inf (the
+ *   scan synchronises the stream).
+ * SH_FLAG_HOST_PTRS: the array arguments are host memory; inputs are
+ *   copied to device workspace, outputs copied back, and the call returns
+ *   when they are (synchronous).  Without it every pointer is device
+ *   memory. */
+#define SH_FLAG_CHECK_FINITE 1
+#define SH_FLAG_HOST_PTRS 2
 
 typedef struct sh_ctx sh_ctx;
 
 /* Create a context bound to a HIP stream (NULL = default stream). */
 sh_ctx *sh_ctx_create(void *hip_stream);
 void sh_ctx_destroy(sh_ctx *ctx);
+/* Set / read the context's SH_FLAG_* bits. */
+int sh_ctx_set_flags(sh_ctx *ctx, int flags);
+int sh_ctx_get_flags(const sh_ctx *ctx);
+/* *nbad = number of NaN or Inf values in the device array a[n]
+ * (synchronises). */
+int sh_count_nonfinite(sh_ctx *ctx, const float *a, long n, long *nbad);
 /* Rebind the context to another stream; the new stream first waits for the
  * work already queued on the old one (the workspace is shared). */
 int sh_ctx_set_stream(sh_ctx *ctx, void *hip_stream);
@@ -52,11 +72,14 @@ int sh_prof_report(sh_ctx *ctx, char *buf, int len);
 
 /* Per-direction iteration record of a dip call: for GN iteration k < 64,
  * cg_iters[d][k] = shaping-CG iterations run (-1: zero denominator),
- * ls_step[d][k] = accepted line-search halving (8 = none accepted). */
+ * ls_step[d][k] = accepted line-search halving (8 = none accepted);
+ * elapsed_ms = device time of the call on its stream, from its first
+ * launch to its last (HIP events). */
 typedef struct {
     int ndir, niter;
     int cg_iters[2][64];
     int ls_step[2][64];
+    float elapsed_ms;
 } sh_dip_report;
 
 /* Local slopes by shaping-regularised plane-wave destruction.
@@ -177,7 +200,8 @@ int sh_somf2d(sh_ctx *ctx, const float *dn, const float *dip, int n1, int n2, in
  * needs about 18 MB.  Every call on one GPU shares that array, whatever
  * its context: calls that overlap on one GPU (two streams or threads)
  * race and must be ordered by the caller.  Launch timing is off inside
- * the call (it would create events). */
+ * the call (it would create events); SH_FLAG_HOST_PTRS is refused
+ * (SH_EINVAL) and SH_FLAG_CHECK_FINITE does not apply. */
 #define SH_SOMF2D_NOALLOC_MAX_BYTES ((size_t)64 << 20)
 size_t sh_somf2d_noalloc_bytes(int n1, int n2, int n3, int ns, int order);
 int sh_somf2d_noalloc(sh_ctx *ctx, const float *dn, const float *dip, int n1, int n2, int n3,

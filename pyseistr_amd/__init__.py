@@ -7,7 +7,7 @@ sint2dc, sint3dc, smoothc, bandpassc, patch2d, patch3d, patch2d_inv,
 patch3d_inv, fkdip.  All computation runs in libseistr_hip.so; there is no CPU
 fallback.
 """
-from ._lib import SeistrError  # noqa: F401
+from ._lib import SeistrError, check_finite  # noqa: F401
 from .bandpass import bandpassc  # noqa: F401
 from .dip import dip2d, dip2dc, dip3d, dip3dc  # noqa: F401
 from .fk import fkdip  # noqa: F401

@@ -26,7 +26,11 @@ class SeistrError(RuntimeError):
 
 class DipReport(ctypes.Structure):
     _fields_ = [("ndir", c_int), ("niter", c_int),
-                ("cg_iters", (c_int * 64) * 2), ("ls_step", (c_int * 64) * 2)]
+                ("cg_iters", (c_int * 64) * 2), ("ls_step", (c_int * 64) * 2), ("elapsed_ms", c_float)]
+
+
+FLAG_CHECK_FINITE = 1
+FLAG_HOST_PTRS = 2
 
 
 _SIGS = {
@@ -34,6 +38,9 @@ _SIGS = {
     "sh_ctx_destroy": (None, [c_void_p]),
     "sh_ctx_set_stream": (c_int, [c_void_p, c_void_p]),
     "sh_ctx_trim": (c_int, [c_void_p]),
+    "sh_ctx_set_flags": (c_int, [c_void_p, c_int]),
+    "sh_ctx_get_flags": (c_int, [c_void_p]),
+    "sh_count_nonfinite": (c_int, [c_void_p, c_void_p, ctypes.c_long, ctypes.POINTER(ctypes.c_long)]),
     "sh_last_error": (ctypes.c_char_p, [c_void_p]),
     "sh_version": (c_int, []),
     "sh_sync": (c_int, [c_void_p]),
@@ -142,11 +149,22 @@ def check(ctx, rc, what):
     if rc != 0:
         msg = load().sh_last_error(ctx)
         msg = msg.decode() if msg else ""
-        if rc == -1:
+        if rc in (-1, -6):  # bad argument, or non-finite input (SH_FLAG_CHECK_FINITE)
             raise ValueError(f"{what}: {msg}")
         if rc == -5:
             raise NotImplementedError(f"{what}: {msg}")
         raise SeistrError(f"{what}: {msg} (status {rc})")
+
+
+def check_finite(on=True):
+    """Make the path's calls on this thread's context scan their inputs for
+    NaN / Inf first and raise ValueError naming the input (off by default,
+    as in the reference, which propagates them)."""
+    ctx = context()
+    lib = load()
+    flags = lib.sh_ctx_get_flags(ctx)
+    flags = (flags | FLAG_CHECK_FINITE) if on else (flags & ~FLAG_CHECK_FINITE)
+    check(ctx, lib.sh_ctx_set_flags(ctx, flags), "check_finite")
 
 
 def profile_enable(on=True):

@@ -13,6 +13,7 @@ SRCS="common dip"
 [ -f "$HERE/bandpass.hip" ] && SRCS="$SRCS bandpass"
 [ -f "$HERE/patch.hip" ] && SRCS="$SRCS patch"
 [ -f "$HERE/fk.hip" ] && SRCS="$SRCS fk"
+[ -f "$HERE/boundary.hip" ] && SRCS="$SRCS boundary"
 OBJDIR="$HERE/_build"
 mkdir -p "$OBJDIR"
 FLAGS="--offload-arch=gfx950 -O3 -fPIC -std=c++17 -ffp-contract=off -Xclang -target-feature -Xclang -packed-fp32-ops -Wno-unused-result"

@@ -145,6 +145,8 @@ void sh_ctx_destroy(sh_ctx *ctx) {
     if (!ctx) return;
     sh_ctx_trim(ctx);
     for (auto e : ctx->pool) (void)hipEventDestroy(e);
+    if (ctx->ev_t0) (void)hipEventDestroy(ctx->ev_t0);
+    if (ctx->ev_t1) (void)hipEventDestroy(ctx->ev_t1);
     if (ctx->ev_switch) (void)hipEventDestroy(ctx->ev_switch);
     if (ctx->aux) {
         (void)hipStreamDestroy(ctx->aux);

@@ -46,6 +46,10 @@ struct sh_ctx {
     // FFT plans of the FK dip filter for the last size it ran on (owned;
     // the deleter destroys them)
     std::shared_ptr<void> fk;
+    // SH_FLAG_* of the public entry points (sh_ctx_set_flags)
+    int flags = 0;
+    // brackets of a dip call's device time (sh_dip_report.elapsed_ms)
+    hipEvent_t ev_t0 = nullptr, ev_t1 = nullptr;
 };
 
 hipEvent_t sh_prof_event(sh_ctx *ctx);
@@ -89,7 +93,8 @@ enum {
     SLOT_INTERP0 = 40,   // interpolation module: 40..51, warm-start network: 52..59
     SLOT_BATCH0 = 64,    // batched 2-D module: 64..79
     SLOT_FK0 = 80,       // FK dip filter: 80..81
-    SLOT_COUNT = 84
+    SLOT_IO0 = 84,       // host-pointer staging and the finite check: 84..99
+    SLOT_COUNT = 100
 };
 
 // ---------------------------------------------------------------------------
@@ -236,6 +241,9 @@ struct Pwsmooth2 {
     float *scr, *tr;    // per-thread solver scratch and running trace
     long nthr;
     float *F;           // LDL^T factors, two variants per trace
+    // (nullable) device flag: when it reads non-zero the spray kernels
+    // return at once (a solver that has stopped)
+    const int *stop = nullptr;
 };
 int pws2_setup(sh_ctx *ctx, Pwsmooth2 &P, int slot_base);
 int pws2_prepare(sh_ctx *ctx, Pwsmooth2 &P, int slot_scr, int slot_F);

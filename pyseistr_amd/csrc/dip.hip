@@ -709,7 +709,7 @@ int dip_direction(sh_ctx *ctx, const DipGeom &g, const SmoothPlan &plan, int axi
 
 }  // namespace
 
-extern "C" int sh_dip(sh_ctx *ctx, const float *data, const float *mask, const float *init, int n1, int n2,
+extern "C" int shd_dip(sh_ctx *ctx, const float *data, const float *mask, const float *init, int n1, int n2,
                       int n3, int niter, int liter, int order, float eps_dv, int r1, int r2, int r3,
                       float *dip, sh_dip_report *rep) {
     (void)eps_dv;

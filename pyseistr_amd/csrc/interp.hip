@@ -61,14 +61,21 @@ __global__ void taps_kernel(const float *__restrict__ pp, float *__restrict__ ta
 
 // y[part*n + i] = sum_iw (x[i+ip+is] - x[i-is]) a_iw(i)  (allpass3_lop fwd)
 // plus partial dots of y with ss and rr (for cgstep) and y.y.
+// The same launch also sums ss.ss, ss.rr and rr.rr (the solver's other
+// dots over the data space, over the same blocks), so part holds six
+// stacked partial arrays.  Returns at once when the solve has stopped.
+struct SolverState;
+__device__ inline int solver_done(const SolverState *st);
+
 template <int NW>
 __global__ void __launch_bounds__(TB)
 pwd_fwd(const float *__restrict__ x, const float *__restrict__ taps, float *__restrict__ y,
         const float *__restrict__ ss, const float *__restrict__ rr, Geo g, double *part, int nblk,
-        const float *base) {
+        const float *base, const SolverState *st) {
     __shared__ double sh[16];
+    if (st && solver_done(st)) return;
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    double d_gg = 0, d_gs = 0, d_gr = 0;
+    double d_gg = 0, d_gs = 0, d_gr = 0, d_ss = 0, d_sr = 0, d_rr = 0;
     long ny = g.n * g.nparts;
     if (i < ny) {
         int part_id = (int)(i / g.n);
@@ -94,16 +101,20 @@ pwd_fwd(const float *__restrict__ x, const float *__restrict__ taps, float *__re
             }
         }
         y[i] = v;
+        const float sv = ss[i], rv = rr[i];
         d_gg = (double)v * v;
-        d_gs = (double)v * ss[i];
-        d_gr = (double)v * rr[i];
+        d_gs = (double)v * sv;
+        d_gr = (double)v * rv;
+        d_ss = (double)sv * sv;
+        d_sr = (double)sv * rv;
+        d_rr = (double)rv * rv;
     }
-    double t0 = block_sum(d_gg, sh);
-    if (threadIdx.x == 0) part[blockIdx.x] = t0;
-    double t1 = block_sum(d_gs, sh);
-    if (threadIdx.x == 0) part[nblk + blockIdx.x] = t1;
-    double t2 = block_sum(d_gr, sh);
-    if (threadIdx.x == 0) part[2 * nblk + blockIdx.x] = t2;
+    const double d[6] = {d_gg, d_gs, d_gr, d_ss, d_sr, d_rr};
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+        const double t = block_sum(d[k], sh);
+        if (threadIdx.x == 0) part[(long)k * nblk + blockIdx.x] = t;
+    }
 }
 
 // g[k] = (A^T rr)[k], zero where known (allpass3_lop adj + ps_solver
@@ -113,8 +124,9 @@ pwd_fwd(const float *__restrict__ x, const float *__restrict__ taps, float *__re
 template <int NW>
 __global__ void __launch_bounds__(TB)
 pwd_adj(const float *__restrict__ rr, const float *__restrict__ taps, const unsigned char *__restrict__ known,
-        float *__restrict__ gout, Geo g, double *part) {
+        float *__restrict__ gout, Geo g, double *part, const SolverState *st) {
     __shared__ double sh[16];
+    if (st && solver_done(st)) return;
     long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
     double d = 0;
     if (k < g.n) {
@@ -197,24 +209,12 @@ __global__ void __launch_bounds__(TB) dot_kernel(const float *__restrict__ a, co
 }
 
 // cgstep update (soint3d_cfuns.c:864-877): S = b S + a g, x += S (nx);
-// Ss = b Ss + a gg, rr += Ss (ny)
+// Ss = b Ss + a gg, rr += Ss (ny).  Step sizes and flags come from the
+// solver state: nothing once it has stopped; S and Ss cleared on its first
+// step (the restart of ps_cgstep), also when no step follows.
 __global__ void cgstep_update(float *__restrict__ S, const float *__restrict__ gv, float *__restrict__ x, long nx,
                               float *__restrict__ Ss, const float *__restrict__ gg, float *__restrict__ rr, long ny,
-                              float a, float b) {
-    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < nx) {
-        float s = S[i] * b;
-        s += a * gv[i];
-        S[i] = s;
-        x[i] += s;
-    }
-    if (i < ny) {
-        float s = Ss[i] * b;
-        s += a * gg[i];
-        Ss[i] = s;
-        rr[i] += s;
-    }
-}
+                              const SolverState *st);
 
 __global__ void known_kernel(const float *__restrict__ src, unsigned char *__restrict__ known, long n) {
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -258,6 +258,207 @@ __global__ void fill_i(int *__restrict__ a, int v, long n) {
     if (i < n) a[i] = v;
 }
 
+// ---------------------------------------------------------------------------
+// Device-resident solver control.  The scalars of ps_cgstep / ps_solver and
+// of the shaping CG live in device memory: a one-block finalize kernel sums
+// the per-block partials in a fixed order (the order of
+// reduce_partials_kernel, so every sum is the one the host used to read
+// back), runs the scalar logic and publishes the step sizes and the stop
+// flag; the vector kernels read them from there.  No iteration waits for
+// the host, and every iteration enqueues the same launches, so one
+// iteration can be captured into a HIP graph and replayed.
+// ---------------------------------------------------------------------------
+struct SolverState {
+    double r0, g0;        // first residual / gradient norms
+    double gnp;           // previous gradient norm (shaping CG)
+    double a, b;          // step sizes (cast to float where used)
+    double gn;            // current gradient norm (shaping CG)
+    int it;               // iterations begun
+    int iters;            // iterations the reference's loop counter reaches
+    int started, done, step, zero, first;
+};
+
+__device__ inline int solver_done(const SolverState *st) { return *(volatile const int *)&st->done; }
+
+__global__ void cgstep_update(float *__restrict__ S, const float *__restrict__ gv, float *__restrict__ x, long nx,
+                              float *__restrict__ Ss, const float *__restrict__ gg, float *__restrict__ rr, long ny,
+                              const SolverState *st) {
+    if (st->done) return;
+    const int zero = st->zero, step = st->step;
+    const float a = (float)st->a, b = (float)st->b;
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nx) {
+        const float sv = zero ? 0.0f : S[i];
+        if (step) {
+            float s = sv * b;
+            s += a * gv[i];
+            S[i] = s;
+            x[i] += s;
+        } else if (zero) {
+            S[i] = 0.0f;
+        }
+    }
+    if (i < ny) {
+        const float sv = zero ? 0.0f : Ss[i];
+        if (step) {
+            float s = sv * b;
+            s += a * gg[i];
+            Ss[i] = s;
+            rr[i] += s;
+        } else if (zero) {
+            Ss[i] = 0.0f;
+        }
+    }
+}
+
+// Sum k of the stacked partials part[k*nblk ...] exactly as
+// reduce_partials_kernel does (1024 threads, strided, then block_sum);
+// the total is valid in thread 0.
+__device__ inline double partials_total(const double *part, int nblk, int k, double *sh) {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < nblk; i += blockDim.x) s += part[(size_t)k * nblk + i];
+    double t = block_sum(s, sh);
+    __syncthreads();
+    return t;
+}
+
+__global__ void __launch_bounds__(1024) solver_init(const double *part, int nblk, SolverState *st, int stop_if_zero) {
+    __shared__ double sh[16];
+    const double r0 = partials_total(part, nblk, 0, sh);
+    if (threadIdx.x == 0) {
+        SolverState z{};
+        z.r0 = r0;
+        z.done = (stop_if_zero && r0 == 0.) ? 1 : 0;
+        *st = z;
+    }
+}
+
+// ps_cgstep / ps_solver scalars (soint3d_cfuns.c:826-887, 1030-1090):
+// pa = |g|^2 partials (nbg blocks); pb = g.g, g.s, g.r, s.s, s.r, r.r
+// partials of the data-space vectors (nby blocks).  fnorm != 0: the two
+// norms that decide the early stop are rounded to float first
+// (ps_solver_prec).
+__global__ void __launch_bounds__(1024) cgstep_scalars(const double *pa, int nbg, const double *pb, int nby,
+                                                       SolverState *st, int fnorm) {
+    __shared__ double sh[16];
+    __shared__ int done;
+    if (threadIdx.x == 0) done = st->done;
+    __syncthreads();
+    if (done) return;
+    double v[7];
+    v[0] = partials_total(pa, nbg, 0, sh);
+    for (int k = 0; k < 6; k++) v[k + 1] = partials_total(pb, nby, k, sh);
+    if (threadIdx.x != 0) return;
+    const double gnorm = v[0], gdg = v[1], gds = v[2], gdr0 = v[3], sds = v[4], sdr0 = v[5], rnorm = v[6];
+    const double TOL = (double)1.e-12f, EPSILON = (double)1.e-12f;
+    const int it = st->it;
+    st->step = 0;
+    st->zero = 0;
+    double dpr, dpg;
+    if (fnorm) {
+        if (it == 0) {
+            st->r0 = (float)rnorm;
+            st->g0 = (float)gnorm;
+            dpr = dpg = 1.0;
+        } else {
+            dpr = (float)rnorm / st->r0;
+            dpg = (float)gnorm / st->g0;
+        }
+        if (dpr < TOL || dpg < TOL) {
+            st->done = 1;
+            st->iters = it;
+            return;
+        }
+    } else {
+        if (it == 0) {
+            st->g0 = gnorm;
+            dpr = dpg = 1.0;
+        } else {
+            dpr = rnorm / st->r0;
+            dpg = gnorm / st->g0;
+        }
+        if ((float)dpr < (float)TOL || (float)dpg < (float)TOL) {
+            st->done = 1;
+            st->iters = it;
+            return;
+        }
+    }
+    st->it = it + 1;
+    st->iters = it + 1;
+    const bool forget = !st->started;
+    st->started = 1;
+    if (forget) {
+        st->zero = 1;  // S = Ss = 0, also when no step follows
+        st->b = 0.0;
+        if (gdg <= 0.) return;
+        st->a = -gdr0 / gdg;
+        st->step = 1;
+        return;
+    }
+    if (gdg == 0. || sds == 0.) return;
+    double determ = 1.0 - (gds / gdg) * (gds / sds);
+    if (determ > EPSILON) determ *= gdg * sds;
+    else determ = gdg * sds * EPSILON;
+    const double gdr = -gdr0, sdr = -sdr0;
+    st->a = (sds * gdr - gds * sdr) / determ;
+    st->b = (-gds * gdr + gdg * sdr) / determ;
+    st->step = 1;
+}
+
+// Enqueues `body` (one solver iteration, launched on ctx->stream) niter
+// times.  By default the iteration is captured once into a HIP graph (on
+// the context's second stream, which only records) and the graph is
+// replayed niter times on the context's stream; SH_SOLVER_GRAPH=0 launches
+// the kernels of every iteration directly.  The launch timers are off
+// while capturing; the replays are timed as one entry.
+template <class Body>
+int run_iterations(sh_ctx *ctx, int niter, const char *name, double bytes_per_iter, Body body) {
+    if (niter <= 0) return SH_OK;
+    const char *ge = getenv("SH_SOLVER_GRAPH");
+    if (ge && ge[0] == '0') {
+        for (int it = 0; it < niter; it++) SH_TRY(body());
+        return SH_OK;
+    }
+    SH_TRY(sh_aux_stream(ctx));
+    hipStream_t main = ctx->stream;
+    const int prof = ctx->prof;
+    hipGraph_t graph = nullptr;
+    SH_HIP(ctx, hipStreamBeginCapture(ctx->aux, hipStreamCaptureModeThreadLocal));
+    ctx->stream = ctx->aux;
+    ctx->prof = 0;
+    const int rc = body();
+    ctx->stream = main;
+    ctx->prof = prof;
+    const hipError_t ec = hipStreamEndCapture(ctx->aux, &graph);
+    if (rc != SH_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    if (ec != hipSuccess) return sh_fail(ctx, SH_EHIP, "hipStreamEndCapture: %s", hipGetErrorString(ec));
+    hipGraphExec_t exec = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ei != hipSuccess) return sh_fail(ctx, SH_EHIP, "hipGraphInstantiate: %s", hipGetErrorString(ei));
+    PROF_A(ctx);
+    hipError_t el = hipSuccess;
+    for (int it = 0; it < niter && el == hipSuccess; it++) el = hipGraphLaunch(exec, main);
+    PROF_B(ctx, name, bytes_per_iter * niter);
+    // the replays still queued keep their own reference to the executable
+    // graph; it is released after the last one has run
+    const hipError_t ed = hipStreamSynchronize(main);
+    (void)hipGraphExecDestroy(exec);
+    if (el != hipSuccess) return sh_fail(ctx, SH_EHIP, "hipGraphLaunch: %s", hipGetErrorString(el));
+    if (ed != hipSuccess) return sh_fail(ctx, SH_EHIP, "hipStreamSynchronize: %s", hipGetErrorString(ed));
+    return SH_OK;
+}
+
+int read_iters(sh_ctx *ctx, const SolverState *st, int *iters) {
+    if (!iters) return SH_OK;
+    SH_HIP(ctx, hipMemcpyAsync(iters, &st->iters, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    SH_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return SH_OK;
+}
+
 // ps_solver + ps_cgstep with the "known" and "x0" options
 template <int NW>
 int soint_solve(sh_ctx *ctx, const Geo &g0, const float *pp, const float *qq, const unsigned char *known, float *x,
@@ -271,9 +472,10 @@ int soint_solve(sh_ctx *ctx, const Geo &g0, const float *pp, const float *qq, co
     SH_TRY(sh_slot(ctx, SLOT_INTERP0, nf * sizeof(float), (void **)&buf));
     float *taps = buf, *gv = taps + (size_t)g.nparts * NA * n, *S = gv + n, *rr = S + n, *gg = rr + ny, *Ss = gg + ny;
     int nbn = ceil_div(n, TB), nby = ceil_div(ny, TB);
-    double *part = nullptr, *dsum = nullptr;
-    SH_TRY(sh_slot(ctx, SLOT_RED, (size_t)nby * 8 * sizeof(double), (void **)&part));
-    SH_TRY(sh_slot(ctx, SLOT_SCAL, 16 * sizeof(double), (void **)&dsum));
+    double *part = nullptr;
+    SH_TRY(sh_slot(ctx, SLOT_RED, (size_t)(6 * nby + nbn) * sizeof(double), (void **)&part));
+    SolverState *st = nullptr;
+    SH_TRY(sh_slot(ctx, SLOT_SCAL, sizeof(SolverState), (void **)&st));
     int *ids = nullptr;
     if (g.drift) {
         // NINT(p) per part and sample, then the largest shift (one sync)
@@ -298,65 +500,40 @@ int soint_solve(sh_ctx *ctx, const Geo &g0, const float *pp, const float *qq, co
     }
     hipLaunchKernelGGL(zero_f, dim3(nbn), dim3(TB), 0, ctx->stream, S, n);
     hipLaunchKernelGGL(zero_f, dim3(nby), dim3(TB), 0, ctx->stream, Ss, ny);
-    // rr = -dd, then rr += A x0 tap by tap (ps_solver 1018-1026)
+    // rr = -dd, then rr += A x0 tap by tap (ps_solver 1018-1026); its
+    // partial rr.rr starts the solver state
     hipLaunchKernelGGL(neg_kernel, dim3(nby), dim3(TB), 0, ctx->stream, dd, rr, ny);
-    hipLaunchKernelGGL((pwd_fwd<NW>), dim3(nby), dim3(TB), 0, ctx->stream, x, taps, rr, Ss, Ss, g, part, nby, rr);
+    hipLaunchKernelGGL((pwd_fwd<NW>), dim3(nby), dim3(TB), 0, ctx->stream, x, taps, rr, Ss, Ss, g, part, nby, rr,
+                       nullptr);
+    hipLaunchKernelGGL(solver_init, dim3(1), dim3(1024), 0, ctx->stream, part, nby, st, 0);
     SH_TRY(sh_launch_check(ctx, "soint init"));
-    double h[8];
-    hipLaunchKernelGGL(dot_kernel, dim3(nby), dim3(TB), 0, ctx->stream, rr, rr, ny, part, 0, nby);
-    SH_TRY(sum_partials(ctx, part, nby, 1, dsum, h));
-    double dpr0 = h[0], dpg0 = 1.0;
-    bool started = false;
-    int it;
-    const double TOL = (double)1.e-12f, EPSILON = (double)1.e-12f;
-    for (it = 0; it < niter; it++) {
-        hipLaunchKernelGGL((pwd_adj<NW>), dim3(nbn), dim3(TB), 0, ctx->stream, rr, taps, known, gv, g, part);
-        SH_TRY(sh_launch_check(ctx, "pwd_adj"));
-        SH_TRY(sum_partials(ctx, part, nbn, 1, dsum, h));
-        double gnorm = h[0];
-        hipLaunchKernelGGL((pwd_fwd<NW>), dim3(nby), dim3(TB), 0, ctx->stream, gv, taps, gg, Ss, rr, g, part, nby,
-                           nullptr);
-        hipLaunchKernelGGL(dot_kernel, dim3(nby), dim3(TB), 0, ctx->stream, Ss, Ss, ny, part, 3, nby);
-        hipLaunchKernelGGL(dot_kernel, dim3(nby), dim3(TB), 0, ctx->stream, Ss, rr, ny, part, 4, nby);
-        hipLaunchKernelGGL(dot_kernel, dim3(nby), dim3(TB), 0, ctx->stream, rr, rr, ny, part, 5, nby);
-        SH_TRY(sh_launch_check(ctx, "pwd_fwd"));
-        SH_TRY(sum_partials(ctx, part, nby, 6, dsum, h));
-        double gdg = h[0], gds = h[1], gdr0 = h[2], sds = h[3], sdr0 = h[4], rnorm = h[5];
-        double dpr, dpg;
-        if (it == 0) {
-            dpg0 = gnorm;
-            dpr = 1.0;
-            dpg = 1.0;
-        } else {
-            dpr = rnorm / dpr0;
-            dpg = gnorm / dpg0;
+    double *pa = part + (size_t)6 * nby;  // |g|^2 partials of pwd_adj
+    // one iteration: g = A^T r (zero where known), A g with the six dots,
+    // the cgstep scalars, the update (ps_solver 1030-1100, ps_cgstep)
+    // (algorithmic bytes: the operator's input, slopes and output; the
+    // dots' extra vectors; the update's vectors read and written)
+    auto iteration = [&]() -> int {
+        {
+            PROF_A(ctx);
+            hipLaunchKernelGGL((pwd_adj<NW>), dim3(nbn), dim3(TB), 0, ctx->stream, rr, taps, known, gv, g, pa, st);
+            PROF_B(ctx, "pwd_adj", 4.0 * (ny + g.nparts * n + n));
         }
-        if ((float)dpr < (float)TOL || (float)dpg < (float)TOL) break;
-        double alfa, beta;
-        bool forget = !started;
-        started = true;
-        if (forget) {
-            hipLaunchKernelGGL(zero_f, dim3(nbn), dim3(TB), 0, ctx->stream, S, n);
-            hipLaunchKernelGGL(zero_f, dim3(nby), dim3(TB), 0, ctx->stream, Ss, ny);
-            beta = 0.0;
-            alfa = gdg;
-            if (alfa <= 0.) continue;
-            alfa = -gdr0 / alfa;
-        } else {
-            if (gdg == 0. || sds == 0.) continue;
-            double determ = 1.0 - (gds / gdg) * (gds / sds);
-            if (determ > EPSILON) determ *= gdg * sds;
-            else determ = gdg * sds * EPSILON;
-            double gdr = -gdr0, sdr = -sdr0;
-            alfa = (sds * gdr - gds * sdr) / determ;
-            beta = (-gds * gdr + gdg * sdr) / determ;
+        {
+            PROF_A(ctx);
+            hipLaunchKernelGGL((pwd_fwd<NW>), dim3(nby), dim3(TB), 0, ctx->stream, gv, taps, gg, Ss, rr, g, part, nby,
+                               nullptr, st);
+            PROF_B(ctx, "pwd_fwd", 4.0 * (n + g.nparts * n + 3 * ny));
         }
-        hipLaunchKernelGGL(cgstep_update, dim3(nby), dim3(TB), 0, ctx->stream, S, gv, x, n, Ss, gg, rr, ny,
-                           (float)alfa, (float)beta);
-        SH_TRY(sh_launch_check(ctx, "cgstep_update"));
-    }
-    if (iters_out) *iters_out = it;
-    return SH_OK;
+        hipLaunchKernelGGL(cgstep_scalars, dim3(1), dim3(1024), 0, ctx->stream, pa, nbn, part, nby, st, 0);
+        PROF_A(ctx);
+        hipLaunchKernelGGL(cgstep_update, dim3(nby), dim3(TB), 0, ctx->stream, S, gv, x, n, Ss, gg, rr, ny, st);
+        PROF_B(ctx, "cgstep_update", 4.0 * (5 * n + 5 * ny));
+        return sh_launch_check(ctx, "soint iteration");
+    };
+    // algorithmic bytes per iteration: r, g, A g (2N each side) and the
+    // slopes of both applications, the model and residual updates
+    SH_TRY(run_iterations(ctx, niter, "soint_iteration", 4.0 * (double)(4 * ny + 4 * n), iteration));
+    return read_iters(ctx, st, iters_out);
 }
 
 // ---------------------------------------------------------------------------
@@ -382,6 +559,46 @@ __global__ void permute_xi(const float *__restrict__ src, float *__restrict__ ds
     }
 }
 
+// The same permutation through 32 x 32 LDS tiles, one k-plane per tile
+// row of blocks: src [a][k][b] -> dst [b][k][a] (A, B the extents of a and
+// b); reads run along b and writes along a, both coalesced.  inv = 0 is
+// a = i3, b = i2; inv = 1 is a = i2, b = i3.
+__global__ void __launch_bounds__(256) permute_xi_tiled(const float *__restrict__ src, float *__restrict__ dst,
+                                                         int n1, int A, int B, int tilesB) {
+    __shared__ float tile[32][33];
+    const int tb = blockIdx.x % tilesB;
+    const int k = blockIdx.x / tilesB;
+    const int b0 = tb * 32, a0 = blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+    for (int j = ty; j < 32; j += 8) {
+        const int a = a0 + j, b = b0 + tx;
+        if (a < A && b < B) tile[j][tx] = src[((long)a * n1 + k) * B + b];
+    }
+    __syncthreads();
+    for (int j = ty; j < 32; j += 8) {
+        const int b = b0 + j, a = a0 + tx;
+        if (a < A && b < B) dst[((long)b * n1 + k) * A + a] = tile[tx][j];
+    }
+}
+
+int permute(sh_ctx *ctx, const float *src, float *dst, int n1, int n2, int n3, int inv) {
+    const char *pe = getenv("SH_PERMUTE_PLAIN");
+    if (pe && pe[0] == '1') {
+        const long n = (long)n1 * n2 * n3;
+        hipLaunchKernelGGL(permute_xi, dim3(ceil_div(n, TB)), dim3(TB), 0, ctx->stream, src, dst, n1, n2, n3, inv);
+        return sh_launch_check(ctx, "permute_xi");
+    }
+    const int A = inv ? n2 : n3, B = inv ? n3 : n2;
+    const int tilesB = ceil_div(B, 32);
+    if ((long)tilesB * n1 > 0x7fffffffL || ceil_div(A, 32) > 65535)
+        return sh_fail(ctx, SH_EINVAL, "permute: %d x %d x %d too large", n1, n2, n3);
+    PROF_A(ctx);
+    hipLaunchKernelGGL(permute_xi_tiled, dim3((unsigned)(tilesB * n1), ceil_div(A, 32)), dim3(256), 0, ctx->stream, src,
+                       dst, n1, A, B, tilesB);
+    PROF_B(ctx, "permute_xi_tiled", 8.0 * (double)n1 * n2 * n3);
+    return sh_launch_check(ctx, "permute_xi_tiled");
+}
+
 struct Shape3 {
     Pwsmooth2 pin, px;  // inline slices (n1 x n2, n3 of them), crossline (n1 x n3, n2 of them)
     bool three;
@@ -392,26 +609,24 @@ struct Shape3 {
 
 // x = S p  (add = 0)  or  p += S^T x  (adjoint, add = 1); pwsmooth3_lop
 int shape_apply(sh_ctx *ctx, Shape3 &Sh, int adj, float *p, float *x) {
-    int g = ceil_div(Sh.n, TB);
     if (!Sh.three) return pws2_apply(ctx, Sh.pin, adj, adj ? 1 : 0, p, x);
     if (!adj) {
         SH_TRY(pws2_apply(ctx, Sh.pin, 0, 0, p, Sh.t1));  // itmp
-        hipLaunchKernelGGL(permute_xi, dim3(g), dim3(TB), 0, ctx->stream, Sh.t1, Sh.t2, Sh.n1, Sh.n2, Sh.n3, 0);
+        SH_TRY(permute(ctx, Sh.t1, Sh.t2, Sh.n1, Sh.n2, Sh.n3, 0));
         SH_TRY(pws2_apply(ctx, Sh.px, 0, 0, Sh.t2, Sh.t1));  // xtmp
-        hipLaunchKernelGGL(permute_xi, dim3(g), dim3(TB), 0, ctx->stream, Sh.t1, x, Sh.n1, Sh.n2, Sh.n3, 1);
-        return sh_launch_check(ctx, "permute");
+        return permute(ctx, Sh.t1, x, Sh.n1, Sh.n2, Sh.n3, 1);
     }
-    hipLaunchKernelGGL(permute_xi, dim3(g), dim3(TB), 0, ctx->stream, x, Sh.t1, Sh.n1, Sh.n2, Sh.n3, 0);  // xtmp
+    SH_TRY(permute(ctx, x, Sh.t1, Sh.n1, Sh.n2, Sh.n3, 0));  // xtmp
     SH_TRY(pws2_apply(ctx, Sh.px, 1, 0, Sh.t2, Sh.t1));  // itmp2 = Sx^T xtmp
-    hipLaunchKernelGGL(permute_xi, dim3(g), dim3(TB), 0, ctx->stream, Sh.t2, Sh.t1, Sh.n1, Sh.n2, Sh.n3, 1);
-    SH_TRY(sh_launch_check(ctx, "permute"));
+    SH_TRY(permute(ctx, Sh.t2, Sh.t1, Sh.n1, Sh.n2, Sh.n3, 1));
     return pws2_apply(ctx, Sh.pin, 1, 1, p, Sh.t1);  // trace += Si^T itmp
 }
 
 // gp = eps p; gx = -eps x + mask r   (ps_conjgrad 306-316 with ps_mask_lop)
 __global__ void sint_pre(const float *__restrict__ p, const float *__restrict__ x, const float *__restrict__ r,
                          const unsigned char *__restrict__ m, float *__restrict__ gp, float *__restrict__ gx, float eps,
-                         long n) {
+                         long n, const SolverState *st) {
+    if (solver_done(st)) return;
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     gp[i] = eps * p[i];
@@ -423,8 +638,9 @@ __global__ void sint_pre(const float *__restrict__ p, const float *__restrict__ 
 // gr = mask gx; partial gp.gp
 __global__ void __launch_bounds__(TB) sint_post(const float *__restrict__ gp, const float *__restrict__ gx,
                                                 const unsigned char *__restrict__ m, float *__restrict__ gr, long n,
-                                                double *part) {
+                                                double *part, const SolverState *st) {
     __shared__ double sh[16];
+    if (solver_done(st)) return;
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     double d = 0;
     if (i < n) {
@@ -435,11 +651,43 @@ __global__ void __launch_bounds__(TB) sint_post(const float *__restrict__ gp, co
     if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
 
-// s = g (first) or s = g + a s; partial sr.sr, sp.sp, sx.sx
+// ps_conjgrad's gradient norm and stopping test (soint3d_cfuns.c:1290-1305):
+// gn = gp.gp; first iteration g0 = gn, later alpha = gn / gnp, stop when
+// alpha or gn / g0 falls below tol.
+__global__ void __launch_bounds__(1024) sint_scalars_g(const double *part, int nblk, SolverState *st, float tol) {
+    __shared__ double sh[16];
+    __shared__ int done;
+    if (threadIdx.x == 0) done = st->done;
+    __syncthreads();
+    if (done) return;
+    const double gn = partials_total(part, nblk, 0, sh);
+    if (threadIdx.x != 0) return;
+    const int it = st->it;
+    st->gn = gn;
+    st->first = it == 0;
+    if (it == 0) {
+        st->g0 = gn;
+        st->a = 0.0;
+    } else {
+        const double alpha = gn / st->gnp;
+        const double dg = gn / st->g0;
+        if (alpha < (double)tol || dg < (double)tol) {
+            st->done = 1;
+            return;
+        }
+        st->a = alpha;
+    }
+}
+
+// s = g (first) or s = g + alpha s; partial sr.sr, sp.sp, sx.sx
 __global__ void __launch_bounds__(TB) sint_dir(const float *__restrict__ gp, const float *__restrict__ gx,
                                                const float *__restrict__ gr, float *__restrict__ sp, float *__restrict__ sx,
-                                               float *__restrict__ sr, float a, int first, long n, double *part, int nblk) {
+                                               float *__restrict__ sr, long n, double *part, int nblk,
+                                               const SolverState *st) {
     __shared__ double sh[16];
+    if (solver_done(st)) return;
+    const int first = st->first;
+    const float a = (float)st->a;
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     double d0 = 0, d1 = 0, d2 = 0;
     if (i < n) {
@@ -462,9 +710,29 @@ __global__ void __launch_bounds__(TB) sint_dir(const float *__restrict__ gp, con
     if (threadIdx.x == 0) part[2 * nblk + blockIdx.x] = t;
 }
 
+// beta = sr.sr + eps (sp.sp - sx.sx), step -gn / beta (ps_conjgrad
+// 1318-1330)
+__global__ void __launch_bounds__(1024) sint_scalars_s(const double *part, int nblk, SolverState *st, float eps) {
+    __shared__ double sh[16];
+    __shared__ int done;
+    if (threadIdx.x == 0) done = st->done;
+    __syncthreads();
+    if (done) return;
+    double h[3];
+    for (int k = 0; k < 3; k++) h[k] = partials_total(part, nblk, k, sh);
+    if (threadIdx.x != 0) return;
+    const double beta = h[0] + (double)eps * (h[1] - h[2]);
+    st->b = -st->gn / beta;
+    st->gnp = st->gn;
+    st->it = st->it + 1;
+    st->iters = st->it;
+}
+
 __global__ void sint_update(float *__restrict__ p, float *__restrict__ x, float *__restrict__ r,
                             const float *__restrict__ sp, const float *__restrict__ sx, const float *__restrict__ sr,
-                            float a, long n) {
+                            long n, const SolverState *st) {
+    if (solver_done(st)) return;
+    const float a = (float)st->b;
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     p[i] += a * sp[i];
@@ -482,7 +750,9 @@ __global__ void sint_r0(const float *__restrict__ dat, const float *__restrict__
     r[i] = v;
 }
 
-// The shaping CG itself (p = model, x = estimated data, dat = data)
+// The shaping CG itself (p = model, x = estimated data, dat = data),
+// ps_conjgrad with hasp0 (soint3d_cfuns.c:1222-1349): every scalar stays on
+// the device (SolverState), so the iterations are queued without waiting.
 int sint_cg(sh_ctx *ctx, Shape3 &Sh, const unsigned char *known, float *p, float *x, const float *dat, float eps1,
             float tol, int niter, int *iters) {
     long n = Sh.n;
@@ -490,49 +760,51 @@ int sint_cg(sh_ctx *ctx, Shape3 &Sh, const unsigned char *known, float *p, float
     float *buf = nullptr;
     SH_TRY(sh_slot(ctx, SLOT_INTERP0 + 2, (size_t)n * 7 * sizeof(float), (void **)&buf));
     float *r = buf, *sp = r + n, *gp = sp + n, *sx = gp + n, *gx = sx + n, *sr = gx + n, *gr = sr + n;
-    double *part = nullptr, *dsum = nullptr;
+    double *part = nullptr;
     SH_TRY(sh_slot(ctx, SLOT_RED, (size_t)nb * 4 * sizeof(double), (void **)&part));
-    SH_TRY(sh_slot(ctx, SLOT_SCAL, 16 * sizeof(double), (void **)&dsum));
+    SolverState *st = nullptr;
+    SH_TRY(sh_slot(ctx, SLOT_SCAL, sizeof(SolverState), (void **)&st));
     float eps = eps1 * eps1;
     SH_TRY(shape_apply(ctx, Sh, 0, p, x));  // x = S p
     hipLaunchKernelGGL(sint_r0, dim3(nb), dim3(TB), 0, ctx->stream, dat, x, known, r, n);
     hipLaunchKernelGGL(dot_kernel, dim3(nb), dim3(TB), 0, ctx->stream, r, r, n, part, 0, nb);
+    // a zero initial residual ends the solve before its first iteration
+    hipLaunchKernelGGL(solver_init, dim3(1), dim3(1024), 0, ctx->stream, part, nb, st, 1);
     SH_TRY(sh_launch_check(ctx, "sint init"));
-    double h[4];
-    SH_TRY(sum_partials(ctx, part, nb, 1, dsum, h));
-    int done_iters = 0;
-    if (h[0] == 0.) {
-        if (iters) *iters = 0;
-        return SH_OK;
-    }
-    double gn, gnp = 0, g0 = 0, alpha = 0;
-    for (int it = 0; it < niter; it++) {
-        hipLaunchKernelGGL(sint_pre, dim3(nb), dim3(TB), 0, ctx->stream, p, x, r, known, gp, gx, eps, n);
+    // the smoothing sprays of a finished solve return at once
+    Sh.pin.stop = &st->done;
+    Sh.px.stop = &st->done;
+    auto iteration = [&]() -> int {
+        {
+            PROF_A(ctx);
+            hipLaunchKernelGGL(sint_pre, dim3(nb), dim3(TB), 0, ctx->stream, p, x, r, known, gp, gx, eps, n, st);
+            PROF_B(ctx, "sint_pre", 4.0 * 6 * (double)n);
+        }
         SH_TRY(sh_launch_check(ctx, "sint_pre"));
         SH_TRY(shape_apply(ctx, Sh, 1, gp, gx));  // gp += S^T gx
         SH_TRY(shape_apply(ctx, Sh, 0, gp, gx));  // gx = S gp
-        hipLaunchKernelGGL(sint_post, dim3(nb), dim3(TB), 0, ctx->stream, gp, gx, known, gr, n, part);
-        SH_TRY(sum_partials(ctx, part, nb, 1, dsum, h));
-        gn = h[0];
-        if (it == 0) {
-            g0 = gn;
-        } else {
-            alpha = gn / gnp;
-            double dg = gn / g0;
-            if (alpha < (double)tol || dg < (double)tol) break;
+        {
+            PROF_A(ctx);
+            hipLaunchKernelGGL(sint_post, dim3(nb), dim3(TB), 0, ctx->stream, gp, gx, known, gr, n, part, st);
+            PROF_B(ctx, "sint_post", 4.0 * 4 * (double)n);
         }
-        hipLaunchKernelGGL(sint_dir, dim3(nb), dim3(TB), 0, ctx->stream, gp, gx, gr, sp, sx, sr, (float)alpha,
-                           it == 0, n, part, nb);
-        SH_TRY(sum_partials(ctx, part, nb, 3, dsum, h));
-        double beta = h[0] + (double)eps * (h[1] - h[2]);
-        double a2 = -gn / beta;
-        hipLaunchKernelGGL(sint_update, dim3(nb), dim3(TB), 0, ctx->stream, p, x, r, sp, sx, sr, (float)a2, n);
-        SH_TRY(sh_launch_check(ctx, "sint_update"));
-        gnp = gn;
-        done_iters = it + 1;
-    }
-    if (iters) *iters = done_iters;
-    return SH_OK;
+        hipLaunchKernelGGL(sint_scalars_g, dim3(1), dim3(1024), 0, ctx->stream, part, nb, st, tol);
+        {
+            PROF_A(ctx);
+            hipLaunchKernelGGL(sint_dir, dim3(nb), dim3(TB), 0, ctx->stream, gp, gx, gr, sp, sx, sr, n, part, nb, st);
+            PROF_B(ctx, "sint_dir", 4.0 * 9 * (double)n);
+        }
+        hipLaunchKernelGGL(sint_scalars_s, dim3(1), dim3(1024), 0, ctx->stream, part, nb, st, eps);
+        PROF_A(ctx);
+        hipLaunchKernelGGL(sint_update, dim3(nb), dim3(TB), 0, ctx->stream, p, x, r, sp, sx, sr, n, st);
+        PROF_B(ctx, "sint_update", 4.0 * 9 * (double)n);
+        return sh_launch_check(ctx, "sint iteration");
+    };
+    const int rc = run_iterations(ctx, niter, "sint_iteration", 0.0, iteration);
+    Sh.pin.stop = nullptr;
+    Sh.px.stop = nullptr;
+    SH_TRY(rc);
+    return read_iters(ctx, st, iters);
 }
 
 // ---------------------------------------------------------------------------
@@ -565,8 +837,10 @@ __global__ void prec_factor(const float *__restrict__ dip, float *__restrict__ F
 // ys are one trace each of scratch.  One thread.
 template <int NW>
 __global__ void prec_chain(int adj, float *__restrict__ x, float *__restrict__ y, const float *__restrict__ dip,
-                           const float *__restrict__ F, float *tt, float *ys, int n1, int n2, float e) {
+                           const float *__restrict__ F, float *tt, float *ys, int n1, int n2, float e,
+                           const SolverState *st) {
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    if (st && solver_done(st)) return;
     constexpr int NB = 2 * NW;
     for (int i1 = 0; i1 < n1; i1++) tt[i1] = 0.0f;
     bool rev[1] = {true};
@@ -597,8 +871,9 @@ __global__ void prec_chain(int adj, float *__restrict__ x, float *__restrict__ y
 template <int NW>
 __global__ void __launch_bounds__(256)
 prec_chain_lds(int adj, float *__restrict__ x, float *__restrict__ y, const float *__restrict__ dip,
-               const float *__restrict__ F, int n1, int n2, float e) {
+               const float *__restrict__ F, int n1, int n2, float e, const SolverState *st) {
     extern __shared__ float sm[];
+    if (st && solver_done(st)) return;
     constexpr int NB = 2 * NW;
     float *Fs = sm, *ds = Fs + (NB + 1) * n1, *tt = ds + n1, *ys = tt + n1;
     const int t = threadIdx.x, T = blockDim.x;
@@ -696,8 +971,9 @@ __device__ void subst_factored(const float *src, const float *Fs, float *y, int 
 template <int NW>
 __global__ void __launch_bounds__(256)
 prec_chain_par(int adj, float *__restrict__ x, float *__restrict__ y, const float *__restrict__ dip,
-               const float *__restrict__ F, int n1, int n2, float e) {
+               const float *__restrict__ F, int n1, int n2, float e, const SolverState *st) {
     extern __shared__ float sm[];
+    if (st && solver_done(st)) return;
     constexpr int NB = 2 * NW, NA = 2 * NW + 1;
     float *Fs = sm, *tp = Fs + (NB + 1) * n1, *tt = tp + NA * n1, *ys = tt + n1, *tm = ys + n1, *rh = tm + n1;
     const int t = threadIdx.x, T = blockDim.x;
@@ -789,28 +1065,59 @@ __global__ void mask_adj(const unsigned char *__restrict__ known, const float *_
     if (i < n) xt[i] = known[i] ? 0.0f + rr[i] : 0.0f;
 }
 
-// gg = M xt, then gg += eps * (eps * rr) with eps = 0 (the solver's
-// regularisation rows, which only touch the sign of zero)
-__global__ void mask_fwd_gg(const unsigned char *__restrict__ known, const float *__restrict__ xt,
-                            const float *__restrict__ rr, float *__restrict__ gg, long n) {
+__global__ void mask_adj_st(const unsigned char *__restrict__ known, const float *__restrict__ rr,
+                            float *__restrict__ xt, long n, const SolverState *st) {
+    if (solver_done(st)) return;
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    float v = known[i] ? 0.0f + xt[i] : 0.0f;
-    const float g2 = 0.0f * rr[i];
-    gg[i] = v + 0.0f * g2;
+    if (i < n) xt[i] = known[i] ? 0.0f + rr[i] : 0.0f;
+}
+
+// gg = M xt, then gg += eps * (eps * rr) with eps = 0 (the solver's
+// regularisation rows, which only touch the sign of zero); partial sums of
+// gg.gg, gg.Ss, gg.rr, Ss.Ss, Ss.rr, rr.rr (part[0..5]) and gv.gv
+// (part[6]), each over the same blocks as a separate dot would use.
+__global__ void __launch_bounds__(TB)
+mask_fwd_gg_dots(const unsigned char *__restrict__ known, const float *__restrict__ xt, const float *__restrict__ rr,
+                 float *__restrict__ gg, const float *__restrict__ Ss, const float *__restrict__ gv, long n,
+                 double *part, int nblk, const SolverState *st) {
+    __shared__ double sh[16];
+    if (solver_done(st)) return;
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double d[7] = {0, 0, 0, 0, 0, 0, 0};
+    if (i < n) {
+        float v = known[i] ? 0.0f + xt[i] : 0.0f;
+        const float r = rr[i], sv = Ss[i], g = gv[i];
+        const float g2 = 0.0f * r;
+        v = v + 0.0f * g2;
+        gg[i] = v;
+        d[0] = (double)v * v;
+        d[1] = (double)v * sv;
+        d[2] = (double)v * r;
+        d[3] = (double)sv * sv;
+        d[4] = (double)sv * r;
+        d[5] = (double)r * r;
+        d[6] = (double)g * g;
+    }
+#pragma unroll
+    for (int k = 0; k < 7; k++) {
+        const double t = block_sum(d[k], sh);
+        if (threadIdx.x == 0) part[(long)k * nblk + blockIdx.x] = t;
+    }
 }
 
 // P (adj = 0: y = P x) or P^T (adj = 1: x = P^T y) for one or two slope
 // planes (predict2_lop: P = P(q) P(p))
 template <int NW>
 int prec_apply(sh_ctx *ctx, int adj, float *x, float *y, const float *pp, const float *qq, const float *Fp,
-                      const float *Fq, float *tmp, float *tt, float *ys, int n1, int n2, float e) {
+                      const float *Fq, float *tmp, float *tt, float *ys, int n1, int n2, float e,
+                      const SolverState *st) {
     constexpr int NB = 2 * NW;
     const size_t ldp = (size_t)(2 * NB + 6) * n1 * sizeof(float);
     const char *mode = getenv("SH_SOINT_PREC_CHAIN");  // "lds" or "global": the slower kernels
     if (ldp <= 65536 && !mode) {
         auto run = [&](int a, float *xx, float *yy, const float *dp, const float *Fd) {
-            hipLaunchKernelGGL((prec_chain_par<NW>), dim3(1), dim3(256), ldp, ctx->stream, a, xx, yy, dp, Fd, n1, n2, e);
+            hipLaunchKernelGGL((prec_chain_par<NW>), dim3(1), dim3(256), ldp, ctx->stream, a, xx, yy, dp, Fd, n1, n2, e,
+                               st);
         };
         if (!qq) run(adj, x, y, pp, Fp);
         else if (adj) {
@@ -825,7 +1132,8 @@ int prec_apply(sh_ctx *ctx, int adj, float *x, float *y, const float *pp, const 
     const size_t lds = (size_t)(NB + 4) * n1 * sizeof(float);
     if (lds <= 65536 && !(mode && !strcmp(mode, "global"))) {
         auto run = [&](int a, float *xx, float *yy, const float *dp, const float *Fd) {
-            hipLaunchKernelGGL((prec_chain_lds<NW>), dim3(1), dim3(256), lds, ctx->stream, a, xx, yy, dp, Fd, n1, n2, e);
+            hipLaunchKernelGGL((prec_chain_lds<NW>), dim3(1), dim3(256), lds, ctx->stream, a, xx, yy, dp, Fd, n1, n2, e,
+                               st);
         };
         if (!qq) run(adj, x, y, pp, Fp);
         else if (adj) {
@@ -838,15 +1146,20 @@ int prec_apply(sh_ctx *ctx, int adj, float *x, float *y, const float *pp, const 
         return sh_launch_check(ctx, "prec_chain_lds");
     }
     if (!qq) {
-        hipLaunchKernelGGL((prec_chain<NW>), dim3(1), dim3(64), 0, ctx->stream, adj, x, y, pp, Fp, tt, ys, n1, n2, e);
+        hipLaunchKernelGGL((prec_chain<NW>), dim3(1), dim3(64), 0, ctx->stream, adj, x, y, pp, Fp, tt, ys, n1, n2, e,
+                           st);
         return sh_launch_check(ctx, "prec_chain");
     }
     if (adj) {
-        hipLaunchKernelGGL((prec_chain<NW>), dim3(1), dim3(64), 0, ctx->stream, 1, tmp, y, qq, Fq, tt, ys, n1, n2, e);
-        hipLaunchKernelGGL((prec_chain<NW>), dim3(1), dim3(64), 0, ctx->stream, 1, x, tmp, pp, Fp, tt, ys, n1, n2, e);
+        hipLaunchKernelGGL((prec_chain<NW>), dim3(1), dim3(64), 0, ctx->stream, 1, tmp, y, qq, Fq, tt, ys, n1, n2, e,
+                           st);
+        hipLaunchKernelGGL((prec_chain<NW>), dim3(1), dim3(64), 0, ctx->stream, 1, x, tmp, pp, Fp, tt, ys, n1, n2, e,
+                           st);
     } else {
-        hipLaunchKernelGGL((prec_chain<NW>), dim3(1), dim3(64), 0, ctx->stream, 0, x, tmp, pp, Fp, tt, ys, n1, n2, e);
-        hipLaunchKernelGGL((prec_chain<NW>), dim3(1), dim3(64), 0, ctx->stream, 0, tmp, y, qq, Fq, tt, ys, n1, n2, e);
+        hipLaunchKernelGGL((prec_chain<NW>), dim3(1), dim3(64), 0, ctx->stream, 0, x, tmp, pp, Fp, tt, ys, n1, n2, e,
+                           st);
+        hipLaunchKernelGGL((prec_chain<NW>), dim3(1), dim3(64), 0, ctx->stream, 0, tmp, y, qq, Fq, tt, ys, n1, n2, e,
+                           st);
     }
     return sh_launch_check(ctx, "prec_chain");
 }
@@ -868,9 +1181,10 @@ int soint2d_prec_solve(sh_ctx *ctx, const float *din, const unsigned char *known
     float *Fp = buf, *Fq = qq ? Fp + nF : nullptr;
     float *p = buf + nF * (qq ? 2 : 1), *gv = p + n, *S = gv + n, *rr = S + n, *gg = rr + n, *Ss = gg + n;
     float *xt = Ss + n, *tmp = xt + n, *tt = tmp + n, *ys = tt + n1;
-    double *part = nullptr, *dsum = nullptr;
-    SH_TRY(sh_slot(ctx, SLOT_RED, (size_t)nb * 8 * sizeof(double), (void **)&part));
-    SH_TRY(sh_slot(ctx, SLOT_SCAL, 16 * sizeof(double), (void **)&dsum));
+    double *part = nullptr;
+    SH_TRY(sh_slot(ctx, SLOT_RED, (size_t)nb * 7 * sizeof(double), (void **)&part));
+    SolverState *st = nullptr;
+    SH_TRY(sh_slot(ctx, SLOT_SCAL, sizeof(SolverState), (void **)&st));
     hipLaunchKernelGGL((prec_factor<NW>), dim3(ceil_div(n2, 64)), dim3(64), 0, ctx->stream, pp, Fp, n1, n2, e);
     if (qq) hipLaunchKernelGGL((prec_factor<NW>), dim3(ceil_div(n2, 64)), dim3(64), 0, ctx->stream, qq, Fq, n1, n2, e);
     hipLaunchKernelGGL(zero_f, dim3(nb), dim3(TB), 0, ctx->stream, p, n);
@@ -878,71 +1192,27 @@ int soint2d_prec_solve(sh_ctx *ctx, const float *din, const unsigned char *known
     hipLaunchKernelGGL(zero_f, dim3(nb), dim3(TB), 0, ctx->stream, Ss, n);
     hipLaunchKernelGGL(neg_kernel, dim3(nb), dim3(TB), 0, ctx->stream, din, rr, n);
     SH_TRY(sh_launch_check(ctx, "soint2d prec setup"));
-    const double TOL = (double)1.e-12f, EPSILON = (double)1.e-12f;
-    double dprr0 = 1.0, dpgm0 = 1.0, h[8];
-    bool started = false;
-    int it;
-    for (it = 0; it < niter; it++) {
-        hipLaunchKernelGGL(mask_adj, dim3(nb), dim3(TB), 0, ctx->stream, known, rr, xt, n);
-        SH_TRY(prec_apply<NW>(ctx, 1, gv, xt, pp, qq, Fp, Fq, tmp, tt, ys, n1, n2, e));
-        SH_TRY(prec_apply<NW>(ctx, 0, gv, xt, pp, qq, Fp, Fq, tmp, tt, ys, n1, n2, e));
-        hipLaunchKernelGGL(mask_fwd_gg, dim3(nb), dim3(TB), 0, ctx->stream, known, xt, rr, gg, n);
-        hipLaunchKernelGGL(dot_kernel, dim3(nb), dim3(TB), 0, ctx->stream, gg, gg, n, part, 0, nb);
-        hipLaunchKernelGGL(dot_kernel, dim3(nb), dim3(TB), 0, ctx->stream, gg, Ss, n, part, 1, nb);
-        hipLaunchKernelGGL(dot_kernel, dim3(nb), dim3(TB), 0, ctx->stream, gg, rr, n, part, 2, nb);
-        hipLaunchKernelGGL(dot_kernel, dim3(nb), dim3(TB), 0, ctx->stream, Ss, Ss, n, part, 3, nb);
-        hipLaunchKernelGGL(dot_kernel, dim3(nb), dim3(TB), 0, ctx->stream, Ss, rr, n, part, 4, nb);
-        hipLaunchKernelGGL(dot_kernel, dim3(nb), dim3(TB), 0, ctx->stream, rr, rr, n, part, 5, nb);
-        hipLaunchKernelGGL(dot_kernel, dim3(nb), dim3(TB), 0, ctx->stream, gv, gv, n, part, 6, nb);
-        SH_TRY(sh_launch_check(ctx, "soint2d prec dots"));
-        SH_TRY(sum_partials(ctx, part, nb, 7, dsum, h));
-        const double gdg = h[0], gds = h[1], gdr0 = h[2], sds = h[3], sdr0 = h[4], rnorm = h[5], gnorm = h[6];
-        // the reference sums these two norms in float; they only decide the
-        // early stop against 1e-12
-        double dprr, dpgm;
-        if (it == 0) {
-            dprr0 = (float)rnorm;
-            dpgm0 = (float)gnorm;
-            dprr = 1.0;
-            dpgm = 1.0;
-        } else {
-            dprr = (float)rnorm / dprr0;
-            dpgm = (float)gnorm / dpgm0;
-        }
-        if (dprr < TOL || dpgm < TOL) {
-            SH_TRY(prec_apply<NW>(ctx, 0, p, out, pp, qq, Fp, Fq, tmp, tt, ys, n1, n2, e));
-            break;
-        }
-        // ps_cgstep (soint2d_cfuns.c:493-545); its first call restarts
-        const bool forget = !started;
-        started = true;
-        bool step = true;
-        double alfa = 0.0, beta = 0.0;
-        if (forget) {
-            hipLaunchKernelGGL(zero_f, dim3(nb), dim3(TB), 0, ctx->stream, S, n);
-            hipLaunchKernelGGL(zero_f, dim3(nb), dim3(TB), 0, ctx->stream, Ss, n);
-            alfa = gdg;
-            if (alfa <= 0.) step = false;
-            else alfa = -gdr0 / alfa;
-        } else if (gdg == 0. || sds == 0.) {
-            step = false;
-        } else {
-            double determ = 1.0 - (gds / gdg) * (gds / sds);
-            if (determ > EPSILON) determ *= gdg * sds;
-            else determ = gdg * sds * EPSILON;
-            const double gdr = -gdr0, sdr = -sdr0;
-            alfa = (sds * gdr - gds * sdr) / determ;
-            beta = (-gds * gdr + gdg * sdr) / determ;
-        }
-        if (step) {
-            hipLaunchKernelGGL(cgstep_update, dim3(nb), dim3(TB), 0, ctx->stream, S, gv, p, n, Ss, gg, rr, n,
-                               (float)alfa, (float)beta);
-            SH_TRY(sh_launch_check(ctx, "cgstep_update"));
-        }
-        if (it == niter - 1) SH_TRY(prec_apply<NW>(ctx, 0, p, out, pp, qq, Fp, Fq, tmp, tt, ys, n1, n2, e));
-    }
-    if (iters_out) *iters_out = it;
-    return SH_OK;
+    hipLaunchKernelGGL(solver_init, dim3(1), dim3(1024), 0, ctx->stream, part, 0, st, 0);
+    SH_TRY(sh_launch_check(ctx, "soint2d prec setup"));
+    double *pa = part + (size_t)6 * nb;
+    // one iteration of ps_solver_prec (soint2d_cfuns.c:1704-1990) with
+    // ps_cgstep (493-545); the reference sums the two norms that decide the
+    // early stop in float
+    auto iteration = [&]() -> int {
+        hipLaunchKernelGGL(mask_adj_st, dim3(nb), dim3(TB), 0, ctx->stream, known, rr, xt, n, st);
+        SH_TRY(prec_apply<NW>(ctx, 1, gv, xt, pp, qq, Fp, Fq, tmp, tt, ys, n1, n2, e, st));
+        SH_TRY(prec_apply<NW>(ctx, 0, gv, xt, pp, qq, Fp, Fq, tmp, tt, ys, n1, n2, e, st));
+        hipLaunchKernelGGL(mask_fwd_gg_dots, dim3(nb), dim3(TB), 0, ctx->stream, known, xt, rr, gg, Ss, gv, n, part,
+                           nb, st);
+        hipLaunchKernelGGL(cgstep_scalars, dim3(1), dim3(1024), 0, ctx->stream, pa, nb, part, nb, st, 1);
+        hipLaunchKernelGGL(cgstep_update, dim3(nb), dim3(TB), 0, ctx->stream, S, gv, p, n, Ss, gg, rr, n, st);
+        return sh_launch_check(ctx, "soint2d prec iteration");
+    };
+    SH_TRY(run_iterations(ctx, niter, "soint2d_prec_iteration", 4.0 * 10.0 * (double)n, iteration));
+    // the model maps to the output through the preconditioner once the
+    // iterations stop (after the last one, or at the early stop)
+    if (niter > 0) SH_TRY(prec_apply<NW>(ctx, 0, p, out, pp, qq, Fp, Fq, tmp, tt, ys, n1, n2, e, nullptr));
+    return read_iters(ctx, st, iters_out);
 }
 
 __global__ void count_kernel(const unsigned char *__restrict__ m, long n, double *part) {
@@ -957,7 +1227,7 @@ __global__ void count_kernel(const unsigned char *__restrict__ m, long n, double
 
 extern "C" {
 
-int sh_soint3d(sh_ctx *ctx, const float *din, const float *mask, const float *dipi, const float *dipx, int n1,
+int shd_soint3d(sh_ctx *ctx, const float *din, const float *mask, const float *dipi, const float *dipx, int n1,
                int n2, int n3, int nw, int nj1, int nj2, int niter, int drift, int seed, int hasmask, float var,
                float *out, int *iters) {
     if (!ctx) return SH_ENULL;
@@ -991,7 +1261,7 @@ int sh_soint3d(sh_ctx *ctx, const float *din, const float *mask, const float *di
     return SH_OK;
 }
 
-int sh_soint2d(sh_ctx *ctx, const float *din, const float *mask, const float *dip, int n1, int n2, int nw, int nj1,
+int shd_soint2d(sh_ctx *ctx, const float *din, const float *mask, const float *dip, int n1, int n2, int nw, int nj1,
                int niter, int drift, int hasmask, int twoplane, float *out, int *iters) {
     if (!ctx) return SH_ENULL;
     if (!din || !dip || !out || (hasmask == 1 && !mask)) return sh_fail(ctx, SH_ENULL, "soint2d: NULL pointer");
@@ -1016,7 +1286,7 @@ int sh_soint2d(sh_ctx *ctx, const float *din, const float *mask, const float *di
     return SH_OK;
 }
 
-int sh_soint2d_prec(sh_ctx *ctx, const float *din, const float *mask, const float *dip, const float *dip2, int n1,
+int shd_soint2d_prec(sh_ctx *ctx, const float *din, const float *mask, const float *dip, const float *dip2, int n1,
                     int n2, int nw, int niter, int hasmask, float *out, int *iters) {
     if (!ctx) return SH_ENULL;
     if (!din || !dip || !out || (hasmask == 1 && !mask)) return sh_fail(ctx, SH_ENULL, "soint2d: NULL pointer");
@@ -1052,10 +1322,7 @@ static int sint_common(sh_ctx *ctx, const float *din, const float *dipi, const f
         SH_TRY(sh_transpose_internal(ctx, dipi + o, diT + o, n1, n2, true));
         if (three) SH_TRY(sh_transpose_internal(ctx, dipx + o, t1 + o, n1, n2, true));
     }
-    if (three) {
-        hipLaunchKernelGGL(permute_xi, dim3(nb), dim3(TB), 0, ctx->stream, t1, dxX, n1, n2, n3, 0);
-        SH_TRY(sh_launch_check(ctx, "permute"));
-    }
+    if (three) SH_TRY(permute(ctx, t1, dxX, n1, n2, n3, 0));
     hipLaunchKernelGGL(known_kernel, dim3(nb), dim3(TB), 0, ctx->stream, mT, known, n);
     // lam = sqrt(#known / N) (soint3d_cfuns.c:2586-2595)
     double *part = nullptr, *dsum = nullptr;
@@ -1085,7 +1352,7 @@ static int sint_common(sh_ctx *ctx, const float *din, const float *dipi, const f
     return SH_OK;
 }
 
-int sh_sint3d(sh_ctx *ctx, const float *din, const float *dipi, const float *dipx, const float *mask, int n1, int n2,
+int shd_sint3d(sh_ctx *ctx, const float *din, const float *dipi, const float *dipx, const float *mask, int n1, int n2,
               int n3, int niter, int ns1, int ns2, int order1, int order2, float eps, float *out, int *iters) {
     if (!ctx) return SH_ENULL;
     if (!din || !dipi || !dipx || !mask || !out) return sh_fail(ctx, SH_ENULL, "sint3d: NULL pointer");
@@ -1093,7 +1360,7 @@ int sh_sint3d(sh_ctx *ctx, const float *din, const float *dipi, const float *dip
     return sint_common(ctx, din, dipi, dipx, mask, n1, n2, n3, niter, ns1, ns2, order1, order2, eps, out, iters);
 }
 
-int sh_sint2d(sh_ctx *ctx, const float *din, const float *dip, const float *mask, int n1, int n2, int niter, int ns,
+int shd_sint2d(sh_ctx *ctx, const float *din, const float *dip, const float *mask, int n1, int n2, int niter, int ns,
               int order, float eps, float *out, int *iters) {
     if (!ctx) return SH_ENULL;
     if (!din || !dip || !mask || !out) return sh_fail(ctx, SH_ENULL, "sint2d: NULL pointer");

@@ -77,10 +77,6 @@ int transpose(sh_ctx *ctx, const float *src, float *dst, int n1, long ntr, bool 
     return sh_launch_check(ctx, "transpose");
 }
 
-__global__ void zero_kernel(float *__restrict__ y, long n) {
-    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) y[i] = 0.0f;
-}
 
 #include "predict.hpp"
 
@@ -352,7 +348,9 @@ factor2_kernel(const float *__restrict__ dip, float *__restrict__ F, int n1, int
 template <int NW>
 __global__ void __launch_bounds__(TB)
 spray2_fwd(const float *__restrict__ x, const float *__restrict__ dip, const float *__restrict__ F,
-           float *__restrict__ U, int n1, int n2, int n3, int ns, float e, float *__restrict__ scr, long nthr) {
+           float *__restrict__ U, int n1, int n2, int n3, int ns, float e, float *__restrict__ scr, long nthr,
+           const int *stop) {
+    if (stop && *(volatile const int *)stop) return;
     long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;  // over n2 * 2
     int i3 = blockIdx.y;
     if (gid >= 2L * n2) return;
@@ -368,6 +366,14 @@ spray2_fwd(const float *__restrict__ x, const float *__restrict__ dip, const flo
     if (dir == 0) {  // centre slot written once
         Sink c{U + (long)ns * plane_slot + i3 * slice + i, n2};
         for (int k = 0; k < n1; k++) c[k] = xs[(long)k * n2 + i];
+    }
+    // slots of this trace that no chain reaches (their source lies outside
+    // the slice) are zero: toward lower i2 the source of slot ns-is-1 is
+    // i+is+1, toward higher i2 that of slot ns+is+1 is i-is-1
+    for (int is = dir == 0 ? max(0, n2 - 1 - i) : min(i, ns); is < ns; is++) {
+        const int slot = dir == 0 ? ns - is - 1 : ns + is + 1;
+        Sink z{U + (long)slot * plane_slot + i3 * slice + i, n2};
+        for (int k = 0; k < n1; k++) z[k] = 0.0f;
     }
     Acc cur{xs + i, n2};
     for (int is = 0; is < ns; is++) {
@@ -393,7 +399,8 @@ template <int NW>
 __global__ void __launch_bounds__(TB)
 spray2_adj(const float *__restrict__ U, const float *__restrict__ dip, const float *__restrict__ F,
            float *__restrict__ C, int n1, int n2, int n3, int ns, float e, float *__restrict__ scr,
-           float *__restrict__ tr, long nthr) {
+           float *__restrict__ tr, long nthr, const int *stop) {
+    if (stop && *(volatile const int *)stop) return;
     long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
     int i3 = blockIdx.y;
     if (gid >= 2L * n2) return;
@@ -697,6 +704,11 @@ int stage3(sh_ctx *ctx, const float *din, const float *dipi, const float *dipx, 
 // Threads of the 2-D spray chains (two directions per trace and slice).
 long pws2_nthr(int n2, int n3) { return (long)ceil_div(2L * n2, TB) * TB * n3; }
 
+// The 2-D chain kernels run one thread per (trace, direction, slice), a
+// few thousand sequential chains: 64-thread blocks spread them over every
+// CU instead of filling a quarter of the chip with 256-thread blocks.
+constexpr int TB2 = 64;
+
 // Factor cache of P into caller-provided buffers: scr holds 2*nthr*n1
 // floats (solver scratch, then the running trace planes), F 2*(2*order+1)*n.
 int pws2_prepare_bufs(sh_ctx *ctx, Pwsmooth2 &P, float *scr, float *F) {
@@ -705,10 +717,10 @@ int pws2_prepare_bufs(sh_ctx *ctx, Pwsmooth2 &P, float *scr, float *F) {
     P.scr = scr;
     P.tr = P.scr + (long)P.nthr * P.n1;
     P.F = F;
-    dim3 grid(ceil_div(2L * P.n2, TB), P.n3);
+    dim3 grid(ceil_div(2L * P.n2, TB2), P.n3);
     PROF_A(ctx);
     SH_DISPATCH_NW(ctx, P.order, NW,
-                   hipLaunchKernelGGL((factor2_kernel<NW>), grid, dim3(TB), 0, ctx->stream, P.dip, P.F, P.n1, P.n2,
+                   hipLaunchKernelGGL((factor2_kernel<NW>), grid, dim3(TB2), 0, ctx->stream, P.dip, P.F, P.n1, P.n2,
                                       P.n3, P.e));
     PROF_B(ctx, "factor2_kernel", 4.0 * (double)n);
     return sh_launch_check(ctx, "factor2_kernel");
@@ -725,11 +737,11 @@ int pws2_prepare(sh_ctx *ctx, Pwsmooth2 &P, int slot_scr, int slot_F) {
 }
 
 int pws2_spray(sh_ctx *ctx, Pwsmooth2 &P, const float *x) {
-    dim3 grid(ceil_div(2L * P.n2, TB), P.n3);
+    dim3 grid(ceil_div(2L * P.n2, TB2), P.n3);
     PROF_A(ctx);
     SH_DISPATCH_NW(ctx, P.order, NW,
-                   hipLaunchKernelGGL((spray2_fwd<NW>), grid, dim3(TB), 0, ctx->stream, x, P.dip, P.F, P.U, P.n1,
-                                      P.n2, P.n3, P.ns, P.e, P.scr, P.nthr));
+                   hipLaunchKernelGGL((spray2_fwd<NW>), grid, dim3(TB2), 0, ctx->stream, x, P.dip, P.F, P.U, P.n1,
+                                      P.n2, P.n3, P.ns, P.e, P.scr, P.nthr, P.stop));
     PROF_B(ctx, "spray2_fwd", 12.0 * 2 * P.ns * (double)P.n1 * P.n2 * P.n3);
     return sh_launch_check(ctx, "spray2_fwd");
 }
@@ -748,8 +760,7 @@ int pws2_setup(sh_ctx *ctx, Pwsmooth2 &P, int slot_base) {
     int g = ceil_div(n, TB);
     hipLaunchKernelGGL(fill_one, dim3(g), dim3(TB), 0, ctx->stream, ones, n);
     SH_TRY(sh_launch_check(ctx, "fill_one"));
-    hipLaunchKernelGGL(zero_kernel, dim3(ceil_div(n * ns2, TB)), dim3(TB), 0, ctx->stream, P.U, n * ns2);
-    SH_TRY(pws2_spray(ctx, P, ones));
+    SH_TRY(pws2_spray(ctx, P, ones));  // writes every slot
     hipLaunchKernelGGL(pwsmooth_norm, dim3(g), dim3(TB), 0, ctx->stream, P.U, P.ww, n, P.ns);
     return sh_launch_check(ctx, "pwsmooth_norm");
 }
@@ -760,19 +771,30 @@ int pws2_apply(sh_ctx *ctx, Pwsmooth2 &P, int adj, int add, float *trace, float 
     int ns2 = 2 * P.ns + 1;
     int g = ceil_div(n, TB);
     if (!adj) {
-        hipLaunchKernelGGL(zero_kernel, dim3(ceil_div(n * ns2, TB)), dim3(TB), 0, ctx->stream, P.U, n * ns2);
-        SH_TRY(pws2_spray(ctx, P, trace));
+        SH_TRY(pws2_spray(ctx, P, trace));  // writes every slot
+        PROF_A(ctx);
         hipLaunchKernelGGL(pwsmooth_fwd, dim3(g), dim3(TB), 0, ctx->stream, P.U, P.ww, smooth, n, P.ns, add);
+        PROF_B(ctx, "pwsmooth_fwd", 4.0 * (ns2 + 2 + add) * (double)n);
         return sh_launch_check(ctx, "pwsmooth_fwd");
     }
-    hipLaunchKernelGGL(pwsmooth_adj_weight, dim3(g), dim3(TB), 0, ctx->stream, smooth, P.ww, P.U, n, P.ns);
+    {
+        PROF_A(ctx);
+        hipLaunchKernelGGL(pwsmooth_adj_weight, dim3(g), dim3(TB), 0, ctx->stream, smooth, P.ww, P.U, n, P.ns);
+        PROF_B(ctx, "pwsmooth_adj_weight", 4.0 * (ns2 + 2) * (double)n);
+    }
     SH_TRY(sh_launch_check(ctx, "pwsmooth_adj_weight"));
-    dim3 grid(ceil_div(2L * P.n2, TB), P.n3);
-    SH_DISPATCH_NW(ctx, P.order, NW,
-                   hipLaunchKernelGGL((spray2_adj<NW>), grid, dim3(TB), 0, ctx->stream, P.U, P.dip, P.F, P.C, P.n1,
-                                      P.n2, P.n3, P.ns, P.e, P.scr, P.tr, P.nthr));
+    dim3 grid(ceil_div(2L * P.n2, TB2), P.n3);
+    {
+        PROF_A(ctx);
+        SH_DISPATCH_NW(ctx, P.order, NW,
+                       hipLaunchKernelGGL((spray2_adj<NW>), grid, dim3(TB2), 0, ctx->stream, P.U, P.dip, P.F, P.C,
+                                          P.n1, P.n2, P.n3, P.ns, P.e, P.scr, P.tr, P.nthr, P.stop));
+        PROF_B(ctx, "spray2_adj", 12.0 * 2 * P.ns * (double)n);
+    }
     SH_TRY(sh_launch_check(ctx, "spray2_adj"));
+    PROF_A(ctx);
     hipLaunchKernelGGL(spray2_adj_combine, dim3(g), dim3(TB), 0, ctx->stream, P.C, P.U, trace, n, P.ns, add);
+    PROF_B(ctx, "spray2_adj_combine", 4.0 * (4 + add) * (double)n);
     return sh_launch_check(ctx, "spray2_adj_combine");
 }
 
@@ -789,7 +811,7 @@ int sh_median_internal(sh_ctx *ctx, const float *U, float *out, long n, int firs
 // ---------------------------------------------------------------------------
 extern "C" {
 
-int sh_pwspray3d(sh_ctx *ctx, const float *din, const float *dipi, const float *dipx, int n1, int n2, int n3,
+int shd_pwspray3d(sh_ctx *ctx, const float *din, const float *dipi, const float *dipx, int n1, int n2, int n3,
                  int ns2, int ns3, int order, float eps, float *u) {
     if (!ctx) return SH_ENULL;
     SH_TRY(check3(ctx, din, dipi, dipx, n1, n2, n3, ns2, ns3, "pwspray3d"));
@@ -812,7 +834,7 @@ int sh_pwspray3d(sh_ctx *ctx, const float *din, const float *dipi, const float *
     return SH_OK;
 }
 
-int sh_somean3d(sh_ctx *ctx, const float *din, const float *dipi, const float *dipx, int n1, int n2, int n3, int r1,
+int shd_somean3d(sh_ctx *ctx, const float *din, const float *dipi, const float *dipx, int n1, int n2, int n3, int r1,
                 int r2, float eps, int order, float *out) {
     (void)eps;
     if (!ctx) return SH_ENULL;
@@ -858,7 +880,7 @@ __global__ void svmf_traces(const float *__restrict__ U, float *__restrict__ out
 
 extern "C" {
 
-int sh_somf3d(sh_ctx *ctx, const float *din, const float *dipi, const float *dipx, int n1, int n2, int n3, int r1,
+int shd_somf3d(sh_ctx *ctx, const float *din, const float *dipi, const float *dipx, int n1, int n2, int n3, int r1,
               int r2, int nmf, int option, int order, float eps, float *out) {
     (void)eps;
     if (!ctx) return SH_ENULL;
@@ -891,7 +913,7 @@ int sh_somf3d(sh_ctx *ctx, const float *din, const float *dipi, const float *dip
     return transpose(ctx, dT, out, n1, ntr, false);
 }
 
-int sh_pwspray2d(sh_ctx *ctx, int adj, const float *dip, int n1, int n2, int ns, int order, float eps, float *x,
+int shd_pwspray2d(sh_ctx *ctx, int adj, const float *dip, int n1, int n2, int ns, int order, float eps, float *x,
                  float *u) {
     if (!ctx) return SH_ENULL;
     if (!dip || !x || !u) return sh_fail(ctx, SH_ENULL, "pwspray2d: NULL pointer");
@@ -910,8 +932,7 @@ int sh_pwspray2d(sh_ctx *ctx, int adj, const float *dip, int n1, int n2, int ns,
     SH_TRY(pws2_prepare(ctx, P, SLOT_SPRAY0 + 4, SLOT_SPRAY0 + 9));
     if (!adj) {
         SH_TRY(transpose(ctx, x, xT, n1, n2, true));
-        hipLaunchKernelGGL(zero_kernel, dim3(ceil_div(n * ns2, TB)), dim3(TB), 0, ctx->stream, P.U, n * ns2);
-        SH_TRY(pws2_spray(ctx, P, xT));
+        SH_TRY(pws2_spray(ctx, P, xT));  // writes every slot
         for (int s = 0; s < ns2; s++) {
             SH_TRY(transpose(ctx, P.U + (long)s * n, tmp, n1, n2, false));
             SH_HIP(ctx, hipMemcpy2DAsync(u + (long)s * n1, (size_t)ns2 * n1 * sizeof(float), tmp,
@@ -926,17 +947,17 @@ int sh_pwspray2d(sh_ctx *ctx, int adj, const float *dip, int n1, int n2, int ns,
                                      (size_t)n1 * sizeof(float), (size_t)n2, hipMemcpyDeviceToDevice, ctx->stream));
         SH_TRY(transpose(ctx, tmp, P.U + (long)s * n, n1, n2, true));
     }
-    dim3 grid(ceil_div(2L * n2, TB), 1);
+    dim3 grid(ceil_div(2L * n2, TB2), 1);
     SH_DISPATCH_NW(ctx, order, NW,
-                   hipLaunchKernelGGL((spray2_adj<NW>), grid, dim3(TB), 0, ctx->stream, P.U, P.dip, P.F, P.C, n1, n2, 1,
-                                      ns, P.e, P.scr, P.tr, P.nthr));
+                   hipLaunchKernelGGL((spray2_adj<NW>), grid, dim3(TB2), 0, ctx->stream, P.U, P.dip, P.F, P.C, n1, n2, 1,
+                                      ns, P.e, P.scr, P.tr, P.nthr, nullptr));
     SH_TRY(sh_launch_check(ctx, "spray2_adj"));
     hipLaunchKernelGGL(spray2_adj_combine, dim3(ceil_div(n, TB)), dim3(TB), 0, ctx->stream, P.C, P.U, xT, n, ns, 0);
     SH_TRY(sh_launch_check(ctx, "spray2_adj_combine"));
     return transpose(ctx, xT, x, n1, n2, false);
 }
 
-int sh_somean2d(sh_ctx *ctx, const float *dn, const float *dip, int n1, int n2, int n3, int ns, int order, int adj,
+int shd_somean2d(sh_ctx *ctx, const float *dn, const float *dip, int n1, int n2, int n3, int ns, int order, int adj,
                 float eps, float *out) {
     if (!ctx) return SH_ENULL;
     if (!dn || !dip || !out) return sh_fail(ctx, SH_ENULL, "somean2d: NULL pointer");
@@ -1024,8 +1045,7 @@ int somf2d_core(sh_ctx *ctx, const float *dn, const float *dip, int n1, int n2, 
         SH_TRY(pws2_prepare_bufs(ctx, P, ws + w.b + w.U, ws + w.b + w.U + w.scr));
     else
         SH_TRY(pws2_prepare(ctx, P, SLOT_SPRAY0 + 7, SLOT_SPRAY0 + 9));
-    hipLaunchKernelGGL(zero_kernel, dim3(ceil_div(n * np, TB)), dim3(TB), 0, ctx->stream, P.U, n * np);
-    SH_TRY(pws2_spray(ctx, P, xT));
+    SH_TRY(pws2_spray(ctx, P, xT));  // writes every slot
     int m = (nmf - 1) / 2;
     if (option == 2) {
         const long ntraces = (long)n2 * n3;
@@ -1076,7 +1096,7 @@ int noalloc_scratch(sh_ctx *ctx, float **p) {
 
 extern "C" {
 
-int sh_somf2d(sh_ctx *ctx, const float *dn, const float *dip, int n1, int n2, int n3, int ns, int nmf, int option,
+int shd_somf2d(sh_ctx *ctx, const float *dn, const float *dip, int n1, int n2, int n3, int ns, int nmf, int option,
               int order, float eps, float *out) {
     if (!ctx) return SH_ENULL;
     SH_TRY(somf2d_args(ctx, dn, dip, out, n1, n2, n3, ns, nmf, option));
@@ -1091,6 +1111,8 @@ size_t sh_somf2d_noalloc_bytes(int n1, int n2, int n3, int ns, int order) {
 int sh_somf2d_noalloc(sh_ctx *ctx, const float *dn, const float *dip, int n1, int n2, int n3, int ns, int nmf,
                       int option, int order, float eps, float *out) {
     if (!ctx) return SH_ENULL;
+    if (ctx->flags & SH_FLAG_HOST_PTRS)
+        return sh_fail(ctx, SH_EINVAL, "somf2d_noalloc: device pointers only (host pointers need staging buffers)");
     SH_TRY(somf2d_args(ctx, dn, dip, out, n1, n2, n3, ns, nmf, option));
     if (order < 1 || order > 4) return sh_fail(ctx, SH_EUNSUP, "PWD order %d not supported (1..4)", order);
     const size_t need = somf2d_ws(n1, n2, n3, ns, order).total;

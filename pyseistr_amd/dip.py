@@ -41,7 +41,8 @@ def dip_device(din, shape, niter=5, liter=20, order=2, eps_dv=0.01, rect=(1, 1, 
     if report:
         m = min(int(niter), 64)  # the record keeps the first 64 iterations
         rec = {"cg_iters": [[rep.cg_iters[d][k] for k in range(m)] for d in range(nd)],
-               "ls_step": [[rep.ls_step[d][k] for k in range(m)] for d in range(nd)]}
+               "ls_step": [[rep.ls_step[d][k] for k in range(m)] for d in range(nd)],
+               "elapsed_ms": float(rep.elapsed_ms)}
         return out, rec
     return out
 

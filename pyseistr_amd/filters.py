@@ -69,13 +69,22 @@ def somean2dc(dn, dip, ns, order, eps, adj=0, verb=1):
     dn = np.asarray(dn)
     n1, n2, n3 = _shape3(dn)
     _lib.check_sizes("somean2dc", n1 * n2 * n3, dip=dip)
-    ctx = _lib.context()
-    d, p = _lib.to_device(dn), _lib.to_device(dip)
-    out = _lib.empty(n1 * n2 * n3)
-    _lib.check(ctx, _lib.load().sh_somean2d(ctx, _lib.ptr(d), _lib.ptr(p), n1, n2, n3, int(ns), int(order),
-                                            int(adj), float(eps), _lib.ptr(out)), "somean2dc")
+    out = somean2d_device(_lib.to_device(dn), _lib.to_device(dip), (n1, n2, n3), ns, order, eps, adj)
     ds = _lib.to_host(out).reshape(n1, n2, n3, order="F")
     return np.squeeze(ds) if n3 == 1 else ds
+
+
+def somean2d_device(d, p, shape, ns, order=2, eps=0.01, adj=0, out=None):
+    """Device call of somean2dc: flat F-order tensors in and out."""
+    n1, n2, n3 = shape
+    _lib.check_sizes("somean2dc", n1 * n2 * n3, data=d, dip=p)
+    _lib.check_out(out, n1 * n2 * n3, "somean2dc")
+    d, p = _lib.to_device(d), _lib.to_device(p)
+    ctx = _lib.context()
+    out = _lib.empty(n1 * n2 * n3) if out is None else out
+    _lib.check(ctx, _lib.load().sh_somean2d(ctx, _lib.ptr(d), _lib.ptr(p), n1, n2, n3, int(ns), int(order),
+                                            int(adj), float(eps), _lib.ptr(out)), "somean2dc")
+    return out
 
 
 def somf2d_device(d, p, shape, ns, order=2, eps=0.01, option=1, out=None):

@@ -21,22 +21,33 @@ def _iters():
     return ctypes.c_int(0)
 
 
+def soint3d_device(d, m, p, q, shape, order=1, niter=100, njs=(1, 1), drift=0, seed=202223, hasmask=1, var=0,
+                   out=None, report=False):
+    """Device call of soint3dc: flat F-order tensors in and out; with
+    report=True also the iterations run (filling it synchronises)."""
+    n1, n2, n3 = shape
+    _lib.check_sizes("soint3dc", n1 * n2 * n3, data=d, mask=m, dipi=p, dipx=q)
+    _lib.check_out(out, n1 * n2 * n3, "soint3dc")
+    d, m, p, q = (_lib.to_device(a) for a in (d, m, p, q))
+    ctx = _lib.context()
+    out = _lib.empty(n1 * n2 * n3) if out is None else out
+    it = _iters()
+    rc = _lib.load().sh_soint3d(ctx, _lib.ptr(d), _lib.ptr(m), _lib.ptr(p), _lib.ptr(q), n1, n2, n3, int(order),
+                                int(njs[0]), int(njs[1]), int(niter), int(drift), int(seed), int(hasmask),
+                                float(var), _lib.ptr(out), ctypes.byref(it) if report else None)
+    _lib.check(ctx, rc, "soint3dc")
+    return (out, it.value) if report else out
+
+
 def soint3dc(din, mask, dipi, dipx, order=1, niter=100, njs=[1, 1], drift=0, seed=202223, hasmask=1, var=0,
              verb=1, report=False):
     din = np.asarray(din)
     n1, n2, n3 = din.shape
     _lib.check_sizes("soint3dc", n1 * n2 * n3, mask=mask, dipi=dipi, dipx=dipx)
-    ctx = _lib.context()
-    d, m = _lib.to_device(din), _lib.to_device(mask)
-    p, q = _lib.to_device(dipi), _lib.to_device(dipx)
-    out = _lib.empty(n1 * n2 * n3)
-    it = _iters()
-    rc = _lib.load().sh_soint3d(ctx, _lib.ptr(d), _lib.ptr(m), _lib.ptr(p), _lib.ptr(q), n1, n2, n3, int(order),
-                                int(njs[0]), int(njs[1]), int(niter), int(drift), int(seed), int(hasmask),
-                                float(var), _lib.ptr(out), ctypes.byref(it))
-    _lib.check(ctx, rc, "soint3dc")
+    out, it = soint3d_device(_lib.to_device(din), _lib.to_device(mask), _lib.to_device(dipi), _lib.to_device(dipx),
+                             (n1, n2, n3), order, niter, njs, drift, seed, hasmask, var, report=True)
     res = _lib.to_host(out).reshape(n1, n2, n3, order="F")
-    return (res, it.value) if report else res
+    return (res, it) if report else res
 
 
 def soint2dc(din, mask, dip, order=1, niter=100, njs=[1, 1], drift=0, hasmask=1, twoplane=0, prec=0, verb=1,
@@ -79,21 +90,31 @@ def _soint2d_prec(din, mask, dip, order, niter, hasmask, twoplane, report):
     return (res, it.value) if report else res
 
 
+def sint3d_device(d, m, p, q, shape, niter=100, eps=0.01, ns1=1, ns2=1, order1=1, order2=1, out=None,
+                  report=False):
+    """Device call of sint3dc (flat F-order tensors; mask m)."""
+    n1, n2, n3 = shape
+    _lib.check_sizes("sint3dc", n1 * n2 * n3, data=d, mask=m, dipi=p, dipx=q)
+    _lib.check_out(out, n1 * n2 * n3, "sint3dc")
+    d, m, p, q = (_lib.to_device(a) for a in (d, m, p, q))
+    ctx = _lib.context()
+    out = _lib.empty(n1 * n2 * n3) if out is None else out
+    it = _iters()
+    rc = _lib.load().sh_sint3d(ctx, _lib.ptr(d), _lib.ptr(p), _lib.ptr(q), _lib.ptr(m), n1, n2, n3, int(niter),
+                               int(ns1), int(ns2), int(order1), int(order2), float(eps), _lib.ptr(out),
+                               ctypes.byref(it) if report else None)
+    _lib.check(ctx, rc, "sint3dc")
+    return (out, it.value) if report else out
+
+
 def sint3dc(din, mask, dipi, dipx, niter=100, eps=0.01, ns1=1, ns2=1, order1=1, order2=1, verb=1, report=False):
     din = np.asarray(din)
     n1, n2, n3 = din.shape
     _lib.check_sizes("sint3dc", n1 * n2 * n3, mask=mask, dipi=dipi, dipx=dipx)
-    ctx = _lib.context()
-    d, m = _lib.to_device(din), _lib.to_device(mask)
-    p, q = _lib.to_device(dipi), _lib.to_device(dipx)
-    out = _lib.empty(n1 * n2 * n3)
-    it = _iters()
-    rc = _lib.load().sh_sint3d(ctx, _lib.ptr(d), _lib.ptr(p), _lib.ptr(q), _lib.ptr(m), n1, n2, n3, int(niter),
-                               int(ns1), int(ns2), int(order1), int(order2), float(eps), _lib.ptr(out),
-                               ctypes.byref(it))
-    _lib.check(ctx, rc, "sint3dc")
+    out, it = sint3d_device(_lib.to_device(din), _lib.to_device(mask), _lib.to_device(dipi), _lib.to_device(dipx),
+                            (n1, n2, n3), niter, eps, ns1, ns2, order1, order2, report=True)
     res = _lib.to_host(out).reshape(n1, n2, n3, order="F")
-    return (res, it.value) if report else res
+    return (res, it) if report else res
 
 
 def sint2dc(din, mask, dip, niter=100, eps=0.01, ns=1, order=1, verb=1, report=False):
