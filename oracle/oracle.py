@@ -22,10 +22,15 @@ def build():
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
 
+# ORACLE_SANITIZED=1 loads the AddressSanitizer / UBSan builds
+# (make -C oracle sanitize; the process needs the ASan runtime preloaded)
+_SUFFIX = "_san" if os.environ.get("ORACLE_SANITIZED") == "1" else ""
+
+
 def _lib(dtype):
     key = np.dtype(dtype).itemsize
     if key not in _LIBS:
-        name = "liboracle_f32.so" if key == 4 else "liboracle_f64.so"
+        name = ("liboracle_f32" if key == 4 else "liboracle_f64") + _SUFFIX + ".so"
         path = os.path.join(_HERE, "lib", name)
         if not os.path.exists(path):
             build()
