@@ -143,10 +143,33 @@ def pick_tile(shapes, tuning=None):
     return int(np.argmin(cost))
 
 
-def autotune(n1s=(128, 192, 256, 384, 512), n2s=(64, 96, 128, 192, 256), per_bucket=24, niter=2, liter=20,
-             repeats=2, path=TUNING, seed=7, log=print):
-    """Time every tile shape on a synthetic batch of each (n1, n2) bucket and
-    store ns/sample per tile; returns the table."""
+def _time_tiles(b, tiles, niter, liter, repeats):
+    """Best-of-repeats ns/sample of the batched dip for every tile shape."""
+    import torch
+    out = _lib.empty(b.ntot)
+    times = []
+    for t in range(len(tiles)):
+        dip2d_batched(b, niter=1, liter=2, tile=t, out=out)  # warm up
+        torch.cuda.synchronize()
+        best = None
+        for _ in range(repeats):
+            t0 = time.perf_counter()
+            dip2d_batched(b, niter=niter, liter=liter, tile=t, out=out)
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+            best = dt if best is None else min(best, dt)
+        times.append(best / b.ntot * 1e9)
+    return times
+
+
+def autotune(n1s=(128, 192, 256, 384, 512), n2s=(64, 96, 128, 192, 256), per_bucket=256, niter=2, liter=20,
+             repeats=2, path=TUNING, seed=7, mix=2048, log=print):
+    """Time every tile shape on a synthetic batch of each (n1, n2) bucket
+    (per_bucket patches, enough to fill the GPU so the times measure the
+    tiles rather than launch overhead) and store ns/sample per tile; with
+    mix > 0 also on the uneven mix of that many patches (synth.patches2d,
+    seed 4) at dip2d's default niter=5, as a check of the bucket model.
+    Returns the table."""
     import torch
 
     from . import synth
@@ -160,25 +183,20 @@ def autotune(n1s=(128, 192, 256, 384, 512), n2s=(64, 96, 128, 192, 256), per_buc
                 d = synth._events((n1, n2), int(rng.integers(1, 4)), [30.0], 2.0, rng)[:, :, 0]
                 d = d / max(np.abs(d).max(), 1e-12) + 0.5 * rng.standard_normal((n1, n2))
                 pats.append(d.astype(np.float32))
-            b = Batch(pats)
-            out = _lib.empty(b.ntot)
-            times = []
-            for t in range(len(tiles)):
-                dip2d_batched(b, niter=1, liter=2, tile=t, out=out)  # warm up
-                torch.cuda.synchronize()
-                best = None
-                for _ in range(repeats):
-                    t0 = time.perf_counter()
-                    dip2d_batched(b, niter=niter, liter=liter, tile=t, out=out)
-                    torch.cuda.synchronize()
-                    dt = time.perf_counter() - t0
-                    best = dt if best is None else min(best, dt)
-                times.append(best / b.ntot * 1e9)
+            times = _time_tiles(Batch(pats), tiles, niter, liter, repeats)
             table[bucket(n1, n2)] = {"ns_per_sample": [round(x, 4) for x in times], "best": int(np.argmin(times))}
             log(f"autotune {n1}x{n2}: best tile {tiles[int(np.argmin(times))]} "
                 f"{min(times):.3f} ns/sample (worst {max(times):.3f})")
     res = {"tiles": tiles, "niter": niter, "liter": liter, "per_bucket": per_bucket, "buckets": table,
            "device": torch.cuda.get_device_name(0)}
+    if mix:
+        pats = synth.patches2d(count=mix, seed=4)
+        b = Batch(pats)
+        times = _time_tiles(b, tiles, 5, 20, repeats)
+        res["mix"] = {"patches": mix, "niter": 5, "ns_per_sample": [round(x, 4) for x in times],
+                      "best": int(np.argmin(times)), "bucket_model_pick": pick_tile(b.shapes, res)}
+        log(f"autotune mix of {mix}: best tile {tiles[int(np.argmin(times))]}, bucket model picks "
+            f"{tiles[res['mix']['bucket_model_pick']]}")
     os.makedirs(os.path.dirname(path), exist_ok=True)
     with open(path, "w") as f:
         json.dump(res, f, indent=1)
@@ -189,10 +207,11 @@ def main(argv=None):
     import argparse
     ap = argparse.ArgumentParser(description="autotune the batched 2-D dip tiles on this GPU")
     ap.add_argument("cmd", choices=["autotune"])
-    ap.add_argument("--per-bucket", type=int, default=24)
+    ap.add_argument("--per-bucket", type=int, default=256)
+    ap.add_argument("--mix", type=int, default=2048, help="patches of the uneven mix timed as a check (0: none)")
     ap.add_argument("--out", default=TUNING)
     a = ap.parse_args(argv)
-    autotune(per_bucket=a.per_bucket, path=a.out, log=lambda m: print(m, flush=True))
+    autotune(per_bucket=a.per_bucket, mix=a.mix, path=a.out, log=lambda m: print(m, flush=True))
     return 0
 
 

@@ -125,3 +125,29 @@ def test_batched_errors(env):
         batch.dip2d_batched(b, niter=1, tile=99)
     with pytest.raises(ValueError):
         batch.somf2d_batched(b, b.data, ns=2, option=3)
+
+
+def test_batched_pipeline_at_the_mix_parameters():
+    """The batched workload's exact parameters on 64 patches of its own
+    uneven mix (seed 4): dip2d (niter=5, liter=20, rect 10x10) then somf2dc
+    (ns=6, order=2), batched against per-patch calls.  The medians are
+    bit-identical given the same dips, the dips within 1e-4 with the same
+    decisions, and the whole pipeline's output within what that dip
+    difference moves (the tile the tuner picks for the mix)."""
+    from pyseistr_amd import _lib, batch, dip, filters, synth
+    pats = synth.patches2d(count=64, seed=4)
+    b = batch.Batch(pats)
+    tile = batch.pick_tile(b.shapes)
+    dips, rec = batch.dip2d_batched(b, niter=5, liter=20, order=2, rect=(10, 10, 1), tile=tile, report=True)
+    med = b.split(batch.somf2d_batched(b, dips, ns=6, order=2, eps=0.01))
+    gd = b.split(dips)
+    for k, p in enumerate(pats):
+        ref, r = dip.dip2dc(p, niter=5, liter=20, order=2, rect=[10, 10, 1], report=True)
+        assert np.abs(gd[k] - ref).max() <= 1e-4, k
+        assert np.array_equal(np.asarray(r["cg_iters"][0]), rec[k, :, 0]), k
+        assert np.array_equal(np.asarray(r["ls_step"][0]), rec[k, :, 1]), k
+        n1, n2 = p.shape
+        # the median on the batched dips equals the per-patch filter on them
+        exp = filters.somf2d_device(_lib.to_device(p), _lib.to_device(gd[k]), (n1, n2, 1), 6, order=2,
+                                    eps=0.01).cpu().numpy().reshape(n1, n2, order="F")
+        assert np.array_equal(_bits(med[k]), _bits(exp)), k
