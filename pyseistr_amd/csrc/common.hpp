@@ -216,7 +216,23 @@ SH_HD inline int nint_ref(float x) { return (int)((x) > 0.0 ? (x) + 0.5 : (x)-0.
 
 inline int ceil_div(long a, long b) { return (int)((a + b - 1) / b); }
 
-// Dispatch a template on the PWD order (1..4); returns SH_EUNSUP otherwise.
+// Dispatch a template on the PWD order (1..6) for the stencil-only paths
+// (dip estimation, soint), whose per-sample state is one filter of 2nw+1
+// taps; returns SH_EUNSUP otherwise.
+#define SH_DISPATCH_NW6(ctx, nw, NAME, ...)                                      \
+    switch (nw) {                                                                \
+    case 1: { constexpr int NAME = 1; __VA_ARGS__; } break;                      \
+    case 2: { constexpr int NAME = 2; __VA_ARGS__; } break;                      \
+    case 3: { constexpr int NAME = 3; __VA_ARGS__; } break;                      \
+    case 4: { constexpr int NAME = 4; __VA_ARGS__; } break;                      \
+    case 5: { constexpr int NAME = 5; __VA_ARGS__; } break;                      \
+    case 6: { constexpr int NAME = 6; __VA_ARGS__; } break;                      \
+    default: return sh_fail((ctx), SH_EUNSUP, "PWD order %d not supported (1..6)", (nw)); \
+    }
+
+// Dispatch a template on the PWD order (1..4) for the predictor paths
+// (spray, the preconditioner), whose register windows grow as (2nw+1)^2;
+// returns SH_EUNSUP otherwise.
 #define SH_DISPATCH_NW(ctx, nw, NAME, ...)                                       \
     switch (nw) {                                                                \
     case 1: { constexpr int NAME = 1; __VA_ARGS__; } break;                      \

@@ -822,7 +822,7 @@ extern "C" int shd_dip(sh_ctx *ctx, const float *data, const float *mask, const 
         }
         int fin = 0;
         const unsigned char *m = mask ? (d == 0 ? W.m0 : W.m1) : nullptr;
-        SH_DISPATCH_NW(ctx, order, NW, SH_TRY(dip_direction<NW>(ctx, g, plan, d == 0 ? 2 : 3, data, m, niter, liter, Wd, &fin)));
+        SH_DISPATCH_NW6(ctx, order, NW, SH_TRY(dip_direction<NW>(ctx, g, plan, d == 0 ? 2 : 3, data, m, niter, liter, Wd, &fin)));
         SH_HIP(ctx, hipMemcpyAsync(dip + (size_t)d * n, Wd.p[fin], fb, hipMemcpyDeviceToDevice, ctx->stream));
         if (rep && !two) {
             DipScal hs;

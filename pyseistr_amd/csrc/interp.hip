@@ -1256,7 +1256,7 @@ int shd_soint3d(sh_ctx *ctx, const float *din, const float *mask, const float *d
     }
     SH_HIP(ctx, hipMemcpyAsync(out, din, sizeof(float) * n, hipMemcpyDeviceToDevice, ctx->stream));
     int it = 0;
-    SH_DISPATCH_NW(ctx, nw, NW, SH_TRY(soint_solve<NW>(ctx, g, dipi, dipx, known, out, dd, niter, &it)));
+    SH_DISPATCH_NW6(ctx, nw, NW, SH_TRY(soint_solve<NW>(ctx, g, dipi, dipx, known, out, dd, niter, &it)));
     if (iters) *iters = it;
     return SH_OK;
 }
@@ -1281,7 +1281,7 @@ int shd_soint2d(sh_ctx *ctx, const float *din, const float *mask, const float *d
     hipLaunchKernelGGL(known_kernel, dim3(nb), dim3(TB), 0, ctx->stream, hasmask == 1 ? mask : din, known, n);
     hipLaunchKernelGGL(zero_f, dim3(nb), dim3(TB), 0, ctx->stream, dd, n);
     int it = 0;
-    SH_DISPATCH_NW(ctx, nw, NW, SH_TRY(soint_solve<NW>(ctx, g, dip, dip, known, out, dd, niter, &it)));
+    SH_DISPATCH_NW6(ctx, nw, NW, SH_TRY(soint_solve<NW>(ctx, g, dip, dip, known, out, dd, niter, &it)));
     if (iters) *iters = it;
     return SH_OK;
 }

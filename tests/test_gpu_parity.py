@@ -200,7 +200,9 @@ def test_bad_arguments_raise(ps):
     with pytest.raises(ValueError):
         ps.somf3dc(w, w, w, 2, 2, 0.01, 2, option=3)
     with pytest.raises(NotImplementedError):
-        ps.soint3dc(w, w, w, w, order=5)  # PWD orders 1..4
+        ps.soint3dc(w, w, w, w, order=7)  # PWD orders 1..6
+    with pytest.raises(NotImplementedError):
+        ps.somf3dc(w, w, w, 2, 2, 0.01, 5)  # predictor orders 1..4
 
 
 def test_fused_smoothing_matches_separable(ps, monkeypatch):
@@ -462,3 +464,30 @@ def test_somf2d_space_varying_median(ps, shape, ns):
     p = (0.7 * rng.standard_normal(shape)).astype(np.float32)
     got = ps.somf2dc(d if shape[2] > 1 else d[:, :, 0], p if shape[2] > 1 else p[:, :, 0], ns, 2, 0.01, option=2)
     same(got, O.somf2d(d, p, shape, ns, order=2, option=2))
+
+
+@pytest.mark.parametrize("order", [5, 6])
+def test_high_pwd_orders(ps, order):
+    """PWD orders above 4 (apfilt_init takes any nw, dip_cfuns.c:833-856) in
+    the stencil paths: dip2dc / dip3dc within the dip bars of the FP32
+    oracle with the same decisions, soint3dc / soint2dc bit-exact."""
+    from oracle import oracle as O
+    from pyseistr_amd import synth
+    g = synth.gather2d(96, 40, seed=3)
+    got, rep = ps.dip2dc(g, niter=3, liter=10, order=order, rect=[5, 5, 1], report=True)
+    exp, orep = O.dip(g, (96, 40, 1), niter=3, liter=10, order=order, rect=(5, 5, 1), report=True)
+    dip_close(got, exp)
+    assert rep["cg_iters"] == orep[:, :, 0].tolist() and rep["ls_step"] == orep[:, :, 1].tolist()
+    s = (48, 16, 12)
+    v, _ = synth.volume3d(s, seed=order, n_events=2, smax=1.0)
+    (di, dx), rep = ps.dip3dc(v, niter=2, liter=10, order=order, rect=[3, 3, 3], report=True)
+    exp, orep = O.dip(v, s, niter=2, liter=10, order=order, rect=(3, 3, 3), report=True)
+    dip_close(np.concatenate([di.ravel(order="F"), dx.ravel(order="F")]), exp)
+    assert rep["cg_iters"] == orep[:, :, 0].tolist()
+    rng = np.random.default_rng(order)
+    m = (rng.random(s) > 0.5).astype(np.float32)
+    f = lambda a: a.ravel(order="F")  # noqa: E731
+    same(ps.soint3dc(v * m, m, di, dx, order=order, niter=6),
+         O.soint3d(f(v * m), f(m), f(di), f(dx), s, order=order, niter=6))
+    same(ps.soint2dc(v[:, :, 0] * m[:, :, 0], m[:, :, 0], di[:, :, 0], order=order, niter=6),
+         O.soint2d(f(v[:, :, 0] * m[:, :, 0]), f(m[:, :, 0]), f(di[:, :, 0]), s[:2], order=order, niter=6))
