@@ -1009,3 +1009,166 @@ SH_HD void predict_adj_factored(const Acc &in, const Acc &q, bool rev, const flo
     }
     pred_adj_apply<NW>(Acc{y.p, y.s}, q, rev, n1, e, out);
 }
+
+// predict_adj_factored with its loads issued D samples ahead into register
+// rings (loops unrolled by D; ring indices clamped into the trace and used
+// only where the unclamped index is valid), and with the input formed on
+// the fly as in[k] = (a[k] + b[k]) when b is given (the adjoint spray's
+// running trace plus the slot it gathers, added in that order).  Same
+// float operations in the same order as predict_adj_factored; y holds the
+// solved trace, out may alias a.
+template <int NW, int D>
+SH_HD void predict_adj_factored_pf(const Acc &a, const Acc *b, const Acc &q, bool rev, const float *F, long fs,
+                                   long fplane, Sink y, int n1, float e, Sink out) {
+    constexpr int NB = 2 * NW, NA = 2 * NW + 1;
+    // forward substitution: ring of d[k], o[m][k] and the input at k
+    {
+        float Lo[NB][NB], Bw[NB];
+#pragma unroll
+        for (int r = 0; r < NB; r++) {
+            Bw[r] = 0.0f;
+#pragma unroll
+            for (int m = 0; m < NB; m++) Lo[r][m] = 0.0f;
+        }
+        float rd[D], ro[D][NB], ri[D];
+        auto fetch = [&](int s, int k) {
+            const long kc = k < n1 ? k : n1 - 1;
+            rd[s] = F[kc * fs];
+#pragma unroll
+            for (int m = 0; m < NB; m++) ro[s][m] = F[(long)(1 + m) * fplane + kc * fs];
+            ri[s] = b ? (a[kc] + (*b)[kc]) : a[kc];
+        };
+        auto step = [&](int s, int k) {
+            const int m1 = k < NB ? k : NB;
+            float bb = ri[s];
+#pragma unroll
+            for (int m = 0; m < NB; m++)
+                if (m < m1) bb -= (Lo[m][m]) * Bw[m];
+            y[k] = bb / rd[s];
+#pragma unroll
+            for (int r = NB - 1; r > 0; r--) {
+                Bw[r] = Bw[r - 1];
+#pragma unroll
+                for (int m = 0; m < NB; m++) Lo[r][m] = Lo[r - 1][m];
+            }
+            Bw[0] = bb;
+#pragma unroll
+            for (int m = 0; m < NB; m++) Lo[0][m] = ro[s][m];
+            fetch(s, k + D);
+        };
+#pragma unroll
+        for (int s = 0; s < D; s++) fetch(s, s);
+        int k0 = 0;
+        for (; k0 + D <= n1; k0 += D) {
+#pragma unroll
+            for (int s = 0; s < D; s++) step(s, k0 + s);
+        }
+#pragma unroll
+        for (int s = 0; s < D; s++)
+            if (k0 + s < n1) step(s, k0 + s);
+    }
+    // backward substitution in place: ring of y[k], o[m][k]
+    {
+        float nx[NB];
+#pragma unroll
+        for (int m = 0; m < NB; m++) nx[m] = 0.0f;
+        float ry[D], rb[D][NB];
+        auto fetchb = [&](int s, int k) {
+            const long kc = k >= 0 ? k : 0;
+            ry[s] = y[kc];
+#pragma unroll
+            for (int m = 0; m < NB; m++) rb[s][m] = F[(long)(1 + m) * fplane + kc * fs];
+        };
+        auto stepb = [&](int s, int k) {
+            float t = ry[s];
+            const int m1 = (n1 - k - 1) < NB ? (n1 - k - 1) : NB;
+#pragma unroll
+            for (int m = 0; m < NB; m++)
+                if (m < m1) t -= rb[s][m] * nx[m];
+#pragma unroll
+            for (int m = NB - 1; m > 0; m--) nx[m] = nx[m - 1];
+            nx[0] = t;
+            y[k] = t;
+            fetchb(s, k - D);
+        };
+#pragma unroll
+        for (int s = 0; s < D; s++) fetchb(s, n1 - 1 - s);
+        int kb = n1 - 1;
+        for (; kb - D + 1 >= 0; kb -= D) {
+#pragma unroll
+            for (int s = 0; s < D; s++) stepb(s, kb - s);
+        }
+#pragma unroll
+        for (int s = 0; s < D; s++)
+            if (kb - s >= 0) stepb(s, kb - s);
+    }
+    // out = A^T A s + e * boundary(s) (pred_adj_apply): ring of the slope
+    // at k + 1 + NW and the solved sample at k + 1 + 2 NW
+    {
+        const Acc s{y.p, y.s};
+        float A[NA][NA], S[NA], T[NA];
+#pragma unroll
+        for (int w = 0; w < NA; w++) taps_at<NW>(q, w - NW, n1, rev, A[w]);
+#pragma unroll
+        for (int w = 0; w < NA; w++) S[w] = (w < n1) ? s[w] : 0.0f;
+#pragma unroll
+        for (int w = 0; w < NA; w++) {
+            const int kk = w - NW;
+            float t = 0.0f;
+            if (kk >= NW && kk < n1 - NW) {
+#pragma unroll
+                for (int j = NA - 1; j >= 0; j--) t += A[w][j] * s[kk - j + NW];
+            }
+            T[w] = t;
+        }
+        const float s0 = s[0], s1 = s[1], s2 = s[n1 - 2], s3 = s[n1 - 1];
+        float rq[D], rs[D];
+        auto fetcha = [&](int r, int k) {
+            const int kq = k + 1 + NW < n1 ? k + 1 + NW : n1 - 1;
+            const int ks = k + 1 + 2 * NW < n1 ? k + 1 + 2 * NW : n1 - 1;
+            rq[r] = q[kq];
+            rs[r] = s[ks];
+        };
+        auto stepa = [&](int r, int k) {
+            float v = 0.0f;
+#pragma unroll
+            for (int w = 0; w < NA; w++) {
+                const int i = k - NW + w;
+                if (i >= NW && i < n1 - NW) v += A[w][2 * NW - w] * T[w];
+            }
+            if (k == 0) v += e * s0;
+            if (k == 1) v += e * s1;
+            if (k == n1 - 2) v += e * s2;
+            if (k == n1 - 1) v += e * s3;
+            out[k] = v;
+#pragma unroll
+            for (int w = 0; w < NA - 1; w++) {
+#pragma unroll
+                for (int j = 0; j < NA; j++) A[w][j] = A[w + 1][j];
+                T[w] = T[w + 1];
+                S[w] = S[w + 1];
+            }
+            taps_from<NW>(rq[r], k + 1 + NW, n1, rev, A[NA - 1]);
+            const int kn = k + 1 + 2 * NW;
+            S[NA - 1] = (kn < n1) ? rs[r] : 0.0f;
+            const int kk = k + 1 + NW;
+            float tn = 0.0f;
+            if (kk >= NW && kk < n1 - NW) {
+#pragma unroll
+                for (int j = NA - 1; j >= 0; j--) tn += A[NA - 1][j] * S[2 * NW - j];
+            }
+            T[NA - 1] = tn;
+            fetcha(r, k + D);
+        };
+#pragma unroll
+        for (int r = 0; r < D; r++) fetcha(r, r);
+        int k0 = 0;
+        for (; k0 + D <= n1; k0 += D) {
+#pragma unroll
+            for (int r = 0; r < D; r++) stepa(r, k0 + r);
+        }
+#pragma unroll
+        for (int r = 0; r < D; r++)
+            if (k0 + r < n1) stepa(r, k0 + r);
+    }
+}

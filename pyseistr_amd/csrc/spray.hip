@@ -419,10 +419,10 @@ spray2_adj(const float *__restrict__ U, const float *__restrict__ dip, const flo
         if (ip < 0 || ip >= n2) continue;
         int slot = dir == 1 ? ns + is + 1 : ns - is - 1;
         Acc u{U + (long)slot * plane_slot + i3 * slice + ip, n2};
-        for (int k = 0; k < n1; k++) t[k] = t[k] + u[k];
         Acc q{ds + (dir == 1 ? ip - 1 : ip), n2};
         const float *Fv = F + (long)(dir == 1 ? 1 : 0) * (NB + 1) * n + i3 * slice + ip;
-        predict_adj_factored<NW>(Acc{t.p, t.s}, q, dir == 1, Fv, n2, n, y, n1, e, t);
+        // t = S^{-1}-adjoint prediction of (t + u), loads issued 8 samples ahead
+        predict_adj_factored_pf<NW, 8>(Acc{t.p, t.s}, &u, q, dir == 1, Fv, n2, n, y, n1, e, t);
     }
     Sink c{C + (long)dir * plane_slot + i3 * slice + i, n2};
     for (int k = 0; k < n1; k++) c[k] = t[k];
