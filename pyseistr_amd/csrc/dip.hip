@@ -612,7 +612,8 @@ int smooth_apply(sh_ctx *ctx, const DipGeom &g, const SmoothPlan &sp, const floa
             // tile: tw i1-samples x chs samples along the axis (+ halos);
             // SH_SMOOTH_TILE=TWxCHS overrides (tw in {32, 64})
             int tw = SM_TW, chs = SM_CHS;
-            if (const char *te = getenv("SH_SMOOTH_TILE")) {
+            const char *te = getenv("SH_SMOOTH_TILE");
+            if (te) {
                 int a = 0, b = 0;
                 if (sscanf(te, "%dx%d", &a, &b) == 2 && (a == 32 || a == 64) && b >= 8 && b <= 512) {
                     tw = a;
@@ -620,6 +621,14 @@ int smooth_apply(sh_ctx *ctx, const DipGeom &g, const SmoothPlan &sp, const floa
                 }
             }
             long ntask = (long)ceil_div(g.n1, tw) * ceil_div(naxis, chs) * nother;
+            // small volumes (a 2-D gather): narrower, then shorter tiles
+            // until the launch has a block for every SIMD (the stencils are
+            // the same; only the CG sums' block partition changes)
+            while (!te && ntask < 1024 && (tw > 32 || chs > 16)) {
+                if (tw > 32) tw = 32;
+                else chs /= 2;
+                ntask = (long)ceil_div(g.n1, tw) * ceil_div(naxis, chs) * nother;
+            }
             dim3 grid((unsigned)std::min<long>(ntask, DIP_GRID_CAP));
             nblk = grid.x;
             size_t lds = (size_t)(chs + 2 * (nb - 1)) * tw * sizeof(float);

@@ -349,8 +349,13 @@ template <int NW>
 __global__ void __launch_bounds__(TB)
 spray2_fwd(const float *__restrict__ x, const float *__restrict__ dip, const float *__restrict__ F,
            float *__restrict__ U, int n1, int n2, int n3, int ns, float e, float *__restrict__ scr, long nthr,
-           const int *stop) {
+           const int *stop, const float *__restrict__ x2, float *__restrict__ U2) {
     if (stop && *(volatile const int *)stop) return;
+    if (blockIdx.z) {  // the second spray of the launch: its own input, output and scratch plane
+        x = x2;
+        U = U2;
+        scr += nthr * (long)n1;
+    }
     long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;  // over n2 * 2
     int i3 = blockIdx.y;
     if (gid >= 2L * n2) return;
@@ -736,13 +741,16 @@ int pws2_prepare(sh_ctx *ctx, Pwsmooth2 &P, int slot_scr, int slot_F) {
     return pws2_prepare_bufs(ctx, P, scr, F);
 }
 
-int pws2_spray(sh_ctx *ctx, Pwsmooth2 &P, const float *x) {
-    dim3 grid(ceil_div(2L * P.n2, TB2), P.n3);
+// Spray of x into P.U; with x2 also the spray of x2 into U2 in the same
+// launch (it uses the scratch plane P.tr, free outside the adjoint), so
+// two latency-bound sets of chains run side by side.
+int pws2_spray(sh_ctx *ctx, Pwsmooth2 &P, const float *x, const float *x2 = nullptr, float *U2 = nullptr) {
+    dim3 grid(ceil_div(2L * P.n2, TB2), P.n3, x2 ? 2 : 1);
     PROF_A(ctx);
     SH_DISPATCH_NW(ctx, P.order, NW,
                    hipLaunchKernelGGL((spray2_fwd<NW>), grid, dim3(TB2), 0, ctx->stream, x, P.dip, P.F, P.U, P.n1,
-                                      P.n2, P.n3, P.ns, P.e, P.scr, P.nthr, P.stop));
-    PROF_B(ctx, "spray2_fwd", 12.0 * 2 * P.ns * (double)P.n1 * P.n2 * P.n3);
+                                      P.n2, P.n3, P.ns, P.e, P.scr, P.nthr, P.stop, x2, U2));
+    PROF_B(ctx, "spray2_fwd", (x2 ? 2 : 1) * 12.0 * 2 * P.ns * (double)P.n1 * P.n2 * P.n3);
     return sh_launch_check(ctx, "spray2_fwd");
 }
 
@@ -763,6 +771,32 @@ int pws2_setup(sh_ctx *ctx, Pwsmooth2 &P, int slot_base) {
     SH_TRY(pws2_spray(ctx, P, ones));  // writes every slot
     hipLaunchKernelGGL(pwsmooth_norm, dim3(g), dim3(TB), 0, ctx->stream, P.U, P.ww, n, P.ns);
     return sh_launch_check(ctx, "pwsmooth_norm");
+}
+
+// pws2_setup + the forward smoothing of x in one pass: the spray of ones
+// (normalisation) and the spray of x run in one launch, each into its own
+// slot volume; results identical to pws2_setup then pws2_apply(adj = 0).
+int pws2_setup_forward(sh_ctx *ctx, Pwsmooth2 &P, int slot_base, const float *x, float *smooth) {
+    long n = (long)P.n1 * P.n2 * P.n3;
+    int ns2 = 2 * P.ns + 1;
+    float *b = nullptr;
+    SH_TRY(sh_slot(ctx, slot_base, (size_t)n * (2 * ns2 + 3) * sizeof(float), (void **)&b));
+    P.U = b;
+    float *U1 = b + (long)ns2 * n;
+    P.ww = b + (long)(2 * ns2) * n;
+    P.C = b + (long)(2 * ns2 + 1) * n;
+    SH_TRY(pws2_prepare(ctx, P, slot_base + 1, slot_base + 2));
+    float *ones = P.C;
+    int g = ceil_div(n, TB);
+    hipLaunchKernelGGL(fill_one, dim3(g), dim3(TB), 0, ctx->stream, ones, n);
+    SH_TRY(sh_launch_check(ctx, "fill_one"));
+    SH_TRY(pws2_spray(ctx, P, x, ones, U1));
+    hipLaunchKernelGGL(pwsmooth_norm, dim3(g), dim3(TB), 0, ctx->stream, U1, P.ww, n, P.ns);
+    SH_TRY(sh_launch_check(ctx, "pwsmooth_norm"));
+    PROF_A(ctx);
+    hipLaunchKernelGGL(pwsmooth_fwd, dim3(g), dim3(TB), 0, ctx->stream, P.U, P.ww, smooth, n, P.ns, 0);
+    PROF_B(ctx, "pwsmooth_fwd", 4.0 * (ns2 + 2) * (double)n);
+    return sh_launch_check(ctx, "pwsmooth_fwd");
 }
 
 // smooth (+)= S trace  or  trace (+)= S^T smooth   (pwsmooth_lop)
@@ -972,10 +1006,10 @@ int shd_somean2d(sh_ctx *ctx, const float *dn, const float *dip, int n1, int n2,
         SH_TRY(transpose(ctx, dn + (long)i3 * n1 * n2, xT + (long)i3 * n1 * n2, n1, n2, true));
     }
     Pwsmooth2 P{n1, n2, n3, ns, order, eps * eps, dT};
-    SH_TRY(pws2_setup(ctx, P, SLOT_SPRAY0 + 6));
     if (!adj) {
-        SH_TRY(pws2_apply(ctx, P, 0, 0, xT, yT));
+        SH_TRY(pws2_setup_forward(ctx, P, SLOT_SPRAY0 + 6, xT, yT));
     } else {
+        SH_TRY(pws2_setup(ctx, P, SLOT_SPRAY0 + 6));
         SH_TRY(pws2_apply(ctx, P, 1, 0, yT, xT));
     }
     for (int i3 = 0; i3 < n3; i3++)
