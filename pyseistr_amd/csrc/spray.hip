@@ -910,6 +910,136 @@ __global__ void svmf_traces(const float *__restrict__ U, float *__restrict__ out
     svmf_one(U, out, n, np, n1, nfw, base, stride);
 }
 
+// The space-varying median of the 3-D filter with its work spread over
+// the whole grid (svmf, sof3d_cfuns.c:1254-1352), in three launches:
+//  1. every first-pass median (slot s, sample k, trace t): the order
+//     statistic of the nfw + 2 columns around s (clamped into the panel),
+//     by a selection network of N >= nfw + 2 registers padded with -inf and
+//     +inf; its magnitude goes to a[s][k][t];
+//  2. per trace the float mean of those magnitudes, summed slot by slot and
+//     sample by sample as the reference does (one thread per trace, eight
+//     loads issued ahead of the additions);
+//  3. per (sample, trace) the window length from |x| against the mean and
+//     the median of that window, by the same padded network.
+// Each step is exact, so the result equals svmf_one's.
+template <int N>
+__global__ void __launch_bounds__(TB)
+svmf_first(const float *__restrict__ U, float *__restrict__ a, long n, int np, int n1, int nfw, long ntr, long total) {
+    const int nf = nfw + 2, m2 = (nfw - 1) / 2, pad = (N - nf) / 2;
+    for (long g = (long)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += (long)gridDim.x * blockDim.x) {
+        const long t = g % ntr;
+        const long r = g / ntr;
+        const int k = (int)(r % n1), s = (int)(r / n1);
+        float v[N];
+#pragma unroll
+        for (int q = 0; q < N; q++) {
+            if (q < nf) {
+                int col = s + q - m2;
+                col = col < 0 ? 0 : (col > np - 1 ? np - 1 : col);
+                v[q] = U[(long)col * n + (long)k * ntr + t];
+            } else {
+                v[q] = (q - nf < pad) ? -INFINITY : INFINITY;
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < MedianNet<N>::NOPS; c++) {
+            const int i0 = MedianNet<N>::OPS[c][0], i1 = MedianNet<N>::OPS[c][1];
+            const float x = v[i0], y = v[i1];
+            v[i0] = fminf(x, y);
+            v[i1] = fmaxf(x, y);
+        }
+        a[g] = fabsf(v[MedianNet<N>::MED]);
+    }
+}
+
+__global__ void __launch_bounds__(TB)
+svmf_mean(const float *__restrict__ a, float *__restrict__ avg, int np, int n1, long ntr) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntr) return;
+    const long cnt = (long)np * n1;
+    float sum = 0.0f;
+    long i = 0;
+    for (; i + 8 <= cnt; i += 8) {
+        float r[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) r[u] = a[(i + u) * ntr + t];
+#pragma unroll
+        for (int u = 0; u < 8; u++) sum = sum + r[u];
+    }
+    for (; i < cnt; i++) sum = sum + a[i * ntr + t];
+    avg[t] = sum / (float)(n1 * np);
+}
+
+template <int N>
+__global__ void __launch_bounds__(TB)
+svmf_final(const float *__restrict__ U, const float *__restrict__ avg, float *__restrict__ out, long n, int np,
+           int nfw, long ntr) {
+    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    const long t = g % ntr;
+    const long k = g / ntr;
+    const int c = (np - 1) / 2;
+    const float av = fabsf(U[(long)c * n + g]), mean = avg[t];
+    int len;
+    if (av < mean) len = (av < mean / 2) ? nfw + 2 : nfw;
+    else len = (av > mean * 2) ? nfw - 4 : nfw - 2;
+    const int h = (len - 1) / 2, pad = (N - len) / 2;
+    float v[N];
+#pragma unroll
+    for (int q = 0; q < N; q++) {
+        if (q < len) {
+            int col = c + q - h;
+            col = col < 0 ? 0 : (col > np - 1 ? np - 1 : col);
+            v[q] = U[(long)col * n + k * ntr + t];
+        } else {
+            v[q] = (q - len < pad) ? -INFINITY : INFINITY;
+        }
+    }
+#pragma unroll
+    for (int cc = 0; cc < MedianNet<N>::NOPS; cc++) {
+        const int i0 = MedianNet<N>::OPS[cc][0], i1 = MedianNet<N>::OPS[cc][1];
+        const float x = v[i0], y = v[i1];
+        v[i0] = fminf(x, y);
+        v[i1] = fmaxf(x, y);
+    }
+    out[g] = v[MedianNet<N>::MED];
+}
+
+// svmf over the trace-fast slot volume U[np][n1][ntr] into out[n1][ntr];
+// the one-thread-per-trace kernel where the window exceeds the networks.
+int svmf3(sh_ctx *ctx, const float *U, float *out, int n1, long ntr, int np, int nfw) {
+    const long n = (long)n1 * ntr;
+    const int nf = nfw + 2;
+    if (nf > 81) {
+        hipLaunchKernelGGL(svmf_traces, dim3(ceil_div(ntr, 64)), dim3(64), 0, ctx->stream, U, out, n, np, n1, nfw, ntr,
+                           1, 0);
+        return sh_launch_check(ctx, "svmf_traces");
+    }
+    float *a = nullptr;
+    SH_TRY(sh_slot(ctx, SLOT_SPRAY0 + 10, ((size_t)np * n + ntr) * sizeof(float), (void **)&a));
+    float *avg = a + (long)np * n;
+    const long total = (long)np * n;
+    const int g1 = (int)std::min<long>(ceil_div(total, TB), 1L << 20);
+    {
+        PROF_A(ctx);
+#define SV1(NN) \
+    if (nf <= NN) { hipLaunchKernelGGL((svmf_first<NN>), dim3(g1), dim3(TB), 0, ctx->stream, U, a, n, np, n1, nfw, ntr, total); } else
+        SV1(9) SV1(13) SV1(17) SV1(33) SV1(49) SV1(65) SV1(81) {}
+#undef SV1
+        PROF_B(ctx, "svmf_first", 4.0 * (double)(np + 1) * n);
+    }
+    SH_TRY(sh_launch_check(ctx, "svmf_first"));
+    hipLaunchKernelGGL(svmf_mean, dim3(ceil_div(ntr, TB)), dim3(TB), 0, ctx->stream, a, avg, np, n1, ntr);
+    SH_TRY(sh_launch_check(ctx, "svmf_mean"));
+    PROF_A(ctx);
+#define SV3(NN) \
+    if (nf <= NN) { hipLaunchKernelGGL((svmf_final<NN>), dim3(ceil_div(n, TB)), dim3(TB), 0, ctx->stream, U, avg, out, n, np, nfw, ntr); } else
+    SV3(9) SV3(13) SV3(17) SV3(33) SV3(49) SV3(65) SV3(81) {}
+#undef SV3
+    PROF_B(ctx, "svmf_final", 4.0 * (double)(nf + 1) * n);
+    return sh_launch_check(ctx, "svmf_final");
+}
+
 }  // namespace
 
 extern "C" {
@@ -933,9 +1063,7 @@ int shd_somf3d(sh_ctx *ctx, const float *din, const float *dipi, const float *di
     const float e = 0.01f * 0.01f;  // eps forced to 0.01 (sof3d_cfuns.c:1586)
     if (option == 2) {  // its first pass reads every slot
         SH_TRY(spray3_run(ctx, order, n1, n2, n3, r1, r2, e, dT, piT, pxT, U, nullptr, nullptr));
-        hipLaunchKernelGGL(svmf_traces, dim3(ceil_div(ntr, 64)), dim3(64), 0, ctx->stream, U, dT, n, np, n1, nmf, ntr,
-                           n2, 0);
-        SH_TRY(sh_launch_check(ctx, "svmf_traces"));
+        SH_TRY(svmf3(ctx, U, dT, n1, ntr, np, nmf));
         return transpose(ctx, dT, out, n1, ntr, false);
     }
     int c = (np - 1) / 2, m = (nmf - 1) / 2;
