@@ -69,6 +69,8 @@ def parse():
     ap.add_argument("--no-batch2d", action="store_true", help="skip the batched 2-D side measurement")
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the side measurements of the 2-D gather, 3-D volume and interpolation workloads")
+    ap.add_argument("--only", choices=["gather_2d", "volume_3d", "interp_3d"],
+                    help="run only that side workload and print its entry (profiling runs)")
     ap.add_argument("--no-neighbours", action="store_true",
                     help="skip the side measurements of smoothc, bandpassc, patch3d / patch3d_inv and fkdip")
     return ap.parse_args()
@@ -321,7 +323,7 @@ def device_time(torch, fn, reps=3):
                 "achieved_GB/s": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
 
 
-def side_configs(torch, cpu=True):
+def side_configs(torch, cpu=True, only=None):
     """The path's other workloads, device-resident, each with its samples/s,
     dominant plain kernel (algorithmic GB/s against 8 TB/s) and a one-core
     CPU leg of the FP32 restatement (pinned, median of 3 where one run
@@ -337,14 +339,27 @@ def side_configs(torch, cpu=True):
     import numpy as np
 
     from oracle import oracle as O
-    from pyseistr_amd import dip, filters, interp, synth
     F = lambda a: np.ascontiguousarray(np.asarray(a, np.float32).ravel(order="F"))  # noqa: E731
     dev = lambda a: torch.from_numpy(F(a)).cuda()  # noqa: E731
     res = {}
 
+    want = (lambda k: only is None or k == only)
     # 2-D gather
     s = (512, 256, 1)
     n = int(np.prod(s))
+    if want("gather_2d"):
+        res["gather_2d"] = _gather_2d(torch, O, F, dev, s, n, cpu)
+    if want("volume_3d"):
+        res["volume_3d"] = _volume_3d(torch, O, F, dev, cpu)
+    if want("interp_3d"):
+        res["interp_3d"] = _interp_3d(torch, O, F, dev, cpu)
+    return res
+
+
+def _gather_2d(torch, O, F, dev, s, n, cpu):
+    import numpy as np
+
+    from pyseistr_amd import dip, filters, synth
     g = synth.gather2d(512, 256, seed=1)
     gd, gp, go = dev(g), torch.empty(n, device="cuda"), torch.empty(n, device="cuda")
 
@@ -362,9 +377,13 @@ def side_configs(torch, cpu=True):
         e["cpu"] = {"samples_per_s": round(n / t, 1), "kind": "port", "cores": 1,
                     "sample": f"the same 512x256 gather, median of {r}, {t:.3f} s"}
         e["gpu_over_cpu"] = round(t / (ms / 1e3), 1)
-    res["gather_2d"] = e
+    return e
 
-    # 3-D volume
+
+def _volume_3d(torch, O, F, dev, cpu):
+    import numpy as np
+
+    from pyseistr_amd import dip, filters, synth
     s = (256, 128, 128)
     n = int(np.prod(s))
     v, _ = synth.volume3d(s, seed=2, n_events=4, smax=2.0, sigma=0.3, spike_frac=0.01)
@@ -388,10 +407,13 @@ def side_configs(torch, cpu=True):
         t, r = pinned(c2cpu)
         e["cpu"] = {"samples_per_s": round(nc / t, 1), "kind": "port", "cores": 1,
                     "sample": f"128x64x64 of the same generator, median of {r}, {t:.2f} s"}
-    res["volume_3d"] = e
-    del vd, vp, vo
+    return e
 
-    # interpolation
+
+def _interp_3d(torch, O, F, dev, cpu):
+    import numpy as np
+
+    from pyseistr_amd import dip, interp, synth
     s = (300, 100, 100)
     n = int(np.prod(s))
     obs, mask, clean = synth.interp3d(s)
@@ -427,8 +449,7 @@ def side_configs(torch, cpu=True):
         e["sint3d(niter=20, ns1=ns2=3, order=2)"]["cpu"] = {
             "samples_per_s": round(nc / t, 1), "kind": "port", "cores": 1,
             "sample": f"120x40x40, median of {r}, {t:.2f} s"}
-    res["interp_3d"] = e
-    return res
+    return e
 
 
 def measured_traffic(name, path=None):
@@ -468,6 +489,10 @@ def main():
     import torch
 
     from pyseistr_amd import _lib, dip, filters, synth, warmstart
+    if args.only:
+        print(json.dumps({"workload": args.only,
+                          "result": side_configs(torch, cpu=not args.no_cpu, only=args.only)[args.only]}), flush=True)
+        return 0
 
     shape = tuple(int(s) for s in args.shape.split(","))
     n = int(np.prod(shape))

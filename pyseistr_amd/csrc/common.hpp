@@ -264,6 +264,8 @@ struct Pwsmooth2 {
 int pws2_setup(sh_ctx *ctx, Pwsmooth2 &P, int slot_base);
 int pws2_prepare(sh_ctx *ctx, Pwsmooth2 &P, int slot_scr, int slot_F);
 int pws2_apply(sh_ctx *ctx, Pwsmooth2 &P, int adj, int add, float *trace, float *smooth);
-int sh_transpose_internal(sh_ctx *ctx, const float *src, float *dst, int n1, long ntr, bool to_fast);
+// nslice slices of n1 * ntr values back to back, each transposed alike
+int sh_transpose_internal(sh_ctx *ctx, const float *src, float *dst, int n1, long ntr, bool to_fast,
+                          int nslice = 1);
 // Median of slots [first, first+nmf) of each of n samples (slot-major U).
 int sh_median_internal(sh_ctx *ctx, const float *U, float *out, long n, int first, int nmf);

@@ -1315,13 +1315,11 @@ static int sint_common(sh_ctx *ctx, const float *din, const float *dipi, const f
     float *dT = b, *mT = b + n, *pT = b + 2 * n, *xT = b + 3 * n, *diT = b + 4 * n, *dxX = b + 5 * n, *t1 = b + 6 * n,
           *t2 = b + 7 * n;
     unsigned char *known = (unsigned char *)(b + 8 * n);
-    for (int i3 = 0; i3 < n3; i3++) {
-        long o = (long)i3 * n1 * n2;
-        SH_TRY(sh_transpose_internal(ctx, din + o, dT + o, n1, n2, true));
-        SH_TRY(sh_transpose_internal(ctx, mask + o, mT + o, n1, n2, true));
-        SH_TRY(sh_transpose_internal(ctx, dipi + o, diT + o, n1, n2, true));
-        if (three) SH_TRY(sh_transpose_internal(ctx, dipx + o, t1 + o, n1, n2, true));
-    }
+    // per-slice transposes [i3][n2][n1] -> [i3][n1][n2], all slices in one launch
+    SH_TRY(sh_transpose_internal(ctx, din, dT, n1, n2, true, n3));
+    SH_TRY(sh_transpose_internal(ctx, mask, mT, n1, n2, true, n3));
+    SH_TRY(sh_transpose_internal(ctx, dipi, diT, n1, n2, true, n3));
+    if (three) SH_TRY(sh_transpose_internal(ctx, dipx, t1, n1, n2, true, n3));
     if (three) SH_TRY(permute(ctx, t1, dxX, n1, n2, n3, 0));
     hipLaunchKernelGGL(known_kernel, dim3(nb), dim3(TB), 0, ctx->stream, mT, known, n);
     // lam = sqrt(#known / N) (soint3d_cfuns.c:2586-2595)
@@ -1345,10 +1343,7 @@ static int sint_common(sh_ctx *ctx, const float *din, const float *dipi, const f
     }
     SH_HIP(ctx, hipMemcpyAsync(pT, dT, sizeof(float) * n, hipMemcpyDeviceToDevice, ctx->stream));
     SH_TRY(sint_cg(ctx, Sh, known, pT, xT, dT, lam, (float)(10 * 1.19209290E-07F), niter, iters));
-    for (int i3 = 0; i3 < n3; i3++) {
-        long o = (long)i3 * n1 * n2;
-        SH_TRY(sh_transpose_internal(ctx, xT + o, out + o, n1, n2, false));
-    }
+    SH_TRY(sh_transpose_internal(ctx, xT, out, n1, n2, false, n3));
     return SH_OK;
 }
 

@@ -30,10 +30,13 @@ namespace {
 constexpr int TB = 256;
 
 // ---------------------------------------------------------------------------
-// Layout transposes: F-order [trace][n1] <-> trace-fast [n1][trace].
+// Layout transposes: F-order [trace][n1] <-> trace-fast [n1][trace], for
+// gridDim.z independent slices of n1 * ntr values back to back.
 // ---------------------------------------------------------------------------
 __global__ void to_trace_fast(const float *__restrict__ src, float *__restrict__ dst, int n1, long ntr) {
     __shared__ float tile[32][33];
+    src += (long)blockIdx.z * n1 * ntr;
+    dst += (long)blockIdx.z * n1 * ntr;
     long t0 = (long)blockIdx.x * 32;
     int k0 = blockIdx.y * 32;
     int tx = threadIdx.x, ty = threadIdx.y;  // 32 x 8
@@ -52,6 +55,8 @@ __global__ void to_trace_fast(const float *__restrict__ src, float *__restrict__
 
 __global__ void from_trace_fast(const float *__restrict__ src, float *__restrict__ dst, int n1, long ntr) {
     __shared__ float tile[32][33];
+    src += (long)blockIdx.z * n1 * ntr;
+    dst += (long)blockIdx.z * n1 * ntr;
     long t0 = (long)blockIdx.x * 32;
     int k0 = blockIdx.y * 32;
     int tx = threadIdx.x, ty = threadIdx.y;
@@ -68,8 +73,9 @@ __global__ void from_trace_fast(const float *__restrict__ src, float *__restrict
     }
 }
 
-int transpose(sh_ctx *ctx, const float *src, float *dst, int n1, long ntr, bool to_fast) {
-    dim3 grid((unsigned)((ntr + 31) / 32), (unsigned)((n1 + 31) / 32));
+int transpose(sh_ctx *ctx, const float *src, float *dst, int n1, long ntr, bool to_fast, int nslice = 1) {
+    if (nslice < 1 || nslice > 65535) return sh_fail(ctx, SH_EINVAL, "transpose: %d slices", nslice);
+    dim3 grid((unsigned)((ntr + 31) / 32), (unsigned)((n1 + 31) / 32), (unsigned)nslice);
     if (to_fast)
         hipLaunchKernelGGL(to_trace_fast, grid, dim3(32, 8), 0, ctx->stream, src, dst, n1, ntr);
     else
@@ -709,10 +715,14 @@ int stage3(sh_ctx *ctx, const float *din, const float *dipi, const float *dipx, 
 // Threads of the 2-D spray chains (two directions per trace and slice).
 long pws2_nthr(int n2, int n3) { return (long)ceil_div(2L * n2, TB) * TB * n3; }
 
-// The 2-D chain kernels run one thread per (trace, direction, slice), a
-// few thousand sequential chains: 64-thread blocks spread them over every
-// CU instead of filling a quarter of the chip with 256-thread blocks.
-constexpr int TB2 = 64;
+// Block size of the 2-D chain kernels (one thread per trace, direction and
+// slice); SH_SPRAY2_TB=64 or 256 overrides.
+int spray2_tb() {
+    const char *e = getenv("SH_SPRAY2_TB");
+    if (e && atoi(e) == 64) return 64;
+    if (e && atoi(e) == 256) return 256;
+    return 256;
+}
 
 // Factor cache of P into caller-provided buffers: scr holds 2*nthr*n1
 // floats (solver scratch, then the running trace planes), F 2*(2*order+1)*n.
@@ -722,6 +732,7 @@ int pws2_prepare_bufs(sh_ctx *ctx, Pwsmooth2 &P, float *scr, float *F) {
     P.scr = scr;
     P.tr = P.scr + (long)P.nthr * P.n1;
     P.F = F;
+    const int TB2 = spray2_tb();
     dim3 grid(ceil_div(2L * P.n2, TB2), P.n3);
     PROF_A(ctx);
     SH_DISPATCH_NW(ctx, P.order, NW,
@@ -745,6 +756,7 @@ int pws2_prepare(sh_ctx *ctx, Pwsmooth2 &P, int slot_scr, int slot_F) {
 // launch (it uses the scratch plane P.tr, free outside the adjoint), so
 // two latency-bound sets of chains run side by side.
 int pws2_spray(sh_ctx *ctx, Pwsmooth2 &P, const float *x, const float *x2 = nullptr, float *U2 = nullptr) {
+    const int TB2 = spray2_tb();
     dim3 grid(ceil_div(2L * P.n2, TB2), P.n3, x2 ? 2 : 1);
     PROF_A(ctx);
     SH_DISPATCH_NW(ctx, P.order, NW,
@@ -817,6 +829,7 @@ int pws2_apply(sh_ctx *ctx, Pwsmooth2 &P, int adj, int add, float *trace, float 
         PROF_B(ctx, "pwsmooth_adj_weight", 4.0 * (ns2 + 2) * (double)n);
     }
     SH_TRY(sh_launch_check(ctx, "pwsmooth_adj_weight"));
+    const int TB2 = spray2_tb();
     dim3 grid(ceil_div(2L * P.n2, TB2), P.n3);
     {
         PROF_A(ctx);
@@ -832,8 +845,8 @@ int pws2_apply(sh_ctx *ctx, Pwsmooth2 &P, int adj, int add, float *trace, float 
     return sh_launch_check(ctx, "spray2_adj_combine");
 }
 
-int sh_transpose_internal(sh_ctx *ctx, const float *src, float *dst, int n1, long ntr, bool to_fast) {
-    return transpose(ctx, src, dst, n1, ntr, to_fast);
+int sh_transpose_internal(sh_ctx *ctx, const float *src, float *dst, int n1, long ntr, bool to_fast, int nslice) {
+    return transpose(ctx, src, dst, n1, ntr, to_fast, nslice);
 }
 
 int sh_median_internal(sh_ctx *ctx, const float *U, float *out, long n, int first, int nmf) {
@@ -1109,6 +1122,7 @@ int shd_pwspray2d(sh_ctx *ctx, int adj, const float *dip, int n1, int n2, int ns
                                      (size_t)n1 * sizeof(float), (size_t)n2, hipMemcpyDeviceToDevice, ctx->stream));
         SH_TRY(transpose(ctx, tmp, P.U + (long)s * n, n1, n2, true));
     }
+    const int TB2 = spray2_tb();
     dim3 grid(ceil_div(2L * n2, TB2), 1);
     SH_DISPATCH_NW(ctx, order, NW,
                    hipLaunchKernelGGL((spray2_adj<NW>), grid, dim3(TB2), 0, ctx->stream, P.U, P.dip, P.F, P.C, n1, n2, 1,
@@ -1129,10 +1143,9 @@ int shd_somean2d(sh_ctx *ctx, const float *dn, const float *dip, int n1, int n2,
     SH_TRY(sh_slot(ctx, SLOT_SPRAY0 + 1, (size_t)n * 3 * sizeof(float), (void **)&b));
     float *xT = b, *dT = b + n, *yT = b + 2 * n;
     // slices [i3][n1][n2] are per-slice transposes of [i3][n2][n1]
-    for (int i3 = 0; i3 < n3; i3++) {
-        SH_TRY(transpose(ctx, dip + (long)i3 * n1 * n2, dT + (long)i3 * n1 * n2, n1, n2, true));
-        SH_TRY(transpose(ctx, dn + (long)i3 * n1 * n2, xT + (long)i3 * n1 * n2, n1, n2, true));
-    }
+    // slices [i3][n1][n2] are per-slice transposes of [i3][n2][n1]
+    SH_TRY(transpose(ctx, dip, dT, n1, n2, true, n3));
+    SH_TRY(transpose(ctx, dn, xT, n1, n2, true, n3));
     Pwsmooth2 P{n1, n2, n3, ns, order, eps * eps, dT};
     if (!adj) {
         SH_TRY(pws2_setup_forward(ctx, P, SLOT_SPRAY0 + 6, xT, yT));
@@ -1140,8 +1153,7 @@ int shd_somean2d(sh_ctx *ctx, const float *dn, const float *dip, int n1, int n2,
         SH_TRY(pws2_setup(ctx, P, SLOT_SPRAY0 + 6));
         SH_TRY(pws2_apply(ctx, P, 1, 0, yT, xT));
     }
-    for (int i3 = 0; i3 < n3; i3++)
-        SH_TRY(transpose(ctx, yT + (long)i3 * n1 * n2, out + (long)i3 * n1 * n2, n1, n2, false));
+    SH_TRY(transpose(ctx, yT, out, n1, n2, false, n3));
     return SH_OK;
 }
 
@@ -1199,10 +1211,9 @@ int somf2d_core(sh_ctx *ctx, const float *dn, const float *dip, int n1, int n2, 
     }
     float *xT = b, *dT = b + n, *yT = b + 2 * n;
     P.dip = dT;
-    for (int i3 = 0; i3 < n3; i3++) {
-        SH_TRY(transpose(ctx, dip + (long)i3 * n1 * n2, dT + (long)i3 * n1 * n2, n1, n2, true));
-        SH_TRY(transpose(ctx, dn + (long)i3 * n1 * n2, xT + (long)i3 * n1 * n2, n1, n2, true));
-    }
+    // slices [i3][n1][n2] are per-slice transposes of [i3][n2][n1]
+    SH_TRY(transpose(ctx, dip, dT, n1, n2, true, n3));
+    SH_TRY(transpose(ctx, dn, xT, n1, n2, true, n3));
     if (ws)
         SH_TRY(pws2_prepare_bufs(ctx, P, ws + w.b + w.U, ws + w.b + w.U + w.scr));
     else
@@ -1217,8 +1228,7 @@ int somf2d_core(sh_ctx *ctx, const float *dn, const float *dip, int n1, int n2, 
     } else {
         SH_TRY(launch_median(ctx, P.U, yT, n, ns - m, nmf));
     }
-    for (int i3 = 0; i3 < n3; i3++)
-        SH_TRY(transpose(ctx, yT + (long)i3 * n1 * n2, out + (long)i3 * n1 * n2, n1, n2, false));
+    SH_TRY(transpose(ctx, yT, out, n1, n2, false, n3));
     return SH_OK;
 }
 
