@@ -175,27 +175,41 @@ def _gen(args):
     return training_volume(seed, shape)
 
 
-def make_dataset(seeds, vol_shape, win, per_vol, target_niter, liter=10, rect=(5, 5, 5), pool=None, log=print):
+def make_dataset(seeds, vol_shape, win, per_vol, target_niter, liter=10, rect=(5, 5, 5), pool=None, log=print,
+                 threads=1):
     """Cut 64^3 windows (x, target dips) from seeded volumes; everything
-    stays on the device.  Returns list of (x[W], target[2W], xscale).
+    stays on the device.  Returns list of (x[W], target[2W], xscale), in
+    seed order.  Window positions follow from each volume's seed alone.
     pool: optional multiprocessing pool for the volume generator, created
-    before the process first touches the GPU."""
+    before the process first touches the GPU.  threads > 1 computes the
+    targets of that many volumes at once, each thread on a stream and
+    library context of its own (small volumes leave the GPU idle between
+    one volume's launches otherwise)."""
+    import queue
+    import threading
+
     import torch
     w1, w2, w3 = win
     wn = w1 * w2 * w3
-    out = []
-    rng = np.random.default_rng(12345)
+    n = int(np.prod(vol_shape))
     t0 = t_log = time.time()
     jobs = [(s, vol_shape) for s in seeds]
     gen = pool.imap(_gen, jobs, chunksize=2) if pool is not None else map(_gen, jobs)
-    for k, v in enumerate(gen):
-        d = torch.from_numpy(v.ravel(order="F")).cuda()
-        n = d.numel()
+    per_seed = [None] * len(seeds)
+    lock = threading.Lock()
+    done = [0]
+    errors = []
+    todo = queue.Queue(maxsize=4 * max(threads, 1))
+
+    def windows(k, v):
+        d = torch.from_numpy(v.ravel(order="F")).cuda(non_blocking=False)
         dips = dip_device(d, vol_shape, niter=target_niter, liter=liter, order=2, rect=rect)
         xs = input_scale(d)
         vol = d.view(vol_shape[2], vol_shape[1], vol_shape[0])
         di = dips[:n].view(vol_shape[2], vol_shape[1], vol_shape[0])
         dx = dips[n:].view(vol_shape[2], vol_shape[1], vol_shape[0])
+        rng = np.random.default_rng((12345, int(seeds[k])))
+        out = []
         for _ in range(per_vol):
             o1 = int(rng.integers(0, vol_shape[0] - w1 + 1))
             o2 = int(rng.integers(0, vol_shape[1] - w2 + 1))
@@ -205,10 +219,48 @@ def make_dataset(seeds, vol_shape, win, per_vol, target_niter, liter=10, rect=(5
             t = torch.cat([di[sl].contiguous().view(-1), dx[sl].contiguous().view(-1)])
             assert x.numel() == wn
             out.append((x, t, xs))
-        if (k + 1) % 25 == 0 or time.time() - t_log > 30:
+        return out
+
+    def worker():
+        stream = torch.cuda.Stream()
+        with torch.cuda.stream(stream):
+            while True:
+                item = todo.get()
+                if item is None:
+                    break
+                k, v = item
+                try:
+                    res = windows(k, v)
+                except Exception as e:  # reported by the caller
+                    with lock:
+                        errors.append(e)
+                    continue
+                with lock:
+                    per_seed[k] = res
+                    done[0] += 1
+        stream.synchronize()
+
+    pool_t = [threading.Thread(target=worker, daemon=True) for _ in range(max(threads, 1))]
+    for th in pool_t:
+        th.start()
+    for k, v in enumerate(gen):
+        if errors:
+            break
+        todo.put((k, v))
+        if time.time() - t_log > 30 or k + 1 == len(seeds):
             t_log = time.time()
-            log(f"dataset: {k + 1}/{len(seeds)} volumes, {len(out)} windows, {time.time() - t0:.0f} s")
-    return out
+            with lock:
+                nd = done[0]
+            log(f"dataset: {nd}/{len(seeds)} volumes, {nd * per_vol} windows, {time.time() - t0:.0f} s")
+    for _ in pool_t:
+        todo.put(None)
+    for th in pool_t:
+        th.join()
+    torch.cuda.synchronize()
+    if errors:
+        raise errors[0]
+    log(f"dataset: {done[0]}/{len(seeds)} volumes, {time.time() - t0:.0f} s")
+    return [w for res in per_seed for w in res]
 
 
 def window_weights(win, margin):
@@ -262,6 +314,11 @@ def main(argv=None):
     ap.add_argument("--seed0", type=int, default=10000)
     ap.add_argument("--out", type=str, default=WEIGHTS)
     ap.add_argument("--workers", type=int, default=8)
+    ap.add_argument("--threads", type=int, default=1, help="volumes whose targets are computed at once")
+    ap.add_argument("--minutes", type=float, default=None,
+                    help="wall-clock budget of the whole run: fewer steps if the dataset took long "
+                         "(the learning-rate schedule spans the steps that fit)")
+    ap.add_argument("--step-ms", type=float, default=3.2, help="expected time per step, for --minutes")
     ap.add_argument("--dilation", type=str, default="1,1,1,1,1,1", help="per-layer dilations")
     ap.add_argument("--window", type=int, default=64, help="cube edge of the training windows")
     ap.add_argument("--margin", type=int, default=4, help="voxels at each window face left out of the loss")
@@ -278,17 +335,22 @@ def main(argv=None):
     seeds = list(range(a.seed0, a.seed0 + a.volumes))
     t0 = time.time()
     win = (a.window,) * 3
-    data = make_dataset(seeds, vol_shape, win, a.per_vol, a.target_niter, pool=pool, log=log)
+    data = make_dataset(seeds, vol_shape, win, a.per_vol, a.target_niter, pool=pool, log=log, threads=a.threads)
     pool.close()
     log(f"dataset ready: {len(data)} windows in {time.time() - t0:.0f} s")
     dil = [int(v) for v in a.dilation.split(",")]
+    steps = a.steps
+    if a.minutes is not None:
+        left = a.minutes * 60 - (time.time() - t0) - 30
+        steps = max(100, min(steps, int(left / (a.step_ms / 1e3))))
+        log(f"{steps} steps fit the remaining {left:.0f} s")
     prior = WarmStart.load(a.init) if a.init else None
     if prior is not None and prior.dil != dil:
         raise SystemExit("--init weights have dilations %s, not %s" % (prior.dil, dil))
-    params, hist = train(data, a.steps, lr=a.lr, log=log, dil=dil, win=win, margin=a.margin,
+    params, hist = train(data, steps, lr=a.lr, log=log, dil=dil, win=win, margin=a.margin,
                          params=prior.params_host if prior is not None else None)
     meta = {"volumes": a.volumes, "vol_shape": vol_shape, "seeds": [a.seed0, a.seed0 + a.volumes - 1],
-            "windows": len(data), "per_vol": a.per_vol, "target_niter": a.target_niter, "steps": a.steps,
+            "windows": len(data), "per_vol": a.per_vol, "target_niter": a.target_niter, "steps": steps,
             "lr": a.lr, "dilation": dil, "window": a.window, "margin": a.margin,
             "final_loss": hist[-1][1] if hist else None,
             "loss_history": hist[::max(1, len(hist) // 50)]}
