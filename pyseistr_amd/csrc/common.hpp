@@ -190,6 +190,17 @@ SH_HD inline void apf_pass(float p, float *a) {
     }
 }
 
+// Tap k alone: the same products as apf_pass's a[k], so the same bits.
+template <int NW>
+SH_HD inline float apf_tap(int k, float p) {
+    constexpr ApfTable<NW> T;
+    constexpr int NF = 2 * NW;
+    double v = T.b[k];
+#pragma unroll
+    for (int j = 0; j < NF; j++) v *= apf_factor<NW>(k, j, p);
+    return (float)v;
+}
+
 template <int NW>
 SH_HD inline void apf_der(float p, float *a) {
     constexpr ApfTable<NW> T;

@@ -8,11 +8,11 @@
 //   csint3d / pwsmooth3_lop / ps_conjgrad (soint3d_cfuns.c:2231-2300,
 //   1222-1349, 2510-2615) and csint2d (soint2d_cfuns.c:2421-2538).
 //
-// Design: the PWD operator and its adjoint are gather kernels over
-// precomputed filter taps (the adjoint enumerates the contributions to each
-// output sample in the reference's scatter order, so both are bit-exact);
-// solver scalars are formed on the host from deterministic device
-// reductions, one small read-back per iteration.
+// Design: the PWD operator and its adjoint are gather kernels that form
+// each filter tap from the slope where they use it (the adjoint enumerates
+// the contributions to each output sample in the reference's scatter
+// order, so both are bit-exact); the solver scalars are formed on the
+// device from deterministic reductions (device-resident control below).
 #include "common.hpp"
 
 #include <algorithm>
@@ -34,29 +34,25 @@ struct Geo {
     int drift = 0;   // shift each stencil by NINT(p) along i1 (allpass "drift")
     int maxid = 0;   // max |NINT(p)| over both slope fields (adjoint search window)
     const int *ids = nullptr;  // NINT(p) per part and sample when drift
+    const float *slope[2] = {nullptr, nullptr};  // p (inline part), q (crossline part)
 };
 
 // PS_NINT (soint3d_cfuns.c:11): round half away from zero, in double
 __device__ inline int ps_nint(float p) { return (int)(p > 0.0 ? p + 0.5 : p - 0.5); }
 
-// taps[part][iw][i] = a_iw(p_part[i]) (apfilt, soint3d_cfuns.c:159-250)
-// with drift, the taps are those of p - NINT(p) and ids[i] = NINT(p)
-// (allpass3_lop, soint3d_cfuns.c:646-652)
-template <int NW>
-__global__ void taps_kernel(const float *__restrict__ pp, float *__restrict__ taps, long n, int drift,
-                            int *__restrict__ ids) {
+// Filter taps a_iw(p) (apfilt, soint3d_cfuns.c:159-250) are formed where
+// they are used, from the slope of the sample they belong to; with drift
+// they are those of p - NINT(p) (allpass3_lop, soint3d_cfuns.c:646-652),
+// and ids[i] = NINT(p) is kept for the shifted stencils.
+__global__ void nint_kernel(const float *__restrict__ pp, long n, int *__restrict__ ids) {
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    float a[2 * NW + 1];
-    float p = pp[i];
-    if (drift) {
-        const int id = ps_nint(p);
-        ids[i] = id;
-        p = p - id;
-    }
-    apf_pass<NW>(p, a);
-#pragma unroll
-    for (int iw = 0; iw <= 2 * NW; iw++) taps[(long)iw * n + i] = a[iw];
+    if (i < n) ids[i] = ps_nint(pp[i]);
+}
+
+// the slope whose taps a sample of a part uses
+__device__ inline float tap_slope(const Geo &g, int part_id, long i) {
+    const float p = g.slope[part_id][i];
+    return g.drift ? p - g.ids[(long)part_id * g.n + i] : p;
 }
 
 // y[part*n + i] = sum_iw (x[i+ip+is] - x[i-is]) a_iw(i)  (allpass3_lop fwd)
@@ -69,7 +65,7 @@ __device__ inline int solver_done(const SolverState *st);
 
 template <int NW>
 __global__ void __launch_bounds__(TB)
-pwd_fwd(const float *__restrict__ x, const float *__restrict__ taps, float *__restrict__ y,
+pwd_fwd(const float *__restrict__ x, float *__restrict__ y,
         const float *__restrict__ ss, const float *__restrict__ rr, Geo g, double *part, int nblk,
         const float *base, const SolverState *st) {
     __shared__ double sh[16];
@@ -93,11 +89,13 @@ pwd_fwd(const float *__restrict__ x, const float *__restrict__ taps, float *__re
         }
         float v = base ? base[i] : 0.0f;
         if (valid) {
-            const float *tp = taps + (long)part_id * (2 * NW + 1) * g.n;
+            const float p = g.slope[part_id][k];
+            float a[2 * NW + 1];
+            apf_pass<NW>(g.drift ? p - id : p, a);
 #pragma unroll
             for (int iw = 0; iw <= 2 * NW; iw++) {
                 int is = (iw - NW) * nj;
-                v += (x[k + ip + is] - x[k - is - id]) * tp[(long)iw * g.n + k];
+                v += (x[k + ip + is] - x[k - is - id]) * a[iw];
             }
         }
         y[i] = v;
@@ -123,7 +121,7 @@ pwd_fwd(const float *__restrict__ x, const float *__restrict__ taps, float *__re
 // first).  Partial sums: g.g.
 template <int NW>
 __global__ void __launch_bounds__(TB)
-pwd_adj(const float *__restrict__ rr, const float *__restrict__ taps, const unsigned char *__restrict__ known,
+pwd_adj(const float *__restrict__ rr, const unsigned char *__restrict__ known,
         float *__restrict__ gout, Geo g, double *part, const SolverState *st) {
     __shared__ double sh[16];
     if (st && solver_done(st)) return;
@@ -138,7 +136,6 @@ pwd_adj(const float *__restrict__ rr, const float *__restrict__ taps, const unsi
             int nj = part_id ? g.nj2 : g.nj1;
             int h = NW * nj;
             long ip = part_id ? (long)g.n1 * g.n2 : g.n1;
-            const float *tp = taps + (long)part_id * (2 * NW + 1) * g.n;
             const float *yy = rr + (long)part_id * g.n;
             const int *idp = g.drift ? g.ids + (long)part_id * g.n : nullptr;
             // with drift a source sample i contributes only when its shifted
@@ -157,7 +154,7 @@ pwd_adj(const float *__restrict__ rr, const float *__restrict__ taps, const unsi
                     int ix = kx - is;
                     if (ix >= h && ix < g.n1 - h) {
                         long i = k - ip - is;
-                        if (live(i, ix)) v += yy[i] * tp[(long)iw * g.n + i];
+                        if (live(i, ix)) v += yy[i] * apf_tap<NW>(iw, tap_slope(g, part_id, i));
                     }
                 }
             }
@@ -169,7 +166,7 @@ pwd_adj(const float *__restrict__ rr, const float *__restrict__ taps, const unsi
                     int ix = kx + is;
                     if (ix >= h && ix < g.n1 - h) {
                         long i = k + is;
-                        v -= yy[i] * tp[(long)iw * g.n + i];
+                        v -= yy[i] * apf_tap<NW>(iw, tap_slope(g, part_id, i));
                     }
                 }
             } else if (mrow) {
@@ -185,7 +182,7 @@ pwd_adj(const float *__restrict__ rr, const float *__restrict__ taps, const unsi
 #pragma unroll
                     for (int iw = 0; iw <= 2 * NW; iw++) {
                         int is = (iw - NW) * nj;
-                        if (ix - is - id == kx) v -= yy[i] * tp[(long)iw * g.n + i];
+                        if (ix - is - id == kx) v -= yy[i] * apf_tap<NW>(iw, tap_slope(g, part_id, i));
                     }
                 }
             }
@@ -316,10 +313,30 @@ __global__ void cgstep_update(float *__restrict__ S, const float *__restrict__ g
 // the total is valid in thread 0.
 __device__ inline double partials_total(const double *part, int nblk, int k, double *sh) {
     double s = 0.0;
+#pragma unroll 4
     for (int i = threadIdx.x; i < nblk; i += blockDim.x) s += part[(size_t)k * nblk + i];
     double t = block_sum(s, sh);
     __syncthreads();
     return t;
+}
+
+// K of them in one strided pass (the loads of all K in flight together);
+// each sum adds its terms in the order partials_total does, so the totals
+// are the same bits.
+template <int K>
+__device__ inline void partials_totals(const double *part, int nblk, double *v, double *sh) {
+    double s[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) s[k] = 0.0;
+#pragma unroll 2
+    for (int i = threadIdx.x; i < nblk; i += blockDim.x)
+#pragma unroll
+        for (int k = 0; k < K; k++) s[k] += part[(size_t)k * nblk + i];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        v[k] = block_sum(s[k], sh);
+        __syncthreads();
+    }
 }
 
 __global__ void __launch_bounds__(1024) solver_init(const double *part, int nblk, SolverState *st, int stop_if_zero) {
@@ -347,7 +364,7 @@ __global__ void __launch_bounds__(1024) cgstep_scalars(const double *pa, int nbg
     if (done) return;
     double v[7];
     v[0] = partials_total(pa, nbg, 0, sh);
-    for (int k = 0; k < 6; k++) v[k + 1] = partials_total(pb, nby, k, sh);
+    partials_totals<6>(pb, nby, v + 1, sh);
     if (threadIdx.x != 0) return;
     const double gnorm = v[0], gdg = v[1], gds = v[2], gdr0 = v[3], sds = v[4], sdr0 = v[5], rnorm = v[6];
     const double TOL = (double)1.e-12f, EPSILON = (double)1.e-12f;
@@ -465,12 +482,13 @@ int soint_solve(sh_ctx *ctx, const Geo &g0, const float *pp, const float *qq, co
                 const float *dd, int niter, int *iters_out) {
     Geo g = g0;
     long n = g.n, ny = g.n * g.nparts;
-    int NA = 2 * NW + 1;
     float *buf = nullptr;
-    // taps (nparts*NA*n), g, S (n each), rr, gg, Ss (ny each)
-    size_t nf = (size_t)g.nparts * NA * n + 2 * n + 3 * ny;
+    // g, S (n each), rr, gg, Ss (ny each)
+    size_t nf = 2 * n + 3 * ny;
     SH_TRY(sh_slot(ctx, SLOT_INTERP0, nf * sizeof(float), (void **)&buf));
-    float *taps = buf, *gv = taps + (size_t)g.nparts * NA * n, *S = gv + n, *rr = S + n, *gg = rr + ny, *Ss = gg + ny;
+    float *gv = buf, *S = gv + n, *rr = S + n, *gg = rr + ny, *Ss = gg + ny;
+    g.slope[0] = pp;
+    g.slope[1] = qq;
     int nbn = ceil_div(n, TB), nby = ceil_div(ny, TB);
     double *part = nullptr;
     SH_TRY(sh_slot(ctx, SLOT_RED, (size_t)(6 * nby + nbn) * sizeof(double), (void **)&part));
@@ -482,12 +500,10 @@ int soint_solve(sh_ctx *ctx, const Geo &g0, const float *pp, const float *qq, co
         SH_TRY(sh_slot(ctx, SLOT_INTERP0 + 9, ((size_t)g.nparts * n + 64) * sizeof(int), (void **)&ids));
     }
     int *dmax = ids ? ids + (size_t)g.nparts * n : nullptr;
-    hipLaunchKernelGGL((taps_kernel<NW>), dim3(nbn), dim3(TB), 0, ctx->stream, pp, taps, n, g.drift, ids);
-    if (g.nparts == 2)
-        hipLaunchKernelGGL((taps_kernel<NW>), dim3(nbn), dim3(TB), 0, ctx->stream, qq, taps + (size_t)NA * n, n,
-                           g.drift, ids ? ids + n : nullptr);
-    SH_TRY(sh_launch_check(ctx, "taps"));
     if (g.drift) {
+        hipLaunchKernelGGL(nint_kernel, dim3(nbn), dim3(TB), 0, ctx->stream, pp, n, ids);
+        if (g.nparts == 2) hipLaunchKernelGGL(nint_kernel, dim3(nbn), dim3(TB), 0, ctx->stream, qq, n, ids + n);
+        SH_TRY(sh_launch_check(ctx, "nint"));
         const long ni = (long)g.nparts * n;
         hipLaunchKernelGGL(fill_i, dim3(1), dim3(TB), 0, ctx->stream, dmax, 0, 1L);
         hipLaunchKernelGGL(maxabs_kernel, dim3(ceil_div(ni, TB)), dim3(TB), 0, ctx->stream, ids, ni, dmax);
@@ -503,7 +519,7 @@ int soint_solve(sh_ctx *ctx, const Geo &g0, const float *pp, const float *qq, co
     // rr = -dd, then rr += A x0 tap by tap (ps_solver 1018-1026); its
     // partial rr.rr starts the solver state
     hipLaunchKernelGGL(neg_kernel, dim3(nby), dim3(TB), 0, ctx->stream, dd, rr, ny);
-    hipLaunchKernelGGL((pwd_fwd<NW>), dim3(nby), dim3(TB), 0, ctx->stream, x, taps, rr, Ss, Ss, g, part, nby, rr,
+    hipLaunchKernelGGL((pwd_fwd<NW>), dim3(nby), dim3(TB), 0, ctx->stream, x, rr, Ss, Ss, g, part, nby, rr,
                        nullptr);
     hipLaunchKernelGGL(solver_init, dim3(1), dim3(1024), 0, ctx->stream, part, nby, st, 0);
     SH_TRY(sh_launch_check(ctx, "soint init"));
@@ -515,12 +531,12 @@ int soint_solve(sh_ctx *ctx, const Geo &g0, const float *pp, const float *qq, co
     auto iteration = [&]() -> int {
         {
             PROF_A(ctx);
-            hipLaunchKernelGGL((pwd_adj<NW>), dim3(nbn), dim3(TB), 0, ctx->stream, rr, taps, known, gv, g, pa, st);
+            hipLaunchKernelGGL((pwd_adj<NW>), dim3(nbn), dim3(TB), 0, ctx->stream, rr, known, gv, g, pa, st);
             PROF_B(ctx, "pwd_adj", 4.0 * (ny + g.nparts * n + n));
         }
         {
             PROF_A(ctx);
-            hipLaunchKernelGGL((pwd_fwd<NW>), dim3(nby), dim3(TB), 0, ctx->stream, gv, taps, gg, Ss, rr, g, part, nby,
+            hipLaunchKernelGGL((pwd_fwd<NW>), dim3(nby), dim3(TB), 0, ctx->stream, gv, gg, Ss, rr, g, part, nby,
                                nullptr, st);
             PROF_B(ctx, "pwd_fwd", 4.0 * (n + g.nparts * n + 3 * ny));
         }
@@ -719,7 +735,7 @@ __global__ void __launch_bounds__(1024) sint_scalars_s(const double *part, int n
     __syncthreads();
     if (done) return;
     double h[3];
-    for (int k = 0; k < 3; k++) h[k] = partials_total(part, nblk, k, sh);
+    partials_totals<3>(part, nblk, h, sh);
     if (threadIdx.x != 0) return;
     const double beta = h[0] + (double)eps * (h[1] - h[2]);
     st->b = -st->gn / beta;
