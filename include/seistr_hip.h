@@ -256,6 +256,7 @@ int sh_sint2d(sh_ctx *ctx, const float *din, const float *dip, const float *mask
  * patch gets exactly the per-patch algorithm; every pipeline stage is one
  * launch over the whole batch. */
 int sh_batch_tile_count(void);
+/* shape of tile cfg (0 .. count-1; -1: the default tile) */
 int sh_batch_tile_shape(int cfg, int *t1, int *t2);
 /* dip2dc per patch: dipc with n3 = 1 (pyseistr/src/dip_cfuns.c:1694-1989,
  * behind dip2dc, pyseistr/dip2d.py:115).  tile: smoothing tile shape

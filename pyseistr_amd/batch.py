@@ -123,9 +123,19 @@ def load_tuning(path=TUNING):
         return json.load(f)
 
 
-def pick_tile(shapes, tuning=None):
-    """Tile index with the least predicted time for this bucket mix (the
-    library default when no tuning table is present)."""
+def default_tile():
+    """Index of the library's default tile shape."""
+    a, b = ctypes.c_int(), ctypes.c_int()
+    _lib.load().sh_batch_tile_shape(-1, ctypes.byref(a), ctypes.byref(b))
+    return tile_shapes().index((a.value, b.value))
+
+
+def pick_tile(shapes, tuning=None, min_gain=0.03):
+    """Tile index with the least predicted time for this bucket mix; the
+    library default unless the table predicts at least min_gain less time
+    than the default for the mix (the tuned times of neighbouring shapes
+    are a few per cent apart, within the run-to-run spread of a batch),
+    and when no tuning table is present."""
     if tuning is None:
         tuning = load_tuning()
     if not tuning:
@@ -140,7 +150,9 @@ def pick_tile(shapes, tuning=None):
         else:  # nearest tuned bucket by size
             k = min(table, key=lambda s: abs(np.log(int(s.split("x")[0]) * int(s.split("x")[1]) / (n1 * n2))))
             cost += np.asarray(table[k]["ns_per_sample"]) * n1 * n2
-    return int(np.argmin(cost))
+    best = int(np.argmin(cost))
+    dflt = tuning["tiles"].index(list(tile_shapes()[default_tile()]))
+    return best if cost[best] <= (1.0 - min_gain) * cost[dflt] else dflt
 
 
 def _time_tiles(b, tiles, niter, liter, repeats):

@@ -492,6 +492,7 @@ extern "C" {
 int sh_batch_tile_count(void) { return NTILE_CFG; }
 
 int sh_batch_tile_shape(int cfg, int *t1, int *t2) {
+    if (cfg == -1) cfg = DEFAULT_TILE;
     if (cfg < 0 || cfg >= NTILE_CFG || !t1 || !t2) return SH_EINVAL;
     *t1 = TILE_CFG[cfg][0];
     *t2 = TILE_CFG[cfg][1];

@@ -84,15 +84,25 @@ __global__ void pack_weights(const float *__restrict__ P, __bf16 *__restrict__ W
 // read-only parameter vector, so they arrive through scalar loads and feed
 // the FMAs as scalar operands (no LDS traffic per FMA).
 __global__ void __launch_bounds__(TB) conv1_fwd(const float *__restrict__ x, float xscale, const float *__restrict__ P,
-                                                __bf16 *__restrict__ out, int n1, int n2, int n3, int dil) {
+                                                __bf16 *__restrict__ out, int n1, int n2, int n3, int dil,
+                                                FastDiv f1, FastDiv f2) {
     const float *__restrict__ w = P + OFF_W1;
     const float *__restrict__ b = P + OFF_B1;
     long n = (long)n1 * n2 * n3;
     long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= n) return;
-    int i1 = (int)(v % n1);
-    long r = v / n1;
-    int i2 = (int)(r % n2), i3 = (int)(r / n2);
+    int i1, i2, i3;
+    if (n < (1L << 31)) {  // multiply-high division (no long division per voxel)
+        const unsigned r = f1.div((unsigned)v), q = f2.div(r);
+        i1 = (int)((unsigned)v - r * (unsigned)n1);
+        i2 = (int)(r - q * (unsigned)n2);
+        i3 = (int)q;
+    } else {
+        i1 = (int)(v % n1);
+        const long r = v / n1;
+        i2 = (int)(r % n2);
+        i3 = (int)(r / n2);
+    }
     float xv[NT];
 #pragma unroll
     for (int t = 0; t < NT; t++) {
@@ -823,7 +833,7 @@ int cnn_forward(sh_ctx *ctx, const float *P, const float *x, float xscale, const
     SH_TRY(sh_launch_check(ctx, "pack_weights"));
     PROF_A(ctx);
     hipLaunchKernelGGL(conv1_fwd, dim3(ceil_div(n, TB)), dim3(TB), 0, ctx->stream, x, xscale, P, W.act[1], n1, n2, n3,
-                       D[0]);
+                       D[0], FastDiv((unsigned)n1), FastDiv((unsigned)n2));
     PROF_B(ctx, "conv1_fwd", (double)n * (4.0 + 64.0));
     SH_TRY(sh_launch_check(ctx, "conv1_fwd"));
     __bf16 *cur = W.act[1];

@@ -98,6 +98,28 @@ enum {
 };
 
 // ---------------------------------------------------------------------------
+// Division by a divisor fixed at launch, for indices below 2^31: a
+// multiply-high and a shift instead of the long integer division sequence
+// (Granlund & Montgomery, "Division by invariant integers using
+// multiplication", 1994: m = floor(2^32 (2^s - d) / d) + 1 with
+// s = ceil(log2 d), q = (mulhi(n, m) + n) >> s; exact for n, d < 2^31).
+// ---------------------------------------------------------------------------
+struct FastDiv {
+    unsigned d = 1, m = 1;
+    int s = 0;
+    FastDiv() = default;
+    explicit FastDiv(unsigned dv) : d(dv) {
+        s = 0;
+        while ((1ull << s) < dv) s++;
+        m = (unsigned)(((1ull << 32) * ((1ull << s) - dv)) / dv + 1);
+    }
+    SH_HD unsigned div(unsigned n) const {
+        const unsigned t = (unsigned)(((unsigned long long)n * m) >> 32);
+        return (t + n) >> s;
+    }
+};
+
+// ---------------------------------------------------------------------------
 // Deterministic reductions.  Each block writes its partial sums (double) to
 // part[k*nblk + block]; reduce_partials sums them in a fixed order.
 // ---------------------------------------------------------------------------
@@ -119,6 +141,31 @@ __device__ inline double block_sum(double v, double *sh) {
     if (threadIdx.x == 0)
         for (int w = 0; w < nw; w++) t += sh[w];
     return t;
+}
+
+// K sums over a 1-D block at once: the shuffle trees of the K values are
+// interleaved and share two barriers, and each total is formed exactly as
+// block_sum forms it (same tree, waves added in order).  Totals in v[] of
+// thread 0; `sh` needs 16 K doubles.
+template <int K>
+__device__ inline void block_sum_multi(double (&v)[K], double *sh) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int nw = (blockDim.x + 63) >> 6;
+    for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+        for (int k = 0; k < K; k++) v[k] += __shfl_down(v[k], off, 64);
+    __syncthreads();
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < K; k++) sh[k * 16 + wid] = v[k];
+    __syncthreads();
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            double t = 0.0;
+            for (int w = 0; w < nw; w++) t += sh[k * 16 + w];
+            v[k] = t;
+        }
 }
 
 // Linear thread id for 2-D/3-D blocks, used with block_sum.

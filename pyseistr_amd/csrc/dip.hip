@@ -51,7 +51,7 @@ __global__ void __launch_bounds__(256)
 pwd_kernel(const float *__restrict__ x, const float *__restrict__ p, const float *__restrict__ dp,
            float lam, float pmin, float pmax, float *__restrict__ pout, float *__restrict__ u1,
            float *__restrict__ u2, const unsigned char *__restrict__ mask, int n1, int n2, int n3,
-           int axis, double *part, const DipScal *sc) {
+           int axis, double *part, const DipScal *sc, FastDiv f1, FastDiv f2) {
     if (MODE == 2 && sc->accepted) return;
     __shared__ double sh[16];
     const long n = (long)n1 * n2 * n3;
@@ -61,10 +61,20 @@ pwd_kernel(const float *__restrict__ x, const float *__restrict__ p, const float
     // device flags make the launch a no-op)
     const long chunk = (n + gridDim.x - 1) / gridDim.x;
     const long i0 = (long)blockIdx.x * chunk, i1e = min(n, i0 + chunk);
+    const bool fast = n < (1L << 31);  // multiply-high division (no long division per sample)
     for (long i = i0 + threadIdx.x; i < i1e; i += blockDim.x) {
-        int i1 = (int)(i % n1);
-        long r = i / n1;
-        int i2 = (int)(r % n2), i3 = (int)(r / n2);
+        int i1, i2, i3;
+        if (fast) {
+            const unsigned r = f1.div((unsigned)i), q = f2.div(r);
+            i1 = (int)((unsigned)i - r * (unsigned)n1);
+            i2 = (int)(r - q * (unsigned)n2);
+            i3 = (int)q;
+        } else {
+            i1 = (int)(i % n1);
+            const long r = i / n1;
+            i2 = (int)(r % n2);
+            i3 = (int)(r / n2);
+        }
         bool valid = i1 >= NW && i1 < n1 - NW && (axis == 2 ? i2 < n2 - 1 : i3 < n3 - 1);
         acc += pwd_sample<NW, MODE>(x, p, dp, lam, pmin, pmax, pout, u1, u2, mask && mask[i], i, ip, valid);
     }
@@ -661,16 +671,17 @@ int pwd_launch(sh_ctx *ctx, int mode, const DipGeom &g, int axis, const float *x
     const float pmax = 3.402823466e+38f, pmin = -3.402823466e+38f;
     static const char *names[3] = {"pwd_kernel<apply>", "pwd_kernel<der_divne>", "pwd_kernel<trial>"};
     static const double bps[3] = {12.0, 20.0, 20.0};
+    const FastDiv fd1((unsigned)g.n1), fd2((unsigned)g.n2);
     PROF_A(ctx);
     if (mode == 0)
         hipLaunchKernelGGL((pwd_kernel<NW, 0>), dim3(nblk), dim3(256), 0, ctx->stream, x, p, dp, lam, pmin, pmax,
-                           pout, W.u1, W.u2, mask, g.n1, g.n2, g.n3, axis, W.part, W.sc);
+                           pout, W.u1, W.u2, mask, g.n1, g.n2, g.n3, axis, W.part, W.sc, fd1, fd2);
     else if (mode == 1)
         hipLaunchKernelGGL((pwd_kernel<NW, 1>), dim3(nblk), dim3(256), 0, ctx->stream, x, p, dp, lam, pmin, pmax,
-                           pout, W.u1, W.u2, mask, g.n1, g.n2, g.n3, axis, W.part, W.sc);
+                           pout, W.u1, W.u2, mask, g.n1, g.n2, g.n3, axis, W.part, W.sc, fd1, fd2);
     else
         hipLaunchKernelGGL((pwd_kernel<NW, 2>), dim3(nblk), dim3(256), 0, ctx->stream, x, p, dp, lam, pmin, pmax,
-                           pout, W.u1, W.u2, mask, g.n1, g.n2, g.n3, axis, W.part, W.sc);
+                           pout, W.u1, W.u2, mask, g.n1, g.n2, g.n3, axis, W.part, W.sc, fd1, fd2);
     PROF_B(ctx, names[mode], bps[mode] * (double)g.n);
     return sh_launch_check(ctx, "pwd_kernel");
 }

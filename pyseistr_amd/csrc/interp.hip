@@ -35,7 +35,24 @@ struct Geo {
     int maxid = 0;   // max |NINT(p)| over both slope fields (adjoint search window)
     const int *ids = nullptr;  // NINT(p) per part and sample when drift
     const float *slope[2] = {nullptr, nullptr};  // p (inline part), q (crossline part)
+    FastDiv d1, d2;  // n1, n2 (sample index -> i1, i2, i3 when n < 2^31)
 };
+
+// (i1, i2, i3) of sample k
+__device__ inline void sample_pos(const Geo &g, long k, int &i1, int &i2, int &i3) {
+    if (g.n < (1L << 31)) {
+        const unsigned r = g.d1.div((unsigned)k);
+        i1 = (int)((unsigned)k - r * (unsigned)g.n1);
+        const unsigned q = g.d2.div(r);
+        i2 = (int)(r - q * (unsigned)g.n2);
+        i3 = (int)q;
+    } else {
+        i1 = (int)(k % g.n1);
+        const long r = k / g.n1;
+        i2 = (int)(r % g.n2);
+        i3 = (int)(r / g.n2);
+    }
+}
 
 // PS_NINT (soint3d_cfuns.c:11): round half away from zero, in double
 __device__ inline int ps_nint(float p) { return (int)(p > 0.0 ? p + 0.5 : p - 0.5); }
@@ -63,136 +80,173 @@ __device__ inline float tap_slope(const Geo &g, int part_id, long i) {
 struct SolverState;
 __device__ inline int solver_done(const SolverState *st);
 
-template <int NW>
+// One output sample of pwd_fwd; loads are unconditional (indices clamped
+// into the arrays) and the result is selected, so the samples a thread
+// handles form one straight-line block whose loads issue together.
+template <int NW, bool DRIFT>
+__device__ inline void pwd_fwd_sample(const float *__restrict__ x, const float *__restrict__ ss,
+                                      const float *__restrict__ rr, const Geo &g, const float *base, long i, long ny,
+                                      float &vout, double *d) {
+    const bool live = i < ny;
+    const long ic = live ? i : ny - 1;
+    const int part_id = ic >= g.n;  // nparts <= 2
+    const long k = ic - part_id * g.n;
+    int ix, iy, iz;
+    sample_pos(g, k, ix, iy, iz);
+    const int nj = part_id ? g.nj2 : g.nj1;
+    const long ip = part_id ? (long)g.n1 * g.n2 : g.n1;
+    bool valid = ix >= NW * nj && ix < g.n1 - NW * nj && (part_id ? iz < g.n3 - 1 : iy < g.n2 - 1);
+    int id = 0;
+    if (DRIFT) {
+        id = g.ids[(long)part_id * g.n + k];
+        valid = valid && ix - NW * nj - id >= 0 && ix + NW * nj - id < g.n1;
+    }
+    float v = base ? base[ic] : 0.0f;
+    const float p = g.slope[part_id][k];
+    float a[2 * NW + 1];
+    apf_pass<NW>(DRIFT ? p - id : p, a);
+#pragma unroll
+    for (int iw = 0; iw <= 2 * NW; iw++) {
+        const int is = (iw - NW) * nj;
+        const float t = (x[valid ? k + ip + is : k] - x[valid ? k - is - id : k]) * a[iw];
+        v = valid ? v + t : v;
+    }
+    const float sv = ss[ic], rv = rr[ic];
+    vout = v;
+    d[0] = live ? (double)v * v : 0.0;
+    d[1] = live ? (double)v * sv : 0.0;
+    d[2] = live ? (double)v * rv : 0.0;
+    d[3] = live ? (double)sv * sv : 0.0;
+    d[4] = live ? (double)sv * rv : 0.0;
+    d[5] = live ? (double)rv * rv : 0.0;
+}
+
+// Each block handles V consecutive groups of TB samples ("virtual blocks",
+// the partition of the partial sums, so the sums are those of V blocks of
+// TB threads): all V samples of a thread are loaded before any is stored.
+template <int NW, bool DRIFT, int V>
 __global__ void __launch_bounds__(TB)
 pwd_fwd(const float *__restrict__ x, float *__restrict__ y,
         const float *__restrict__ ss, const float *__restrict__ rr, Geo g, double *part, int nblk,
         const float *base, const SolverState *st) {
-    __shared__ double sh[16];
+    __shared__ double sh[16 * 6];
     if (st && solver_done(st)) return;
-    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    double d_gg = 0, d_gs = 0, d_gr = 0, d_ss = 0, d_sr = 0, d_rr = 0;
-    long ny = g.n * g.nparts;
-    if (i < ny) {
-        int part_id = (int)(i / g.n);
-        long k = i - part_id * g.n;
-        int ix = (int)(k % g.n1);
-        long r = k / g.n1;
-        int iy = (int)(r % g.n2), iz = (int)(r / g.n2);
-        int nj = part_id ? g.nj2 : g.nj1;
-        long ip = part_id ? (long)g.n1 * g.n2 : g.n1;
-        bool valid = ix >= NW * nj && ix < g.n1 - NW * nj && (part_id ? iz < g.n3 - 1 : iy < g.n2 - 1);
-        int id = 0;
-        if (valid && g.drift) {
-            id = g.ids[(long)part_id * g.n + k];
-            valid = ix - NW * nj - id >= 0 && ix + NW * nj - id < g.n1;
-        }
-        float v = base ? base[i] : 0.0f;
-        if (valid) {
-            const float p = g.slope[part_id][k];
-            float a[2 * NW + 1];
-            apf_pass<NW>(g.drift ? p - id : p, a);
+    const long ny = g.n * g.nparts;
+    float v[V];
+    double d[V][6];
 #pragma unroll
-            for (int iw = 0; iw <= 2 * NW; iw++) {
-                int is = (iw - NW) * nj;
-                v += (x[k + ip + is] - x[k - is - id]) * a[iw];
-            }
-        }
-        y[i] = v;
-        const float sv = ss[i], rv = rr[i];
-        d_gg = (double)v * v;
-        d_gs = (double)v * sv;
-        d_gr = (double)v * rv;
-        d_ss = (double)sv * sv;
-        d_sr = (double)sv * rv;
-        d_rr = (double)rv * rv;
+    for (int u = 0; u < V; u++)
+        pwd_fwd_sample<NW, DRIFT>(x, ss, rr, g, base, ((long)blockIdx.x * V + u) * TB + threadIdx.x, ny, v[u], d[u]);
+#pragma unroll
+    for (int u = 0; u < V; u++) {
+        const long i = ((long)blockIdx.x * V + u) * TB + threadIdx.x;
+        if (i < ny) y[i] = v[u];
     }
-    const double d[6] = {d_gg, d_gs, d_gr, d_ss, d_sr, d_rr};
 #pragma unroll
-    for (int k = 0; k < 6; k++) {
-        const double t = block_sum(d[k], sh);
-        if (threadIdx.x == 0) part[(long)k * nblk + blockIdx.x] = t;
+    for (int u = 0; u < V; u++) {
+        block_sum_multi<6>(d[u], sh);
+        const long blk = (long)blockIdx.x * V + u;
+        if (threadIdx.x == 0 && blk < nblk)
+#pragma unroll
+            for (int k = 0; k < 6; k++) part[(long)k * nblk + blk] = d[u][k];
     }
 }
 
 // g[k] = (A^T rr)[k], zero where known (allpass3_lop adj + ps_solver
 // 1056-1060).  Contributions are summed in the order the reference's
 // scatter loop delivers them (increasing source index i, inline part
-// first).  Partial sums: g.g.
-template <int NW>
-__global__ void __launch_bounds__(TB)
-pwd_adj(const float *__restrict__ rr, const unsigned char *__restrict__ known,
-        float *__restrict__ gout, Geo g, double *part, const SolverState *st) {
-    __shared__ double sh[16];
-    if (st && solver_done(st)) return;
-    long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    double d = 0;
-    if (k < g.n) {
-        int kx = (int)(k % g.n1);
-        long r = k / g.n1;
-        int ky = (int)(r % g.n2), kz = (int)(r / g.n2);
-        float v = 0.0f;
-        for (int part_id = 0; part_id < g.nparts; part_id++) {
-            int nj = part_id ? g.nj2 : g.nj1;
-            int h = NW * nj;
-            long ip = part_id ? (long)g.n1 * g.n2 : g.n1;
-            const float *yy = rr + (long)part_id * g.n;
-            const int *idp = g.drift ? g.ids + (long)part_id * g.n : nullptr;
-            // with drift a source sample i contributes only when its shifted
-            // stencil stays inside the trace
-            auto live = [&](long i, int ix) {
-                if (!idp) return true;
-                const int id = idp[i];
-                return ix - h - id >= 0 && ix + h - id < g.n1;
-            };
-            // "+" terms: i = k - ip - is, source row valid if it has a successor
-            bool prow = part_id ? (kz >= 1) : (ky >= 1);
-            if (prow) {
+// first).  One output sample; the non-drift terms are branch-free
+// (clamped indices, selected contributions).
+template <int NW, bool DRIFT>
+__device__ inline float pwd_adj_sample(const float *__restrict__ rr, const unsigned char *__restrict__ known,
+                                       const Geo &g, long k) {
+    int kx, ky, kz;
+    sample_pos(g, k, kx, ky, kz);
+    float v = 0.0f;
+    for (int part_id = 0; part_id < g.nparts; part_id++) {
+        const int nj = part_id ? g.nj2 : g.nj1;
+        const int h = NW * nj;
+        const long ip = part_id ? (long)g.n1 * g.n2 : g.n1;
+        const float *yy = rr + (long)part_id * g.n;
+        const int *idp = DRIFT ? g.ids + (long)part_id * g.n : nullptr;
+        // with drift a source sample i contributes only when its shifted
+        // stencil stays inside the trace
+        auto live = [&](long i, int ix) {
+            if (!DRIFT) return true;
+            const int id = idp[i];
+            return ix - h - id >= 0 && ix + h - id < g.n1;
+        };
+        // "+" terms: i = k - ip - is, source row valid if it has a successor
+        const bool prow = part_id ? (kz >= 1) : (ky >= 1);
 #pragma unroll
-                for (int iw = 2 * NW; iw >= 0; iw--) {
-                    int is = (iw - NW) * nj;
-                    int ix = kx - is;
-                    if (ix >= h && ix < g.n1 - h) {
-                        long i = k - ip - is;
-                        if (live(i, ix)) v += yy[i] * apf_tap<NW>(iw, tap_slope(g, part_id, i));
-                    }
-                }
+        for (int iw = 2 * NW; iw >= 0; iw--) {
+            const int is = (iw - NW) * nj;
+            const int ix = kx - is;
+            bool on = prow && ix >= h && ix < g.n1 - h;
+            const long i = on ? k - ip - is : k;
+            if (DRIFT) on = on && live(i, ix);
+            const float t = yy[i] * apf_tap<NW>(iw, tap_slope(g, part_id, i));
+            v = on ? v + t : v;
+        }
+        const bool mrow = part_id ? (kz < g.n3 - 1) : (ky < g.n2 - 1);
+        if (!DRIFT) {
+#pragma unroll
+            for (int iw = 0; iw <= 2 * NW; iw++) {
+                const int is = (iw - NW) * nj;
+                const int ix = kx + is;
+                const bool on = mrow && ix >= h && ix < g.n1 - h;
+                const long i = on ? k + is : k;
+                const float t = yy[i] * apf_tap<NW>(iw, tap_slope(g, part_id, i));
+                v = on ? v - t : v;
             }
-            bool mrow = part_id ? (kz < g.n3 - 1) : (ky < g.n2 - 1);
-            if (mrow && !idp) {
+        } else if (mrow) {
+            // "-" terms with drift: sources ix of this row with
+            // ix - is - NINT(p(ix)) = kx, in the scatter's order (ix,
+            // then iw, ascending); |is + NINT| <= h + maxid
+            const int W = h + min(g.maxid, g.n1);
+            const int lo = max(h, kx - W), hi = min(g.n1 - h - 1, kx + W);
+            for (int ix = lo; ix <= hi; ix++) {
+                const long i = k - kx + ix;
+                if (!live(i, ix)) continue;
+                const int id = idp[i];
 #pragma unroll
                 for (int iw = 0; iw <= 2 * NW; iw++) {
                     int is = (iw - NW) * nj;
-                    int ix = kx + is;
-                    if (ix >= h && ix < g.n1 - h) {
-                        long i = k + is;
-                        v -= yy[i] * apf_tap<NW>(iw, tap_slope(g, part_id, i));
-                    }
-                }
-            } else if (mrow) {
-                // "-" terms with drift: sources ix of this row with
-                // ix - is - NINT(p(ix)) = kx, in the scatter's order (ix,
-                // then iw, ascending); |is + NINT| <= h + maxid
-                const int W = h + min(g.maxid, g.n1);
-                const int lo = max(h, kx - W), hi = min(g.n1 - h - 1, kx + W);
-                for (int ix = lo; ix <= hi; ix++) {
-                    const long i = k - kx + ix;
-                    if (!live(i, ix)) continue;
-                    const int id = idp[i];
-#pragma unroll
-                    for (int iw = 0; iw <= 2 * NW; iw++) {
-                        int is = (iw - NW) * nj;
-                        if (ix - is - id == kx) v -= yy[i] * apf_tap<NW>(iw, tap_slope(g, part_id, i));
-                    }
+                    if (ix - is - id == kx) v -= yy[i] * apf_tap<NW>(iw, tap_slope(g, part_id, i));
                 }
             }
         }
-        if (known[k]) v = 0.0f;
-        gout[k] = v;
-        d = (double)v * v;
     }
-    double t = block_sum(d, sh);
-    if (threadIdx.x == 0) part[blockIdx.x] = t;
+    return known[k] ? 0.0f : v;
+}
+
+// g[k] = (A^T rr)[k], zero where known; V virtual blocks of TB samples per
+// block as in pwd_fwd.  Partial sums: g.g.
+template <int NW, bool DRIFT, int V>
+__global__ void __launch_bounds__(TB)
+pwd_adj(const float *__restrict__ rr, const unsigned char *__restrict__ known,
+        float *__restrict__ gout, Geo g, double *part, int nblk, const SolverState *st) {
+    __shared__ double sh[16];
+    if (st && solver_done(st)) return;
+    float v[V];
+#pragma unroll
+    for (int u = 0; u < V; u++) {
+        const long k = ((long)blockIdx.x * V + u) * TB + threadIdx.x;
+        v[u] = pwd_adj_sample<NW, DRIFT>(rr, known, g, k < g.n ? k : g.n - 1);
+    }
+#pragma unroll
+    for (int u = 0; u < V; u++) {
+        const long k = ((long)blockIdx.x * V + u) * TB + threadIdx.x;
+        if (k < g.n) gout[k] = v[u];
+    }
+#pragma unroll
+    for (int u = 0; u < V; u++) {
+        const long k = ((long)blockIdx.x * V + u) * TB + threadIdx.x;
+        double d[1] = {k < g.n ? (double)v[u] * v[u] : 0.0};
+        block_sum_multi<1>(d, sh);
+        const long blk = (long)blockIdx.x * V + u;
+        if (threadIdx.x == 0 && blk < nblk) part[blk] = d[0];
+    }
 }
 
 // partial sums over a vector: sum a*b (stacked into part[k*nblk + b])
@@ -476,6 +530,46 @@ int read_iters(sh_ctx *ctx, const SolverState *st, int *iters) {
     return SH_OK;
 }
 
+// Virtual blocks per block of the soint operator kernels (1, 2 or 4;
+// SH_PWD_V overrides): identical results, more loads in flight per wave.
+int pwd_vblocks() {
+    const char *e = getenv("SH_PWD_V");
+    const int v = e ? atoi(e) : 2;
+    return (v == 1 || v == 2 || v == 4) ? v : 2;
+}
+
+template <int NW>
+int pwd_fwd_launch(sh_ctx *ctx, const Geo &g, const float *x, float *y, const float *ss, const float *rr,
+                   double *part, int nblk, const float *base, const SolverState *st) {
+    const int V = pwd_vblocks();
+    const dim3 grid(ceil_div(nblk, V));
+#define PF_(D, VV)                                                                                             \
+    hipLaunchKernelGGL((pwd_fwd<NW, D, VV>), grid, dim3(TB), 0, ctx->stream, x, y, ss, rr, g, part, nblk, base, st)
+    if (g.drift) {
+        if (V == 1) PF_(true, 1); else if (V == 2) PF_(true, 2); else PF_(true, 4);
+    } else {
+        if (V == 1) PF_(false, 1); else if (V == 2) PF_(false, 2); else PF_(false, 4);
+    }
+#undef PF_
+    return sh_launch_check(ctx, "pwd_fwd");
+}
+
+template <int NW>
+int pwd_adj_launch(sh_ctx *ctx, const Geo &g, const float *rr, const unsigned char *known, float *gout, double *part,
+                   int nblk, const SolverState *st) {
+    const int V = pwd_vblocks();
+    const dim3 grid(ceil_div(nblk, V));
+#define PA_(D, VV)                                                                                             \
+    hipLaunchKernelGGL((pwd_adj<NW, D, VV>), grid, dim3(TB), 0, ctx->stream, rr, known, gout, g, part, nblk, st)
+    if (g.drift) {
+        if (V == 1) PA_(true, 1); else if (V == 2) PA_(true, 2); else PA_(true, 4);
+    } else {
+        if (V == 1) PA_(false, 1); else if (V == 2) PA_(false, 2); else PA_(false, 4);
+    }
+#undef PA_
+    return sh_launch_check(ctx, "pwd_adj");
+}
+
 // ps_solver + ps_cgstep with the "known" and "x0" options
 template <int NW>
 int soint_solve(sh_ctx *ctx, const Geo &g0, const float *pp, const float *qq, const unsigned char *known, float *x,
@@ -489,6 +583,8 @@ int soint_solve(sh_ctx *ctx, const Geo &g0, const float *pp, const float *qq, co
     float *gv = buf, *S = gv + n, *rr = S + n, *gg = rr + ny, *Ss = gg + ny;
     g.slope[0] = pp;
     g.slope[1] = qq;
+    g.d1 = FastDiv((unsigned)g.n1);
+    g.d2 = FastDiv((unsigned)g.n2);
     int nbn = ceil_div(n, TB), nby = ceil_div(ny, TB);
     double *part = nullptr;
     SH_TRY(sh_slot(ctx, SLOT_RED, (size_t)(6 * nby + nbn) * sizeof(double), (void **)&part));
@@ -519,8 +615,7 @@ int soint_solve(sh_ctx *ctx, const Geo &g0, const float *pp, const float *qq, co
     // rr = -dd, then rr += A x0 tap by tap (ps_solver 1018-1026); its
     // partial rr.rr starts the solver state
     hipLaunchKernelGGL(neg_kernel, dim3(nby), dim3(TB), 0, ctx->stream, dd, rr, ny);
-    hipLaunchKernelGGL((pwd_fwd<NW>), dim3(nby), dim3(TB), 0, ctx->stream, x, rr, Ss, Ss, g, part, nby, rr,
-                       nullptr);
+    SH_TRY(pwd_fwd_launch<NW>(ctx, g, x, rr, Ss, Ss, part, nby, rr, nullptr));
     hipLaunchKernelGGL(solver_init, dim3(1), dim3(1024), 0, ctx->stream, part, nby, st, 0);
     SH_TRY(sh_launch_check(ctx, "soint init"));
     double *pa = part + (size_t)6 * nby;  // |g|^2 partials of pwd_adj
@@ -531,13 +626,12 @@ int soint_solve(sh_ctx *ctx, const Geo &g0, const float *pp, const float *qq, co
     auto iteration = [&]() -> int {
         {
             PROF_A(ctx);
-            hipLaunchKernelGGL((pwd_adj<NW>), dim3(nbn), dim3(TB), 0, ctx->stream, rr, known, gv, g, pa, st);
+            SH_TRY(pwd_adj_launch<NW>(ctx, g, rr, known, gv, pa, nbn, st));
             PROF_B(ctx, "pwd_adj", 4.0 * (ny + g.nparts * n + n));
         }
         {
             PROF_A(ctx);
-            hipLaunchKernelGGL((pwd_fwd<NW>), dim3(nby), dim3(TB), 0, ctx->stream, gv, gg, Ss, rr, g, part, nby,
-                               nullptr, st);
+            SH_TRY(pwd_fwd_launch<NW>(ctx, g, gv, gg, Ss, rr, part, nby, nullptr, st));
             PROF_B(ctx, "pwd_fwd", 4.0 * (n + g.nparts * n + 3 * ny));
         }
         hipLaunchKernelGGL(cgstep_scalars, dim3(1), dim3(1024), 0, ctx->stream, pa, nbn, part, nby, st, 0);
@@ -700,7 +794,7 @@ __global__ void __launch_bounds__(TB) sint_dir(const float *__restrict__ gp, con
                                                const float *__restrict__ gr, float *__restrict__ sp, float *__restrict__ sx,
                                                float *__restrict__ sr, long n, double *part, int nblk,
                                                const SolverState *st) {
-    __shared__ double sh[16];
+    __shared__ double sh[16 * 3];
     if (solver_done(st)) return;
     const int first = st->first;
     const float a = (float)st->a;
@@ -718,12 +812,13 @@ __global__ void __launch_bounds__(TB) sint_dir(const float *__restrict__ gp, con
         sp[i] = vp; sx[i] = vx; sr[i] = vr;
         d0 = (double)vr * vr; d1 = (double)vp * vp; d2 = (double)vx * vx;
     }
-    double t = block_sum(d0, sh);
-    if (threadIdx.x == 0) part[blockIdx.x] = t;
-    t = block_sum(d1, sh);
-    if (threadIdx.x == 0) part[nblk + blockIdx.x] = t;
-    t = block_sum(d2, sh);
-    if (threadIdx.x == 0) part[2 * nblk + blockIdx.x] = t;
+    double d[3] = {d0, d1, d2};
+    block_sum_multi<3>(d, sh);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = d[0];
+        part[nblk + blockIdx.x] = d[1];
+        part[2 * nblk + blockIdx.x] = d[2];
+    }
 }
 
 // beta = sr.sr + eps (sp.sp - sx.sx), step -gn / beta (ps_conjgrad

@@ -351,7 +351,7 @@ factor2_kernel(const float *__restrict__ dip, float *__restrict__ F, int n1, int
     factor_sweep_pf<NW, 1, 8>(q, rev, n1, e, Fs, n);
 }
 
-template <int NW>
+template <int NW, int PFD>
 __global__ void __launch_bounds__(TB)
 spray2_fwd(const float *__restrict__ x, const float *__restrict__ dip, const float *__restrict__ F,
            float *__restrict__ U, int n1, int n2, int n3, int ns, float e, float *__restrict__ scr, long nthr,
@@ -397,7 +397,7 @@ spray2_fwd(const float *__restrict__ x, const float *__restrict__ dip, const flo
         Acc qs[1] = {q};
         bool rv[1] = {dir == 1};
         const float *Fv = F + (long)dir * (NB + 1) * n + i3 * slice + ip;
-        solve_factored_pf<NW, 1, 8>(in, qs, rv, Fv, n2, n, y, n1, e, out);
+        solve_factored_pf<NW, 1, PFD>(in, qs, rv, Fv, n2, n, y, n1, e, out);
         cur = Acc{out.p, n2};
     }
 }
@@ -406,7 +406,7 @@ spray2_fwd(const float *__restrict__ x, const float *__restrict__ dip, const flo
 // chain dir 1 gathers slots ns+is+1 of traces i+is+1 (is = ns-1..0) through
 // adjoint predictions; dir 0 likewise from the left.  Results go to
 // C[dir][i3][k][i]; the centre slot is added by spray2_adj_combine.
-template <int NW>
+template <int NW, int PFD>
 __global__ void __launch_bounds__(TB)
 spray2_adj(const float *__restrict__ U, const float *__restrict__ dip, const float *__restrict__ F,
            float *__restrict__ C, int n1, int n2, int n3, int ns, float e, float *__restrict__ scr,
@@ -432,8 +432,8 @@ spray2_adj(const float *__restrict__ U, const float *__restrict__ dip, const flo
         Acc u{U + (long)slot * plane_slot + i3 * slice + ip, n2};
         Acc q{ds + (dir == 1 ? ip - 1 : ip), n2};
         const float *Fv = F + (long)(dir == 1 ? 1 : 0) * (NB + 1) * n + i3 * slice + ip;
-        // t = S^{-1}-adjoint prediction of (t + u), loads issued 8 samples ahead
-        predict_adj_factored_pf<NW, 8>(Acc{t.p, t.s}, &u, q, dir == 1, Fv, n2, n, y, n1, e, t);
+        // t = S^{-1}-adjoint prediction of (t + u), loads issued PFD samples ahead
+        predict_adj_factored_pf<NW, PFD>(Acc{t.p, t.s}, &u, q, dir == 1, Fv, n2, n, y, n1, e, t);
     }
     Sink c{C + (long)dir * plane_slot + i3 * slice + i, n2};
     for (int k = 0; k < n1; k++) c[k] = t[k];
@@ -724,6 +724,30 @@ int spray2_tb() {
     return 256;
 }
 
+// Prefetch distance (samples) of the chain kernels' load rings, 8 or 16
+// (SH_SPRAY2_PF sets both); identical results.  The chains run at about
+// one wave per SIMD, so the deeper ring's registers cost no occupancy.
+// Measured (sint3d 300x100x100, somean2d 512x256): 16 takes the forward
+// chains 10.0 -> 9.6 ms and 2.01 -> 1.95 ms, the adjoint chains
+// 12.9 -> 13.4 ms, so the forward chains use 16 and the adjoint ones 8.
+int spray2_pf(bool forward) {
+    const char *e = getenv("SH_SPRAY2_PF");
+    if (e && atoi(e) == 8) return 8;
+    if (e && atoi(e) == 16) return 16;
+    return forward ? 16 : 8;
+}
+
+#define SPRAY2_PF_DISPATCH(PF, FWD, ...)         \
+    do {                                         \
+        if (spray2_pf(FWD) == 16) {              \
+            constexpr int PF = 16;               \
+            __VA_ARGS__;                         \
+        } else {                                 \
+            constexpr int PF = 8;                \
+            __VA_ARGS__;                         \
+        }                                        \
+    } while (0)
+
 // Factor cache of P into caller-provided buffers: scr holds 2*nthr*n1
 // floats (solver scratch, then the running trace planes), F 2*(2*order+1)*n.
 int pws2_prepare_bufs(sh_ctx *ctx, Pwsmooth2 &P, float *scr, float *F) {
@@ -759,9 +783,9 @@ int pws2_spray(sh_ctx *ctx, Pwsmooth2 &P, const float *x, const float *x2 = null
     const int TB2 = spray2_tb();
     dim3 grid(ceil_div(2L * P.n2, TB2), P.n3, x2 ? 2 : 1);
     PROF_A(ctx);
-    SH_DISPATCH_NW(ctx, P.order, NW,
-                   hipLaunchKernelGGL((spray2_fwd<NW>), grid, dim3(TB2), 0, ctx->stream, x, P.dip, P.F, P.U, P.n1,
-                                      P.n2, P.n3, P.ns, P.e, P.scr, P.nthr, P.stop, x2, U2));
+    SPRAY2_PF_DISPATCH(PF, true, SH_DISPATCH_NW(ctx, P.order, NW,
+                   hipLaunchKernelGGL((spray2_fwd<NW, PF>), grid, dim3(TB2), 0, ctx->stream, x, P.dip, P.F, P.U, P.n1,
+                                      P.n2, P.n3, P.ns, P.e, P.scr, P.nthr, P.stop, x2, U2)));
     PROF_B(ctx, "spray2_fwd", (x2 ? 2 : 1) * 12.0 * 2 * P.ns * (double)P.n1 * P.n2 * P.n3);
     return sh_launch_check(ctx, "spray2_fwd");
 }
@@ -833,9 +857,9 @@ int pws2_apply(sh_ctx *ctx, Pwsmooth2 &P, int adj, int add, float *trace, float 
     dim3 grid(ceil_div(2L * P.n2, TB2), P.n3);
     {
         PROF_A(ctx);
-        SH_DISPATCH_NW(ctx, P.order, NW,
-                       hipLaunchKernelGGL((spray2_adj<NW>), grid, dim3(TB2), 0, ctx->stream, P.U, P.dip, P.F, P.C,
-                                          P.n1, P.n2, P.n3, P.ns, P.e, P.scr, P.tr, P.nthr, P.stop));
+        SPRAY2_PF_DISPATCH(PF, false, SH_DISPATCH_NW(ctx, P.order, NW,
+                       hipLaunchKernelGGL((spray2_adj<NW, PF>), grid, dim3(TB2), 0, ctx->stream, P.U, P.dip, P.F, P.C,
+                                          P.n1, P.n2, P.n3, P.ns, P.e, P.scr, P.tr, P.nthr, P.stop)));
         PROF_B(ctx, "spray2_adj", 12.0 * 2 * P.ns * (double)n);
     }
     SH_TRY(sh_launch_check(ctx, "spray2_adj"));
@@ -1124,9 +1148,9 @@ int shd_pwspray2d(sh_ctx *ctx, int adj, const float *dip, int n1, int n2, int ns
     }
     const int TB2 = spray2_tb();
     dim3 grid(ceil_div(2L * n2, TB2), 1);
-    SH_DISPATCH_NW(ctx, order, NW,
-                   hipLaunchKernelGGL((spray2_adj<NW>), grid, dim3(TB2), 0, ctx->stream, P.U, P.dip, P.F, P.C, n1, n2, 1,
-                                      ns, P.e, P.scr, P.tr, P.nthr, nullptr));
+    SPRAY2_PF_DISPATCH(PF, false, SH_DISPATCH_NW(ctx, order, NW,
+                   hipLaunchKernelGGL((spray2_adj<NW, PF>), grid, dim3(TB2), 0, ctx->stream, P.U, P.dip, P.F, P.C, n1, n2, 1,
+                                      ns, P.e, P.scr, P.tr, P.nthr, nullptr)));
     SH_TRY(sh_launch_check(ctx, "spray2_adj"));
     hipLaunchKernelGGL(spray2_adj_combine, dim3(ceil_div(n, TB)), dim3(TB), 0, ctx->stream, P.C, P.U, xT, n, ns, 0);
     SH_TRY(sh_launch_check(ctx, "spray2_adj_combine"));

@@ -73,3 +73,24 @@ def test_complex_input_rejected_before_any_device_call():
                  lambda: ps.fkdip(z, 0.1)):
         with pytest.raises(TypeError):
             call()
+
+
+def test_batch_tile_picker():
+    """The batched path's tile query and picker run on the host: tile -1
+    is the library default; the picker keeps the default unless the table
+    predicts at least min_gain less time for the mix."""
+    from pyseistr_amd import batch
+    tiles = batch.tile_shapes()
+    d = batch.default_tile()
+    assert 0 <= d < len(tiles)
+    base = [1.0] * len(tiles)
+    other = (d + 1) % len(tiles)
+
+    def table(gain):
+        ns = list(base)
+        ns[other] = 1.0 - gain
+        return {"tiles": [list(t) for t in tiles], "buckets": {"128x64": {"ns_per_sample": ns}}}
+
+    assert batch.pick_tile([(128, 64)] * 4, table(0.01)) == d
+    assert batch.pick_tile([(128, 64)] * 4, table(0.10)) == other
+    assert batch.pick_tile([(100, 70)], table(0.10), min_gain=0.2) == d

@@ -102,3 +102,51 @@ def test_tiled_permutation_equals_plain(ps, monkeypatch):
     b = ps.sint3dc(d, m, p, q, niter=4, eps=0.01, ns1=2, ns2=1, order1=2, order2=2)
     monkeypatch.delenv("SH_PERMUTE_PLAIN")
     same(a, b)
+
+
+def test_spray2_prefetch_distances_identical(ps, monkeypatch):
+    """The 2-D chain kernels' load rings at 8 and 16 samples ahead give the
+    same bits (spray, adjoint spray, somean2d and sint3d's shaping CG)."""
+    d, m, p, q = _case3((60, 23, 9), 11)
+    d2, p2 = d[:, :, 0], p[:, :, 0]
+    rng = np.random.default_rng(3)
+    u = rng.standard_normal(60 * 23 * 7).astype(np.float32)
+
+    from pyseistr_amd import filters
+
+    def calls():
+        return [filters.pwspray2d(d2, p2, 3, 2, 0.01),
+                filters.pwspray2d_adjoint(u, p2, 3, 2, 0.01, (60, 23)),
+                ps.somean2dc(d, p, 4, 2, 0.01),
+                ps.sint3dc(d, m, p, q, niter=5, eps=0.01, ns1=3, ns2=2, order1=2, order2=2)]
+
+    out = {}
+    for pf in ("8", "16"):
+        monkeypatch.setenv("SH_SPRAY2_PF", pf)
+        out[pf] = calls()
+    monkeypatch.delenv("SH_SPRAY2_PF")
+    for k, (a, b) in enumerate(zip(out["8"], out["16"])):
+        same(a, b, f"call {k}")
+
+
+def test_pwd_virtual_blocks_identical(ps, monkeypatch):
+    """soint's operator kernels give the same bits with 1, 2 or 4 groups of
+    samples per block (the partial-sum partition is unchanged), with and
+    without drift, in 3-D and 2-D."""
+    d, m, p, q = _case3((45, 19, 13), 21)
+    d2, m2, p2 = d[:, :, 0], m[:, :, 0], p[:, :, 0]
+
+    def calls():
+        return [ps.soint3dc(d, m, p, q, order=2, niter=9, report=True),
+                ps.soint3dc(d, m, 3 * p, 3 * q, order=1, niter=6, drift=1, report=True),
+                ps.soint2dc(d2, m2, p2, order=3, niter=9, report=True)]
+
+    out = {}
+    for v in ("1", "2", "4"):
+        monkeypatch.setenv("SH_PWD_V", v)
+        out[v] = calls()
+    monkeypatch.delenv("SH_PWD_V")
+    for v in ("2", "4"):
+        for k, ((a, ia), (b, ib)) in enumerate(zip(out["1"], out[v])):
+            same(a, b, f"V={v} call {k}")
+            assert ia == ib
