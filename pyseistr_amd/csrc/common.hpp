@@ -123,8 +123,46 @@ struct FastDiv {
 // Deterministic reductions.  Each block writes its partial sums (double) to
 // part[k*nblk + block]; reduce_partials sums them in a fixed order.
 // ---------------------------------------------------------------------------
+// Lane 0 of the wave ends with the tree sum v[0] + v[32], then + the
+// lane-16 half, ..., then + lane 1 (the order of a shuffle-down loop over
+// offsets 32, 16, ..., 1).  The partner values move without LDS: halves of
+// the wave by v_permlane32_swap, 16-lane rows by v_permlane16_swap, and
+// shifts within a row by DPP row_shl (lane i reads lane i + s of its row;
+// lanes 0 .. s-1 of row 0, the only ones lane 0's sum depends on, always
+// have their partner).  Every lane of the wave must be active.
+__device__ inline double dpp_row_shl(double v, int s) {
+    const int lo = __double2loint(v), hi = __double2hiint(v);
+    int l2, h2;
+    switch (s) {
+    case 8: l2 = __builtin_amdgcn_update_dpp(0, lo, 0x108, 0xF, 0xF, false);
+            h2 = __builtin_amdgcn_update_dpp(0, hi, 0x108, 0xF, 0xF, false); break;
+    case 4: l2 = __builtin_amdgcn_update_dpp(0, lo, 0x104, 0xF, 0xF, false);
+            h2 = __builtin_amdgcn_update_dpp(0, hi, 0x104, 0xF, 0xF, false); break;
+    case 2: l2 = __builtin_amdgcn_update_dpp(0, lo, 0x102, 0xF, 0xF, false);
+            h2 = __builtin_amdgcn_update_dpp(0, hi, 0x102, 0xF, 0xF, false); break;
+    default: l2 = __builtin_amdgcn_update_dpp(0, lo, 0x101, 0xF, 0xF, false);
+             h2 = __builtin_amdgcn_update_dpp(0, hi, 0x101, 0xF, 0xF, false); break;
+    }
+    return __hiloint2double(h2, l2);
+}
+
 __device__ inline double wave_sum(double v) {
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    {
+        const unsigned lo = (unsigned)__double2loint(v), hi = (unsigned)__double2hiint(v);
+        const auto a = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+        const auto b = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+        v += __hiloint2double((int)b[1], (int)a[1]);  // lanes 0..31: + lanes 32..63
+    }
+    {
+        const unsigned lo = (unsigned)__double2loint(v), hi = (unsigned)__double2hiint(v);
+        const auto a = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+        const auto b = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+        v += __hiloint2double((int)b[1], (int)a[1]);  // lanes 0..15: + lanes 16..31
+    }
+    v += dpp_row_shl(v, 8);
+    v += dpp_row_shl(v, 4);
+    v += dpp_row_shl(v, 2);
+    v += dpp_row_shl(v, 1);
     return v;
 }
 
@@ -151,9 +189,8 @@ template <int K>
 __device__ inline void block_sum_multi(double (&v)[K], double *sh) {
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const int nw = (blockDim.x + 63) >> 6;
-    for (int off = 32; off > 0; off >>= 1)
 #pragma unroll
-        for (int k = 0; k < K; k++) v[k] += __shfl_down(v[k], off, 64);
+    for (int k = 0; k < K; k++) v[k] = wave_sum(v[k]);
     __syncthreads();
     if (lane == 0)
 #pragma unroll
