@@ -432,8 +432,8 @@ bfactor2(const float *__restrict__ dipT, float *__restrict__ F, const BPatch *__
     constexpr int NB = 2 * NW;
     Acc qa[1] = {Acc{dipT + q.off + (v == 0 ? i : i - 1), q.n2}};
     bool rev[1] = {v == 1};
-    Sink Fs{F + (long)v * (NB + 1) * ntot + q.off + i, q.n2};
-    factor_sweep_pf<NW, 1, 8>(qa, rev, q.n1, e, Fs, ntot);
+    const FactorOut Fs = fac_out_at(F + (long)v * fac_plane_floats(NB, ntot), NB, ntot, q.off + i, q.n2);
+    factor_sweep_pf<NW, 1, 8>(qa, rev, q.n1, e, Fs);
 }
 
 // Forward spray chains of pwspray_lop (sof_cfuns.c:998-1028), one thread per
@@ -468,8 +468,8 @@ bspray2(const float *__restrict__ xT, const float *__restrict__ dipT, const floa
         Acc in[1] = {cur};
         Acc qs[1] = {qa};
         bool rv[1] = {dir == 1};
-        const float *Fv = F + (long)dir * (NB + 1) * ntot + q.off + ip;
-        solve_factored_pf<NW, 1, 8>(in, qs, rv, Fv, n2, ntot, y, n1, e, out);
+        const Factor Fv = fac_at(F + (long)dir * fac_plane_floats(NB, ntot), NB, ntot, q.off + ip, n2);
+        solve_factored_pf<NW, 1, 8>(in, qs, rv, Fv, y, n1, e, out);
         cur = Acc{out.p, n2};
     }
 }
@@ -580,12 +580,11 @@ int sh_somf2d_batched(sh_ctx *ctx, const float *data, const float *dip, const lo
     BatchTables T;
     SH_TRY(build_tables(ctx, offs, n1s, n2s, npatch, TILE_CFG[DEFAULT_TILE][0], TILE_CFG[DEFAULT_TILE][1], T));
     long n = T.ntot;
-    int NB1 = 2 * order + 1;
     float *b = nullptr, *U = nullptr, *F = nullptr, *scr = nullptr;
     SH_TRY(sh_slot(ctx, SLOT_BATCH0 + 5, (size_t)n * 3 * sizeof(float), (void **)&b));
     float *xT = b, *dT = b + n, *yT = b + 2 * n;
     SH_TRY(sh_slot(ctx, SLOT_BATCH0 + 6, (size_t)n * np * sizeof(float), (void **)&U));
-    SH_TRY(sh_slot(ctx, SLOT_BATCH0 + 7, (size_t)2 * NB1 * n * sizeof(float), (void **)&F));
+    SH_TRY(sh_slot(ctx, SLOT_BATCH0 + 7, (size_t)2 * fac_plane_floats(2 * order, n) * sizeof(float), (void **)&F));
     long nthr = (long)T.ncblk * TB;
     SH_TRY(sh_slot(ctx, SLOT_BATCH0 + 8, (size_t)nthr * T.n1max * sizeof(float), (void **)&scr));
     hipLaunchKernelGGL(bzero, dim3(ceil_div(n * 3, TB)), dim3(TB), 0, ctx->stream, b, n * 3);

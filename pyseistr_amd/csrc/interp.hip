@@ -932,8 +932,9 @@ int sint_cg(sh_ctx *ctx, Shape3 &Sh, const unsigned char *known, float *p, float
 // ---------------------------------------------------------------------------
 #include "predict.hpp"
 
-// factor of the forward-direction prediction through slope trace i2
-// (F layout [i2][NB + 1][n1])
+// factor of the forward-direction prediction through slope trace i2 (trace
+// i2's factor, a Factor with fs = 1, at F + i2 * fac_plane_floats(NB, n1):
+// the NB n1 off-diagonal entries row by row, then the n1 diagonal ones)
 template <int NW>
 __global__ void prec_factor(const float *__restrict__ dip, float *__restrict__ F, int n1, int n2, float e) {
     const int i2 = blockIdx.x * blockDim.x + threadIdx.x;
@@ -941,7 +942,7 @@ __global__ void prec_factor(const float *__restrict__ dip, float *__restrict__ F
     constexpr int NB = 2 * NW;
     Acc q[1] = {Acc{dip + (long)i2 * n1, 1}};
     bool rev[1] = {true};
-    factor_sweep<NW, 1>(q, rev, n1, e, Sink{F + (long)i2 * (NB + 1) * n1, 1}, n1);
+    factor_sweep<NW, 1>(q, rev, n1, e, fac_out_at(F + (long)i2 * fac_plane_floats(NB, n1), NB, n1, 0, 1));
 }
 
 // predict_lop without add: y = P x (adj = 0) or x = P^T y (adj = 1); tt and
@@ -958,8 +959,9 @@ __global__ void prec_chain(int adj, float *__restrict__ x, float *__restrict__ y
     if (adj) {
         for (int i2 = n2 - 1; i2 >= 0; i2--) {
             const long o = (long)i2 * n1;
-            predict_adj_factored<NW>(Acc{tt, 1}, Acc{dip + o, 1}, true, F + (long)i2 * (NB + 1) * n1, 1, n1,
-                                     Sink{ys, 1}, n1, e, Sink{tt, 1});
+            predict_adj_factored<NW>(Acc{tt, 1}, Acc{dip + o, 1}, true,
+                                     fac_at(F + (long)i2 * fac_plane_floats(NB, n1), NB, n1, 0, 1), Sink{ys, 1}, n1, e,
+                                     Sink{tt, 1});
             for (int i1 = 0; i1 < n1; i1++) tt[i1] += y[o + i1];
             for (int i1 = 0; i1 < n1; i1++) x[o + i1] = 0.0f + tt[i1];
         }
@@ -969,8 +971,8 @@ __global__ void prec_chain(int adj, float *__restrict__ x, float *__restrict__ y
             for (int i1 = 0; i1 < n1; i1++) tt[i1] += x[o + i1];
             for (int i1 = 0; i1 < n1; i1++) y[o + i1] = 0.0f + tt[i1];
             Acc in[1] = {Acc{tt, 1}}, q[1] = {Acc{dip + o, 1}};
-            solve_factored_pf<NW, 1, 8>(in, q, rev, F + (long)i2 * (NB + 1) * n1, 1, n1, Sink{ys, 1}, n1, e,
-                                        Sink{tt, 1});
+            solve_factored_pf<NW, 1, 8>(in, q, rev, fac_at(F + (long)i2 * fac_plane_floats(NB, n1), NB, n1, 0, 1),
+                                        Sink{ys, 1}, n1, e, Sink{tt, 1});
         }
     }
 }
@@ -983,7 +985,7 @@ template <int NW>
 __global__ void __launch_bounds__(256)
 prec_chain_lds(int adj, float *__restrict__ x, float *__restrict__ y, const float *__restrict__ dip,
                const float *__restrict__ F, int n1, int n2, float e, const SolverState *st) {
-    extern __shared__ float sm[];
+    extern __shared__ __align__(16) float sm[];
     if (st && solver_done(st)) return;
     constexpr int NB = 2 * NW;
     float *Fs = sm, *ds = Fs + (NB + 1) * n1, *tt = ds + n1, *ys = tt + n1;
@@ -993,7 +995,7 @@ prec_chain_lds(int adj, float *__restrict__ x, float *__restrict__ y, const floa
     for (int s = 0; s < n2; s++) {
         const int i2 = adj ? n2 - 1 - s : s;
         const long o = (long)i2 * n1;
-        const float *Fi = F + (long)i2 * (NB + 1) * n1;
+        const float *Fi = F + (long)i2 * fac_plane_floats(NB, n1);
         for (int i = t; i < (NB + 1) * n1; i += T) Fs[i] = Fi[i];
         for (int i = t; i < n1; i += T) ds[i] = dip[o + i];
         if (!adj)
@@ -1005,10 +1007,11 @@ prec_chain_lds(int adj, float *__restrict__ x, float *__restrict__ y, const floa
         __syncthreads();
         if (t == 0) {
             if (adj) {
-                predict_adj_factored<NW>(Acc{tt, 1}, Acc{ds, 1}, true, Fs, 1, n1, Sink{ys, 1}, n1, e, Sink{tt, 1});
+                predict_adj_factored<NW>(Acc{tt, 1}, Acc{ds, 1}, true, fac_at(Fs, NB, n1, 0, 1), Sink{ys, 1}, n1, e,
+                                         Sink{tt, 1});
             } else {
                 Acc in[1] = {Acc{tt, 1}}, q[1] = {Acc{ds, 1}};
-                solve_factored<NW, 1>(in, q, rev, Fs, 1, n1, Sink{ys, 1}, n1, e, Sink{tt, 1});
+                solve_factored<NW, 1>(in, q, rev, fac_at(Fs, NB, n1, 0, 1), Sink{ys, 1}, n1, e, Sink{tt, 1});
             }
         }
         __syncthreads();
@@ -1021,8 +1024,9 @@ prec_chain_lds(int adj, float *__restrict__ x, float *__restrict__ y, const floa
     }
 }
 
-// Substitutions of one prediction with a stored factor Fs (plane 0 = d,
-// plane 1 + m = o[m]; n1 per plane): forward b[k] = src[k] - ...,
+// Substitutions of one prediction with a stored factor Fs (a trace's
+// Factor with fs = 1: row k's off-diagonal entries at NB k, d[k] at
+// NB n1 + k): forward b[k] = src[k] - ...,
 // y[k] = b[k] / d[k], then backward into out (which may be y).  The loops of
 // solve_factored / predict_adj_factored, operation for operation.  (Loads
 // issued ahead into register rings, or the loops unrolled by four, measured
@@ -1043,10 +1047,10 @@ __device__ void subst_factored(const float *src, const float *Fs, float *y, int 
 #pragma unroll
         for (int m = 0; m < NB; m++)
             if (m < m1) b -= (Lo[m][m]) * Bw[m];
-        y[k] = b / Fs[k];
+        y[k] = b / Fs[NB * n1 + k];
         float ok[NB];
 #pragma unroll
-        for (int m = 0; m < NB; m++) ok[m] = Fs[(1 + m) * n1 + k];
+        for (int m = 0; m < NB; m++) ok[m] = Fs[NB * k + m];
 #pragma unroll
         for (int r = NB - 1; r > 0; r--) {
             Bw[r] = Bw[r - 1];
@@ -1065,7 +1069,7 @@ __device__ void subst_factored(const float *src, const float *Fs, float *y, int 
         const int m1 = (n1 - k - 1) < NB ? (n1 - k - 1) : NB;
 #pragma unroll
         for (int m = 0; m < NB; m++)
-            if (m < m1) t -= Fs[(1 + m) * n1 + k] * nx[m];
+            if (m < m1) t -= Fs[NB * k + m] * nx[m];
 #pragma unroll
         for (int m = NB - 1; m > 0; m--) nx[m] = nx[m - 1];
         nx[0] = t;
@@ -1083,7 +1087,7 @@ template <int NW>
 __global__ void __launch_bounds__(256)
 prec_chain_par(int adj, float *__restrict__ x, float *__restrict__ y, const float *__restrict__ dip,
                const float *__restrict__ F, int n1, int n2, float e, const SolverState *st) {
-    extern __shared__ float sm[];
+    extern __shared__ __align__(16) float sm[];
     if (st && solver_done(st)) return;
     constexpr int NB = 2 * NW, NA = 2 * NW + 1;
     float *Fs = sm, *tp = Fs + (NB + 1) * n1, *tt = tp + NA * n1, *ys = tt + n1, *tm = ys + n1, *rh = tm + n1;
@@ -1092,7 +1096,7 @@ prec_chain_par(int adj, float *__restrict__ x, float *__restrict__ y, const floa
     for (int s = 0; s < n2; s++) {
         const int i2 = adj ? n2 - 1 - s : s;
         const long o = (long)i2 * n1;
-        const float *Fi = F + (long)i2 * (NB + 1) * n1;
+        const float *Fi = F + (long)i2 * fac_plane_floats(NB, n1);
         for (int i = t; i < (NB + 1) * n1; i += T) Fs[i] = Fi[i];
         const Acc q{dip + o, 1};
         for (int i = t; i < n1; i += T) {
@@ -1285,7 +1289,7 @@ int soint2d_prec_solve(sh_ctx *ctx, const float *din, const unsigned char *known
     const float e = 0.0001f;
     const int nb = ceil_div(n, TB);
     // F for pp (and qq), p, g, S, rr, gg, Ss, xt, tmp, tt, ys
-    const size_t nF = (size_t)(NB + 1) * n;
+    const size_t nF = (size_t)n2 * fac_plane_floats(NB, n1);
     const size_t total = nF * (qq ? 2 : 1) + 8 * (size_t)n + 2 * (size_t)n1;
     float *buf = nullptr;
     SH_TRY(sh_slot(ctx, SLOT_INTERP0 + 10, total * sizeof(float), (void **)&buf));

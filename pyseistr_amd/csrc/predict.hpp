@@ -24,6 +24,69 @@ struct Sink {
     SH_HD float &operator[](long k) const { return p[k * s]; }
 };
 
+// LDL^T factor of one trace's banded matrix, rows k = 0 .. n1-1: d[k] at
+// d[k * fs]; the NB off-diagonal entries o[0..NB) of row k side by side at
+// o[NB * k * fs] (element e of a plane of traces keeps its row at NB * e of
+// the off-diagonal block and its d at e of the diagonal plane).  A row is
+// one 16-byte load for NB = 4, 8 (8-byte loads for NB = 2, 6), so a step of
+// a substitution issues one or two loads for the factor instead of NB + 1.
+struct Factor {
+    const float *d, *o;
+    long fs;
+};
+struct FactorOut {
+    float *d, *o;
+    long fs;
+};
+
+template <int NB>
+SH_HD inline void fac_row(const Factor &F, long k, float *ok) {
+    const float *p = F.o + (long)NB * k * F.fs;
+    if constexpr (NB % 4 == 0) {
+#pragma unroll
+        for (int c = 0; c < NB / 4; c++) {
+            const float4 v = reinterpret_cast<const float4 *>(p)[c];
+            ok[4 * c] = v.x;
+            ok[4 * c + 1] = v.y;
+            ok[4 * c + 2] = v.z;
+            ok[4 * c + 3] = v.w;
+        }
+    } else {
+#pragma unroll
+        for (int c = 0; c < NB / 2; c++) {
+            const float2 v = reinterpret_cast<const float2 *>(p)[c];
+            ok[2 * c] = v.x;
+            ok[2 * c + 1] = v.y;
+        }
+    }
+}
+
+template <int NB>
+SH_HD inline void fac_put(const FactorOut &F, long k, float dk, const float *ok) {
+    F.d[k * F.fs] = dk;
+    float *p = F.o + (long)NB * k * F.fs;
+    if constexpr (NB % 4 == 0) {
+#pragma unroll
+        for (int c = 0; c < NB / 4; c++)
+            reinterpret_cast<float4 *>(p)[c] = make_float4(ok[4 * c], ok[4 * c + 1], ok[4 * c + 2], ok[4 * c + 3]);
+    } else {
+#pragma unroll
+        for (int c = 0; c < NB / 2; c++) reinterpret_cast<float2 *>(p)[c] = make_float2(ok[2 * c], ok[2 * c + 1]);
+    }
+}
+
+// Floats per factor variant of a plane of n elements (the off-diagonal
+// block first, so it starts 16-byte aligned when the buffer does; the
+// variant stride is kept a multiple of 4 floats).
+SH_HD inline long fac_plane_floats(int nb, long n) { return ((long)(nb + 1) * n + 3) / 4 * 4; }
+// Factor of the trace whose sample 0 is element e0 of the variant at Fv.
+SH_HD inline Factor fac_at(const float *Fv, int nb, long n, long e0, long fs) {
+    return Factor{Fv + (long)nb * n + e0, Fv + (long)nb * e0, fs};
+}
+SH_HD inline FactorOut fac_out_at(float *Fv, int nb, long n, long e0, long fs) {
+    return FactorOut{Fv + (long)nb * n + e0, Fv + (long)nb * e0, fs};
+}
+
 // Taps of the filter at sample kk (zero outside the trace), reversed when rev.
 template <int NW>
 SH_HD inline void taps_at(const Acc &q, int kk, int n1, bool rev, float *a) {
@@ -337,10 +400,10 @@ SH_HD void predict_adj(const Acc &in, const Acc &q, bool rev, Sink scr, long pla
 // matrix; solve_factored then builds the right-hand side, substitutes
 // forward and backward with the stored factor.  The arithmetic is the same
 // as the fused path, operation for operation.
-// Factor layout: F plane 0 = d[k], plane 1+m = o[m][k] (stride fs).
+// Factor layout: struct Factor (d plane, off-diagonal rows side by side).
 // ---------------------------------------------------------------------------
 template <int NW, int NF>
-SH_HD void factor_sweep(const Acc *q, const bool *rev, int n1, float e, Sink F, long fplane) {
+SH_HD void factor_sweep(const Acc *q, const bool *rev, int n1, float e, FactorOut F) {
     constexpr int NA = 2 * NW + 1, NB = 2 * NW;
     float A[NF][NA][NA];
     float Ld[NB], Lo[NB][NB];
@@ -399,9 +462,7 @@ SH_HD void factor_sweep(const Acc *q, const bool *rev, int n1, float e, Sink F, 
                 ok[nidx] = s / dk;
             }
         }
-        F.p[(long)k * F.s] = dk;
-#pragma unroll
-        for (int m = 0; m < NB; m++) F.p[(long)(1 + m) * fplane + (long)k * F.s] = ok[m];
+        fac_put<NB>(F, k, dk, ok);
 #pragma unroll
         for (int r = NB - 1; r > 0; r--) {
             Ld[r] = Ld[r - 1];
@@ -440,7 +501,7 @@ SH_HD inline void taps_from(float qv, int kk, int n1, bool rev, float *a) {
 // factor_sweep with the slope loads software-pipelined D samples ahead (a
 // register ring; loop unrolled by D); same arithmetic, identical factor.
 template <int NW, int NF, int D>
-SH_HD void factor_sweep_pf(const Acc *q, const bool *rev, int n1, float e, Sink F, long fplane) {
+SH_HD void factor_sweep_pf(const Acc *q, const bool *rev, int n1, float e, FactorOut F) {
     constexpr int NA = 2 * NW + 1, NB = 2 * NW;
     float A[NF][NA][NA];
     float Ld[NB], Lo[NB][NB];
@@ -505,9 +566,7 @@ SH_HD void factor_sweep_pf(const Acc *q, const bool *rev, int n1, float e, Sink 
                 ok[nidx] = sm / dk;
             }
         }
-        F.p[(long)k * F.s] = dk;
-#pragma unroll
-        for (int m = 0; m < NB; m++) F.p[(long)(1 + m) * fplane + (long)k * F.s] = ok[m];
+        fac_put<NB>(F, k, dk, ok);
 #pragma unroll
         for (int r = NB - 1; r > 0; r--) {
             Ld[r] = Ld[r - 1];
@@ -541,8 +600,8 @@ SH_HD void factor_sweep_pf(const Acc *q, const bool *rev, int n1, float e, Sink 
 
 // Forward prediction with a precomputed factor F; y is per-thread scratch.
 template <int NW, int NF>
-SH_HD void solve_factored(const Acc *in, const Acc *q, const bool *rev, const float *F, long fs, long fplane,
-                          Sink y, int n1, float e, Sink out) {
+SH_HD void solve_factored(const Acc *in, const Acc *q, const bool *rev, const Factor &F, Sink y, int n1, float e,
+                          Sink out) {
     constexpr int NA = 2 * NW + 1, NB = 2 * NW;
     float tb[4];
     if (NF == 1) {
@@ -599,11 +658,10 @@ SH_HD void solve_factored(const Acc *in, const Acc *q, const bool *rev, const fl
 #pragma unroll
         for (int m = 0; m < NB; m++)
             if (m < m1) b -= (Lo[m][m]) * Bw[m];
-        float dk = F[(long)k * fs];
+        float dk = F.d[(long)k * F.fs];
         y[k] = b / dk;
         float ok[NB];
-#pragma unroll
-        for (int m = 0; m < NB; m++) ok[m] = F[(long)(1 + m) * fplane + (long)k * fs];
+        fac_row<NB>(F, k, ok);
 #pragma unroll
         for (int r = NB - 1; r > 0; r--) {
             Bw[r] = Bw[r - 1];
@@ -640,9 +698,11 @@ SH_HD void solve_factored(const Acc *in, const Acc *q, const bool *rev, const fl
     for (int k = n1 - 1; k >= 0; k--) {
         float t = y[k];
         int m1 = (n1 - k - 1) < NB ? (n1 - k - 1) : NB;
+        float ok[NB];
+        fac_row<NB>(F, k, ok);
 #pragma unroll
         for (int m = 0; m < NB; m++)
-            if (m < m1) t -= F[(long)(1 + m) * fplane + (long)k * fs] * nx[m];
+            if (m < m1) t -= ok[m] * nx[m];
 #pragma unroll
         for (int m = NB - 1; m > 0; m--) nx[m] = nx[m - 1];
         nx[0] = t;
@@ -659,8 +719,8 @@ SH_HD void solve_factored(const Acc *in, const Acc *q, const bool *rev, const fl
 // index is valid).  The arithmetic is solve_factored's, operation for
 // operation, so the results are identical.
 template <int NW, int NF, int D>
-SH_HD void solve_factored_pf(const Acc *in, const Acc *q, const bool *rev, const float *F, long fs, long fplane,
-                             Sink y, int n1, float e, Sink out) {
+SH_HD void solve_factored_pf(const Acc *in, const Acc *q, const bool *rev, const Factor &F, Sink y, int n1, float e,
+                             Sink out) {
     constexpr int NA = 2 * NW + 1, NB = 2 * NW;
     float tb[4];
     if (NF == 1) {
@@ -700,9 +760,8 @@ SH_HD void solve_factored_pf(const Acc *in, const Acc *q, const bool *rev, const
     float rd[D], ro[D][NB], rq[D][NF], ri[D][NF];
     auto fetch = [&](int s, int k) {
         const long kc = k < n1 ? k : n1 - 1;
-        rd[s] = F[kc * fs];
-#pragma unroll
-        for (int m = 0; m < NB; m++) ro[s][m] = F[(long)(1 + m) * fplane + kc * fs];
+        rd[s] = F.d[kc * F.fs];
+        fac_row<NB>(F, kc, ro[s]);
         const int kq = k + 1 + NW < n1 ? k + 1 + NW : n1 - 1;
         const int ki = k + 1 + 2 * NW < n1 ? k + 1 + 2 * NW : n1 - 1;
 #pragma unroll
@@ -786,8 +845,7 @@ SH_HD void solve_factored_pf(const Acc *in, const Acc *q, const bool *rev, const
     auto fetchb = [&](int s, int k) {
         const long kc = k >= 0 ? k : 0;
         ry[s] = y[kc];
-#pragma unroll
-        for (int m = 0; m < NB; m++) rb[s][m] = F[(long)(1 + m) * fplane + kc * fs];
+        fac_row<NB>(F, kc, rb[s]);
     };
     auto stepb = [&](int s, int k) {
         float t = ry[s];
@@ -820,8 +878,8 @@ SH_HD void solve_factored_pf(const Acc *in, const Acc *q, const bool *rev, const
 // y[r] and writes out[r] when live[r]; a dead one must still point at
 // readable inputs.  Each live result is bit-identical to solve_factored.
 template <int NW, int NF, int M>
-SH_HD void solve_factored_multi(const Acc *in, const bool *live, const Acc *q, const bool *rev, const float *F,
-                                long fs, long fplane, const Sink *y, int n1, float e, const Sink *out) {
+SH_HD void solve_factored_multi(const Acc *in, const bool *live, const Acc *q, const bool *rev, const Factor &F,
+                                const Sink *y, int n1, float e, const Sink *out) {
     constexpr int NA = 2 * NW + 1, NB = 2 * NW;
     float tb[M][4];
 #pragma unroll
@@ -868,10 +926,9 @@ SH_HD void solve_factored_multi(const Acc *in, const bool *live, const Acc *q, c
     }
     for (int k = 0; k < n1; k++) {
         int m1 = k < NB ? k : NB;
-        float dk = F[(long)k * fs];
+        float dk = F.d[(long)k * F.fs];
         float ok[NB];
-#pragma unroll
-        for (int m = 0; m < NB; m++) ok[m] = F[(long)(1 + m) * fplane + (long)k * fs];
+        fac_row<NB>(F, k, ok);
         float bnew[M];
 #pragma unroll
         for (int r = 0; r < M; r++) {
@@ -942,8 +999,9 @@ SH_HD void solve_factored_multi(const Acc *in, const bool *live, const Acc *q, c
     for (int k = n1 - 1; k >= 0; k--) {
         int m1 = (n1 - k - 1) < NB ? (n1 - k - 1) : NB;
         float ok[NB];
+        fac_row<NB>(F, k, ok);
 #pragma unroll
-        for (int m = 0; m < NB; m++) ok[m] = (m < m1) ? F[(long)(1 + m) * fplane + (long)k * fs] : 0.0f;
+        for (int m = 0; m < NB; m++) ok[m] = (m < m1) ? ok[m] : 0.0f;
 #pragma unroll
         for (int r = 0; r < M; r++) {
             if (!live[r]) continue;
@@ -963,8 +1021,8 @@ SH_HD void solve_factored_multi(const Acc *in, const bool *live, const Acc *q, c
 // backward substitution), then out = A^T A s + e * boundary(s)
 // (predict_step with adj, sof_cfuns.c:622-649).  y holds s.
 template <int NW>
-SH_HD void predict_adj_factored(const Acc &in, const Acc &q, bool rev, const float *F, long fs, long fplane,
-                                Sink y, int n1, float e, Sink out) {
+SH_HD void predict_adj_factored(const Acc &in, const Acc &q, bool rev, const Factor &F, Sink y, int n1, float e,
+                                Sink out) {
     constexpr int NB = 2 * NW;
     float Lo[NB][NB], Bw[NB];
 #pragma unroll
@@ -979,10 +1037,9 @@ SH_HD void predict_adj_factored(const Acc &in, const Acc &q, bool rev, const flo
 #pragma unroll
         for (int m = 0; m < NB; m++)
             if (m < m1) b -= (Lo[m][m]) * Bw[m];
-        y[k] = b / F[(long)k * fs];
+        y[k] = b / F.d[(long)k * F.fs];
         float ok[NB];
-#pragma unroll
-        for (int m = 0; m < NB; m++) ok[m] = F[(long)(1 + m) * fplane + (long)k * fs];
+        fac_row<NB>(F, k, ok);
 #pragma unroll
         for (int r = NB - 1; r > 0; r--) {
             Bw[r] = Bw[r - 1];
@@ -999,9 +1056,11 @@ SH_HD void predict_adj_factored(const Acc &in, const Acc &q, bool rev, const flo
     for (int k = n1 - 1; k >= 0; k--) {
         float t = y[k];
         int m1 = (n1 - k - 1) < NB ? (n1 - k - 1) : NB;
+        float ok[NB];
+        fac_row<NB>(F, k, ok);
 #pragma unroll
         for (int m = 0; m < NB; m++)
-            if (m < m1) t -= F[(long)(1 + m) * fplane + (long)k * fs] * nx[m];
+            if (m < m1) t -= ok[m] * nx[m];
 #pragma unroll
         for (int m = NB - 1; m > 0; m--) nx[m] = nx[m - 1];
         nx[0] = t;
@@ -1018,8 +1077,8 @@ SH_HD void predict_adj_factored(const Acc &in, const Acc &q, bool rev, const flo
 // float operations in the same order as predict_adj_factored; y holds the
 // solved trace, out may alias a.
 template <int NW, int D>
-SH_HD void predict_adj_factored_pf(const Acc &a, const Acc *b, const Acc &q, bool rev, const float *F, long fs,
-                                   long fplane, Sink y, int n1, float e, Sink out) {
+SH_HD void predict_adj_factored_pf(const Acc &a, const Acc *b, const Acc &q, bool rev, const Factor &F, Sink y,
+                                   int n1, float e, Sink out) {
     constexpr int NB = 2 * NW, NA = 2 * NW + 1;
     // forward substitution: ring of d[k], o[m][k] and the input at k
     {
@@ -1033,9 +1092,8 @@ SH_HD void predict_adj_factored_pf(const Acc &a, const Acc *b, const Acc &q, boo
         float rd[D], ro[D][NB], ri[D];
         auto fetch = [&](int s, int k) {
             const long kc = k < n1 ? k : n1 - 1;
-            rd[s] = F[kc * fs];
-#pragma unroll
-            for (int m = 0; m < NB; m++) ro[s][m] = F[(long)(1 + m) * fplane + kc * fs];
+            rd[s] = F.d[kc * F.fs];
+            fac_row<NB>(F, kc, ro[s]);
             ri[s] = b ? (a[kc] + (*b)[kc]) : a[kc];
         };
         auto step = [&](int s, int k) {
@@ -1076,8 +1134,7 @@ SH_HD void predict_adj_factored_pf(const Acc &a, const Acc *b, const Acc &q, boo
         auto fetchb = [&](int s, int k) {
             const long kc = k >= 0 ? k : 0;
             ry[s] = y[kc];
-#pragma unroll
-            for (int m = 0; m < NB; m++) rb[s][m] = F[(long)(1 + m) * fplane + kc * fs];
+            fac_row<NB>(F, kc, rb[s]);
         };
         auto stepb = [&](int s, int k) {
             float t = ry[s];

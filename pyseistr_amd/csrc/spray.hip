@@ -135,10 +135,10 @@ factor3_kernel(const float *__restrict__ pi, const float *__restrict__ px, float
         nf++;
     }
     constexpr int NB = 2 * NW;
-    Sink Fs{F + (long)v * (NB + 1) * n + j, ntr};
+    const FactorOut Fs = fac_out_at(F + (long)v * fac_plane_floats(NB, n), NB, n, j, ntr);
     // slope loads pipelined 8 samples ahead (identical factor)
-    if (nf == 1) factor_sweep_pf<NW, 1, 8>(q, rev, n1, e, Fs, n);
-    else factor_sweep_pf<NW, 2, 8>(q, rev, n1, e, Fs, n);
+    if (nf == 1) factor_sweep_pf<NW, 1, 8>(q, rev, n1, e, Fs);
+    else factor_sweep_pf<NW, 2, 8>(q, rev, n1, e, Fs);
 }
 
 // Block -> (trace block, node) of a level launch over a 1-D grid of
@@ -204,11 +204,12 @@ spray3_level(const float *__restrict__ pi, const float *__restrict__ px, float *
         nf++;
     }
     constexpr int NB = 2 * NW;
-    const float *Fv = F + (long)node_variant(nd.update, nd.up2, nd.up3) * (NB + 1) * n + j;
+    const Factor Fv = fac_at(F + (long)node_variant(nd.update, nd.up2, nd.up3) * fac_plane_floats(NB, n), NB, n, j,
+                             ntr);
     if constexpr (PF > 0)
-        solve_factored_pf<NW, NF, PF>(in, q, rev, Fv, ntr, n, y, n1, e, out);
+        solve_factored_pf<NW, NF, PF>(in, q, rev, Fv, y, n1, e, out);
     else
-        solve_factored<NW, NF>(in, q, rev, Fv, ntr, n, y, n1, e, out);
+        solve_factored<NW, NF>(in, q, rev, Fv, y, n1, e, out);
 }
 
 // Two-source slots grouped by (level, quadrant): every slot of a group
@@ -267,8 +268,8 @@ spray3_group(const float *__restrict__ pi, const float *__restrict__ px, float *
     Acc q[2] = {Acc{pi + (up2 ? j - 1 : j), ntr}, Acc{px + (up3 ? j - n2 : j), ntr}};
     bool rev[2] = {up2 != 0, up3 != 0};
     constexpr int NB = 2 * NW;
-    const float *Fv = F + (long)node_variant(3, up2, up3) * (NB + 1) * n + j;
-    solve_factored_multi<NW, 2, M>(in, live, q, rev, Fv, ntr, n, y, n1, e, out);
+    const Factor Fv = fac_at(F + (long)node_variant(3, up2, up3) * fac_plane_floats(NB, n), NB, n, j, ntr);
+    solve_factored_multi<NW, 2, M>(in, live, q, rev, Fv, y, n1, e, out);
 }
 
 // centre slot = data (trace-fast copy)
@@ -347,8 +348,8 @@ factor2_kernel(const float *__restrict__ dip, float *__restrict__ F, int n1, int
     constexpr int NB = 2 * NW;
     Acc q[1] = {Acc{dip + i3 * slice + (v == 0 ? i : i - 1), n2}};
     bool rev[1] = {v == 1};
-    Sink Fs{F + (long)v * (NB + 1) * n + i3 * slice + i, n2};
-    factor_sweep_pf<NW, 1, 8>(q, rev, n1, e, Fs, n);
+    const FactorOut Fs = fac_out_at(F + (long)v * fac_plane_floats(NB, n), NB, n, i3 * slice + i, n2);
+    factor_sweep_pf<NW, 1, 8>(q, rev, n1, e, Fs);
 }
 
 template <int NW, int PFD>
@@ -396,8 +397,8 @@ spray2_fwd(const float *__restrict__ x, const float *__restrict__ dip, const flo
         Acc in[1] = {cur};
         Acc qs[1] = {q};
         bool rv[1] = {dir == 1};
-        const float *Fv = F + (long)dir * (NB + 1) * n + i3 * slice + ip;
-        solve_factored_pf<NW, 1, PFD>(in, qs, rv, Fv, n2, n, y, n1, e, out);
+        const Factor Fv = fac_at(F + (long)dir * fac_plane_floats(NB, n), NB, n, i3 * slice + ip, n2);
+        solve_factored_pf<NW, 1, PFD>(in, qs, rv, Fv, y, n1, e, out);
         cur = Acc{out.p, n2};
     }
 }
@@ -431,9 +432,9 @@ spray2_adj(const float *__restrict__ U, const float *__restrict__ dip, const flo
         int slot = dir == 1 ? ns + is + 1 : ns - is - 1;
         Acc u{U + (long)slot * plane_slot + i3 * slice + ip, n2};
         Acc q{ds + (dir == 1 ? ip - 1 : ip), n2};
-        const float *Fv = F + (long)(dir == 1 ? 1 : 0) * (NB + 1) * n + i3 * slice + ip;
+        const Factor Fv = fac_at(F + (long)(dir == 1 ? 1 : 0) * fac_plane_floats(NB, n), NB, n, i3 * slice + ip, n2);
         // t = S^{-1}-adjoint prediction of (t + u), loads issued PFD samples ahead
-        predict_adj_factored_pf<NW, PFD>(Acc{t.p, t.s}, &u, q, dir == 1, Fv, n2, n, y, n1, e, t);
+        predict_adj_factored_pf<NW, PFD>(Acc{t.p, t.s}, &u, q, dir == 1, Fv, y, n1, e, t);
     }
     Sink c{C + (long)dir * plane_slot + i3 * slice + i, n2};
     for (int k = 0; k < n1; k++) c[k] = t[k];
@@ -575,10 +576,9 @@ int spray3_run(sh_ctx *ctx, int order, int n1, int n2, int n3, int ns2, int ns3,
     // per-thread scratch planes: one per node of a level (ungrouped), or
     // MAX_GROUPS x GM per launch (grouped)
     long nthr = gx8 * TB * (long)std::max<size_t>(maxn, (size_t)MAX_GROUPS * GM);
-    int NB1 = 2 * order + 1;
     float *scr = nullptr, *F = nullptr;
     SH_TRY(sh_slot(ctx, SLOT_SPRAY0 + 4, (size_t)nthr * n1 * sizeof(float), (void **)&scr));
-    SH_TRY(sh_slot(ctx, SLOT_SPRAY0 + 8, (size_t)8 * NB1 * n * sizeof(float), (void **)&F));
+    SH_TRY(sh_slot(ctx, SLOT_SPRAY0 + 8, (size_t)8 * fac_plane_floats(2 * order, n) * sizeof(float), (void **)&F));
     {
         PROF_A(ctx);
         SH_DISPATCH_NW(ctx, order, NW,
@@ -600,11 +600,12 @@ int spray3_run(sh_ctx *ctx, int order, int n1, int n2, int n3, int ns2, int ns3,
     const char *xe = getenv("SH_SPRAY_XCD");
     const int xcd = (xe && xe[0] == '0') ? 0 : 1;
     // SH_SPRAY_PF=d: prefetch distance (samples) of the level predictions,
-    // 4, 8 or 0 (no prefetch); identical results.  Default 8 for one source
-    // trace, 4 for two (more registers per sample; at 8 the kernel drops to
-    // one wave per SIMD).  Measured at 512x256x256 (somf3d, r = 4): two
-    // sources 14.4 / 9.1 / 12.0 ms and one source 6.3 / 4.7 / 4.4 ms for
-    // d = 0 / 4 / 8.
+    // 0 (no prefetch), 2, 4, 8, or 6 (two-source levels; the one-source
+    // ones then use 8); identical results.  Default 8 for one source trace,
+    // 6 for two (more registers per sample; at 8 the kernel drops to one
+    // wave per SIMD).  Measured at 512x256x256 (somf3d, r = 4), ms per
+    // warm step: two sources 9.45 / 8.85 for d = 4 / 6 (12.0 at 8 with the
+    // factor in planes), one source 4.63 / 4.42 for d = 4 / 8.
     const char *pe = getenv("SH_SPRAY_PF");
     for (size_t L = 1; L < levels.size(); L++) {
         if (grouped) {  // two-source slots by (quadrant, group size)
@@ -664,16 +665,17 @@ int spray3_run(sh_ctx *ctx, int order, int n1, int n2, int n3, int ns2, int ns3,
     SH_DISPATCH_NW(ctx, order, NW,                                                                            \
                    hipLaunchKernelGGL((spray3_level<NW, NF_, PF_>), grid, dim3(TB), 0, ctx->stream, piT, pxT, U, \
                                       F, lv, cnt, xcd, n1, n2, n3, np2, e, scr, nthr))
-            const int pf = pe ? atoi(pe) : (nf == 1 ? 8 : 4);
+            const int pf = pe ? atoi(pe) : (nf == 1 ? 8 : 6);
             // timer names carry the kernel's trailing template arguments
             // (sources, prefetch distance)
             if (nf == 1) {
-                if (pf == 8) { SL(1, 8); PROF_B(ctx, "spray3_level<1,8>", by); }
+                if (pf == 8 || pf == 6) { SL(1, 8); PROF_B(ctx, "spray3_level<1,8>", by); }
                 else if (pf == 4) { SL(1, 4); PROF_B(ctx, "spray3_level<1,4>", by); }
                 else if (pf == 2) { SL(1, 2); PROF_B(ctx, "spray3_level<1,2>", by); }
                 else { SL(1, 0); PROF_B(ctx, "spray3_level<1,0>", by); }
             } else {
                 if (pf == 8) { SL(2, 8); PROF_B(ctx, "spray3_level<2,8>", by); }
+                else if (pf == 6) { SL(2, 6); PROF_B(ctx, "spray3_level<2,6>", by); }
                 else if (pf == 4) { SL(2, 4); PROF_B(ctx, "spray3_level<2,4>", by); }
                 else if (pf == 2) { SL(2, 2); PROF_B(ctx, "spray3_level<2,2>", by); }
                 else { SL(2, 0); PROF_B(ctx, "spray3_level<2,0>", by); }
@@ -768,11 +770,10 @@ int pws2_prepare_bufs(sh_ctx *ctx, Pwsmooth2 &P, float *scr, float *F) {
 
 int pws2_prepare(sh_ctx *ctx, Pwsmooth2 &P, int slot_scr, int slot_F) {
     long n = (long)P.n1 * P.n2 * P.n3;
-    int NB1 = 2 * P.order + 1;
     const long nthr = pws2_nthr(P.n2, P.n3);
     float *scr = nullptr, *F = nullptr;
     SH_TRY(sh_slot(ctx, slot_scr, (size_t)nthr * P.n1 * 2 * sizeof(float), (void **)&scr));
-    SH_TRY(sh_slot(ctx, slot_F, (size_t)2 * NB1 * n * sizeof(float), (void **)&F));
+    SH_TRY(sh_slot(ctx, slot_F, (size_t)2 * fac_plane_floats(2 * P.order, n) * sizeof(float), (void **)&F));
     return pws2_prepare_bufs(ctx, P, scr, F);
 }
 
@@ -1212,7 +1213,7 @@ Somf2dWs somf2d_ws(int n1, int n2, int n3, int ns, int order) {
     w.b = up(3 * n);
     w.U = up(n * (2 * ns + 1));
     w.scr = up((size_t)pws2_nthr(n2, n3) * n1 * 2);
-    w.F = up((size_t)2 * (2 * order + 1) * n);
+    w.F = up((size_t)2 * fac_plane_floats(2 * order, (long)n));
     w.total = w.b + w.U + w.scr + w.F;
     return w;
 }

@@ -31,6 +31,6 @@ def test_committed_traffic_file_parses():
     if os.path.exists(path):
         # the library's timer names for the warm path's level launches
         # (sources, prefetch distance) must resolve in the committed file
-        for name in ("spray3_level<2,4>", "spray3_level<1,8>"):
+        for name in ("spray3_level<2,6>", "spray3_level<1,8>"):
             t = bench.measured_traffic(name, path)
             assert t is not None and t > 0, name
