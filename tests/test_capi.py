@@ -91,6 +91,7 @@ def test_batch_tile_picker():
         ns[other] = 1.0 - gain
         return {"tiles": [list(t) for t in tiles], "buckets": {"128x64": {"ns_per_sample": ns}}}
 
-    assert batch.pick_tile([(128, 64)] * 4, table(0.01)) == d
-    assert batch.pick_tile([(128, 64)] * 4, table(0.10)) == other
-    assert batch.pick_tile([(100, 70)], table(0.10), min_gain=0.2) == d
+    assert batch.pick_tile([(128, 64)] * 4, table(0.05)) == d
+    assert batch.pick_tile([(128, 64)] * 4, table(0.20)) == other
+    assert batch.pick_tile([(128, 64)] * 4, table(0.05), min_gain=0.03) == other
+    assert batch.pick_tile([(100, 70)], table(0.20), min_gain=0.3) == d
