@@ -617,6 +617,10 @@ def main():
                 nc = len(cold_err)
             info["cold_rmse_at_that_niter"] = round(cold_err[nc - 1], 5)
             errs[f"cold(niter={nc})"] = round(cold_true[nc - 1], 5)
+            # the cold start at the warm start's error against the true slopes
+            w_true = errs[f"warm(niter={nw})"]
+            errs["cold niter at the warm start's true-slope error"] = next(
+                (j + 1 for j, e in enumerate(cold_true) if e <= w_true), None)
         info["rmse_vs_true_slopes"] = errs
 
     def cold_step():
