@@ -75,6 +75,30 @@ SH_HD inline void fac_put(const FactorOut &F, long k, float dk, const float *ok)
     }
 }
 
+// Filter taps of a slope plane stored like a factor (taps[0..NB) of
+// sample k side by side at o[NB * k * fs], taps[NB] at d[k * fs]): a[j],
+// reversed when rev, zero outside the trace (taps_at on the stored values).
+template <int NW>
+SH_HD inline void tap_row(const Factor &T, long k, float *b) {
+    constexpr int NB = 2 * NW;
+    fac_row<NB>(T, k, b);
+    b[NB] = T.d[k * T.fs];
+}
+template <int NW>
+SH_HD inline void taps_stored(const float *b, int kk, int n1, bool rev, float *a) {
+    constexpr int NA = 2 * NW + 1;
+    const bool in = kk >= 0 && kk < n1;
+#pragma unroll
+    for (int j = 0; j < NA; j++) a[j] = in ? (rev ? b[NA - 1 - j] : b[j]) : 0.0f;
+}
+
+// compile-time flag passed to the step lambdas (interior samples: no
+// boundary tests)
+template <bool V>
+struct Mid {
+    static constexpr bool value = V;
+};
+
 // Floats per factor variant of a plane of n elements (the off-diagonal
 // block first, so it starts 16-byte aligned when the buffer does; the
 // variant stride is kept a multiple of 4 floats).
@@ -718,9 +742,9 @@ SH_HD void solve_factored(const Acc *in, const Acc *q, const bool *rev, const Fa
 // their index into the trace (they are only consumed where the unclamped
 // index is valid).  The arithmetic is solve_factored's, operation for
 // operation, so the results are identical.
-template <int NW, int NF, int D>
+template <int NW, int NF, int D, bool TP = false, bool MIDR = false>
 SH_HD void solve_factored_pf(const Acc *in, const Acc *q, const bool *rev, const Factor &F, Sink y, int n1, float e,
-                             Sink out) {
+                             Sink out, const Factor *tq = nullptr) {
     constexpr int NA = 2 * NW + 1, NB = 2 * NW;
     float tb[4];
     if (NF == 1) {
@@ -736,7 +760,15 @@ SH_HD void solve_factored_pf(const Acc *in, const Acc *q, const bool *rev, const
 #pragma unroll
     for (int f = 0; f < NF; f++) {
 #pragma unroll
-        for (int w = 0; w < NA; w++) taps_at<NW>(q[f], w - NW, n1, rev[f], A[f][w]);
+        for (int w = 0; w < NA; w++) {
+            if constexpr (TP) {
+                float b[NA];
+                tap_row<NW>(tq[f], (w - NW) >= 0 && (w - NW) < n1 ? w - NW : 0, b);
+                taps_stored<NW>(b, w - NW, n1, rev[f], A[f][w]);
+            } else {
+                taps_at<NW>(q[f], w - NW, n1, rev[f], A[f][w]);
+            }
+        }
 #pragma unroll
         for (int w = 0; w < NA; w++) I[f][w] = (w < n1) ? in[f][w] : 0.0f;
 #pragma unroll
@@ -756,8 +788,10 @@ SH_HD void solve_factored_pf(const Acc *in, const Acc *q, const bool *rev, const
 #pragma unroll
         for (int m = 0; m < NB; m++) Lo[r][m] = 0.0f;
     }
-    // forward ring: d[k], o[m][k], slope at k + 1 + NW, input at k + 1 + 2 NW
-    float rd[D], ro[D][NB], rq[D][NF], ri[D][NF];
+    // forward ring: d[k], o[m][k], slope (or its taps) at k + 1 + NW,
+    // input at k + 1 + 2 NW
+    constexpr int NQ = TP ? NA : 1;
+    float rd[D], ro[D][NB], rq[D][NF][NQ], ri[D][NF];
     auto fetch = [&](int s, int k) {
         const long kc = k < n1 ? k : n1 - 1;
         rd[s] = F.d[kc * F.fs];
@@ -766,11 +800,15 @@ SH_HD void solve_factored_pf(const Acc *in, const Acc *q, const bool *rev, const
         const int ki = k + 1 + 2 * NW < n1 ? k + 1 + 2 * NW : n1 - 1;
 #pragma unroll
         for (int f = 0; f < NF; f++) {
-            rq[s][f] = q[f][kq];
+            if constexpr (TP) tap_row<NW>(tq[f], kq, rq[s][f]);
+            else rq[s][f][0] = q[f][kq];
             ri[s][f] = in[f][ki];
         }
     };
-    auto step = [&](int s, int k) {
+    // MID: a sample of [2 NW, n1 - 2 NW - 1), where every boundary test
+    // below is known to pass or fail (same operations as the general step)
+    auto step = [&](auto mid, int s, int k) {
+        constexpr bool MID = decltype(mid)::value;
         float rhs = 0.0f;
 #pragma unroll
         for (int f = 0; f < NF; f++) {
@@ -778,15 +816,17 @@ SH_HD void solve_factored_pf(const Acc *in, const Acc *q, const bool *rev, const
 #pragma unroll
             for (int j = 0; j < NA; j++) {
                 int kk = k + j - NW;
-                if (kk >= NW && kk < n1 - NW) of += A[f][j][j] * T[f][j];
+                if (MID || (kk >= NW && kk < n1 - NW)) of += A[f][j][j] * T[f][j];
             }
             rhs = (f == 0) ? of : (rhs + of);
         }
-        if (k == 0) rhs += e * tb[0];
-        if (k == 1) rhs += e * tb[1];
-        if (k == n1 - 2) rhs += e * tb[2];
-        if (k == n1 - 1) rhs += e * tb[3];
-        int m1 = k < NB ? k : NB;
+        if (!MID) {
+            if (k == 0) rhs += e * tb[0];
+            if (k == 1) rhs += e * tb[1];
+            if (k == n1 - 2) rhs += e * tb[2];
+            if (k == n1 - 1) rhs += e * tb[3];
+        }
+        int m1 = MID ? NB : (k < NB ? k : NB);
         float b = rhs;
 #pragma unroll
         for (int m = 0; m < NB; m++)
@@ -814,12 +854,21 @@ SH_HD void solve_factored_pf(const Acc *in, const Acc *q, const bool *rev, const
                 I[f][w] = I[f][w + 1];
                 T[f][w] = T[f][w + 1];
             }
-            taps_from<NW>(rq[s][f], k + 1 + NW, n1, rev[f], A[f][NA - 1]);
+            if constexpr (TP) {
+                if (MID) {
+#pragma unroll
+                    for (int j = 0; j < NA; j++) A[f][NA - 1][j] = rev[f] ? rq[s][f][NA - 1 - j] : rq[s][f][j];
+                } else {
+                    taps_stored<NW>(rq[s][f], k + 1 + NW, n1, rev[f], A[f][NA - 1]);
+                }
+            } else {
+                taps_from<NW>(rq[s][f][0], k + 1 + NW, n1, rev[f], A[f][NA - 1]);
+            }
             int kn = k + 1 + 2 * NW;
-            I[f][NA - 1] = (kn < n1) ? ri[s][f] : 0.0f;
+            I[f][NA - 1] = (MID || kn < n1) ? ri[s][f] : 0.0f;
             int kk = k + 1 + NW;
             float tn = 0.0f;
-            if (kk >= NW && kk < n1 - NW) {
+            if (MID || (kk >= NW && kk < n1 - NW)) {
 #pragma unroll
                 for (int j = 0; j < NA; j++) tn += A[f][NA - 1][j] * I[f][j];
             }
@@ -831,12 +880,17 @@ SH_HD void solve_factored_pf(const Acc *in, const Acc *q, const bool *rev, const
     for (int s = 0; s < D; s++) fetch(s, s);
     int k0 = 0;
     for (; k0 + D <= n1; k0 += D) {
+        if (MIDR && k0 >= 2 * NW && k0 + D <= n1 - 2 * NW - 1) {
 #pragma unroll
-        for (int s = 0; s < D; s++) step(s, k0 + s);
+            for (int s = 0; s < D; s++) step(Mid<true>{}, s, k0 + s);
+        } else {
+#pragma unroll
+            for (int s = 0; s < D; s++) step(Mid<false>{}, s, k0 + s);
+        }
     }
 #pragma unroll
     for (int s = 0; s < D; s++)
-        if (k0 + s < n1) step(s, k0 + s);
+        if (k0 + s < n1) step(Mid<false>{}, s, k0 + s);
     // backward ring: y[k], o[m][k]
     float nx[NB];
 #pragma unroll
@@ -847,9 +901,11 @@ SH_HD void solve_factored_pf(const Acc *in, const Acc *q, const bool *rev, const
         ry[s] = y[kc];
         fac_row<NB>(F, kc, rb[s]);
     };
-    auto stepb = [&](int s, int k) {
+    // MID: k <= n1 - 1 - NB (every band entry below the diagonal is used)
+    auto stepb = [&](auto mid, int s, int k) {
+        constexpr bool MID = decltype(mid)::value;
         float t = ry[s];
-        int m1 = (n1 - k - 1) < NB ? (n1 - k - 1) : NB;
+        int m1 = MID ? NB : ((n1 - k - 1) < NB ? (n1 - k - 1) : NB);
 #pragma unroll
         for (int m = 0; m < NB; m++)
             if (m < m1) t -= rb[s][m] * nx[m];
@@ -863,12 +919,17 @@ SH_HD void solve_factored_pf(const Acc *in, const Acc *q, const bool *rev, const
     for (int s = 0; s < D; s++) fetchb(s, n1 - 1 - s);
     int kb = n1 - 1;
     for (; kb - D + 1 >= 0; kb -= D) {
+        if (MIDR && kb <= n1 - 1 - NB) {
 #pragma unroll
-        for (int s = 0; s < D; s++) stepb(s, kb - s);
+            for (int s = 0; s < D; s++) stepb(Mid<true>{}, s, kb - s);
+        } else {
+#pragma unroll
+            for (int s = 0; s < D; s++) stepb(Mid<false>{}, s, kb - s);
+        }
     }
 #pragma unroll
     for (int s = 0; s < D; s++)
-        if (kb - s >= 0) stepb(s, kb - s);
+        if (kb - s >= 0) stepb(Mid<false>{}, s, kb - s);
 }
 
 // M forward predictions that share the slopes, the directions and the
@@ -1076,9 +1137,9 @@ SH_HD void predict_adj_factored(const Acc &in, const Acc &q, bool rev, const Fac
 // running trace plus the slot it gathers, added in that order).  Same
 // float operations in the same order as predict_adj_factored; y holds the
 // solved trace, out may alias a.
-template <int NW, int D>
+template <int NW, int D, bool TP = false, bool MIDR = false>
 SH_HD void predict_adj_factored_pf(const Acc &a, const Acc *b, const Acc &q, bool rev, const Factor &F, Sink y,
-                                   int n1, float e, Sink out) {
+                                   int n1, float e, Sink out, const Factor *tq = nullptr) {
     constexpr int NB = 2 * NW, NA = 2 * NW + 1;
     // forward substitution: ring of d[k], o[m][k] and the input at k
     {
@@ -1096,8 +1157,9 @@ SH_HD void predict_adj_factored_pf(const Acc &a, const Acc *b, const Acc &q, boo
             fac_row<NB>(F, kc, ro[s]);
             ri[s] = b ? (a[kc] + (*b)[kc]) : a[kc];
         };
-        auto step = [&](int s, int k) {
-            const int m1 = k < NB ? k : NB;
+        auto step = [&](auto mid, int s, int k) {
+            constexpr bool MID = decltype(mid)::value;  // k >= NB
+            const int m1 = MID ? NB : (k < NB ? k : NB);
             float bb = ri[s];
 #pragma unroll
             for (int m = 0; m < NB; m++)
@@ -1118,12 +1180,17 @@ SH_HD void predict_adj_factored_pf(const Acc &a, const Acc *b, const Acc &q, boo
         for (int s = 0; s < D; s++) fetch(s, s);
         int k0 = 0;
         for (; k0 + D <= n1; k0 += D) {
+            if (MIDR && k0 >= NB) {
 #pragma unroll
-            for (int s = 0; s < D; s++) step(s, k0 + s);
+                for (int s = 0; s < D; s++) step(Mid<true>{}, s, k0 + s);
+            } else {
+#pragma unroll
+                for (int s = 0; s < D; s++) step(Mid<false>{}, s, k0 + s);
+            }
         }
 #pragma unroll
         for (int s = 0; s < D; s++)
-            if (k0 + s < n1) step(s, k0 + s);
+            if (k0 + s < n1) step(Mid<false>{}, s, k0 + s);
     }
     // backward substitution in place: ring of y[k], o[m][k]
     {
@@ -1136,9 +1203,10 @@ SH_HD void predict_adj_factored_pf(const Acc &a, const Acc *b, const Acc &q, boo
             ry[s] = y[kc];
             fac_row<NB>(F, kc, rb[s]);
         };
-        auto stepb = [&](int s, int k) {
+        auto stepb = [&](auto mid, int s, int k) {
+            constexpr bool MID = decltype(mid)::value;  // k <= n1 - 1 - NB
             float t = ry[s];
-            const int m1 = (n1 - k - 1) < NB ? (n1 - k - 1) : NB;
+            const int m1 = MID ? NB : ((n1 - k - 1) < NB ? (n1 - k - 1) : NB);
 #pragma unroll
             for (int m = 0; m < NB; m++)
                 if (m < m1) t -= rb[s][m] * nx[m];
@@ -1152,12 +1220,17 @@ SH_HD void predict_adj_factored_pf(const Acc &a, const Acc *b, const Acc &q, boo
         for (int s = 0; s < D; s++) fetchb(s, n1 - 1 - s);
         int kb = n1 - 1;
         for (; kb - D + 1 >= 0; kb -= D) {
+            if (MIDR && kb <= n1 - 1 - NB) {
 #pragma unroll
-            for (int s = 0; s < D; s++) stepb(s, kb - s);
+                for (int s = 0; s < D; s++) stepb(Mid<true>{}, s, kb - s);
+            } else {
+#pragma unroll
+                for (int s = 0; s < D; s++) stepb(Mid<false>{}, s, kb - s);
+            }
         }
 #pragma unroll
         for (int s = 0; s < D; s++)
-            if (kb - s >= 0) stepb(s, kb - s);
+            if (kb - s >= 0) stepb(Mid<false>{}, s, kb - s);
     }
     // out = A^T A s + e * boundary(s) (pred_adj_apply): ring of the slope
     // at k + 1 + NW and the solved sample at k + 1 + 2 NW
@@ -1165,7 +1238,15 @@ SH_HD void predict_adj_factored_pf(const Acc &a, const Acc *b, const Acc &q, boo
         const Acc s{y.p, y.s};
         float A[NA][NA], S[NA], T[NA];
 #pragma unroll
-        for (int w = 0; w < NA; w++) taps_at<NW>(q, w - NW, n1, rev, A[w]);
+        for (int w = 0; w < NA; w++) {
+            if constexpr (TP) {
+                float bt[NA];
+                tap_row<NW>(*tq, (w - NW) >= 0 && (w - NW) < n1 ? w - NW : 0, bt);
+                taps_stored<NW>(bt, w - NW, n1, rev, A[w]);
+            } else {
+                taps_at<NW>(q, w - NW, n1, rev, A[w]);
+            }
+        }
 #pragma unroll
         for (int w = 0; w < NA; w++) S[w] = (w < n1) ? s[w] : 0.0f;
 #pragma unroll
@@ -1179,24 +1260,30 @@ SH_HD void predict_adj_factored_pf(const Acc &a, const Acc *b, const Acc &q, boo
             T[w] = t;
         }
         const float s0 = s[0], s1 = s[1], s2 = s[n1 - 2], s3 = s[n1 - 1];
-        float rq[D], rs[D];
+        constexpr int NQ = TP ? NA : 1;
+        float rq[D][NQ], rs[D];
         auto fetcha = [&](int r, int k) {
             const int kq = k + 1 + NW < n1 ? k + 1 + NW : n1 - 1;
             const int ks = k + 1 + 2 * NW < n1 ? k + 1 + 2 * NW : n1 - 1;
-            rq[r] = q[kq];
+            if constexpr (TP) tap_row<NW>(*tq, kq, rq[r]);
+            else rq[r][0] = q[kq];
             rs[r] = s[ks];
         };
-        auto stepa = [&](int r, int k) {
+        // MID: k in [2 NW, n1 - 2 NW - 1), every boundary test known
+        auto stepa = [&](auto mid, int r, int k) {
+            constexpr bool MID = decltype(mid)::value;
             float v = 0.0f;
 #pragma unroll
             for (int w = 0; w < NA; w++) {
                 const int i = k - NW + w;
-                if (i >= NW && i < n1 - NW) v += A[w][2 * NW - w] * T[w];
+                if (MID || (i >= NW && i < n1 - NW)) v += A[w][2 * NW - w] * T[w];
             }
-            if (k == 0) v += e * s0;
-            if (k == 1) v += e * s1;
-            if (k == n1 - 2) v += e * s2;
-            if (k == n1 - 1) v += e * s3;
+            if (!MID) {
+                if (k == 0) v += e * s0;
+                if (k == 1) v += e * s1;
+                if (k == n1 - 2) v += e * s2;
+                if (k == n1 - 1) v += e * s3;
+            }
             out[k] = v;
 #pragma unroll
             for (int w = 0; w < NA - 1; w++) {
@@ -1205,12 +1292,21 @@ SH_HD void predict_adj_factored_pf(const Acc &a, const Acc *b, const Acc &q, boo
                 T[w] = T[w + 1];
                 S[w] = S[w + 1];
             }
-            taps_from<NW>(rq[r], k + 1 + NW, n1, rev, A[NA - 1]);
+            if constexpr (TP) {
+                if (MID) {
+#pragma unroll
+                    for (int j = 0; j < NA; j++) A[NA - 1][j] = rev ? rq[r][NA - 1 - j] : rq[r][j];
+                } else {
+                    taps_stored<NW>(rq[r], k + 1 + NW, n1, rev, A[NA - 1]);
+                }
+            } else {
+                taps_from<NW>(rq[r][0], k + 1 + NW, n1, rev, A[NA - 1]);
+            }
             const int kn = k + 1 + 2 * NW;
-            S[NA - 1] = (kn < n1) ? rs[r] : 0.0f;
+            S[NA - 1] = (MID || kn < n1) ? rs[r] : 0.0f;
             const int kk = k + 1 + NW;
             float tn = 0.0f;
-            if (kk >= NW && kk < n1 - NW) {
+            if (MID || (kk >= NW && kk < n1 - NW)) {
 #pragma unroll
                 for (int j = NA - 1; j >= 0; j--) tn += A[NA - 1][j] * S[2 * NW - j];
             }
@@ -1221,11 +1317,16 @@ SH_HD void predict_adj_factored_pf(const Acc &a, const Acc *b, const Acc &q, boo
         for (int r = 0; r < D; r++) fetcha(r, r);
         int k0 = 0;
         for (; k0 + D <= n1; k0 += D) {
+            if (MIDR && k0 >= 2 * NW && k0 + D <= n1 - 2 * NW - 1) {
 #pragma unroll
-            for (int r = 0; r < D; r++) stepa(r, k0 + r);
+                for (int r = 0; r < D; r++) stepa(Mid<true>{}, r, k0 + r);
+            } else {
+#pragma unroll
+                for (int r = 0; r < D; r++) stepa(Mid<false>{}, r, k0 + r);
+            }
         }
 #pragma unroll
         for (int r = 0; r < D; r++)
-            if (k0 + r < n1) stepa(r, k0 + r);
+            if (k0 + r < n1) stepa(Mid<false>{}, r, k0 + r);
     }
 }

@@ -352,6 +352,18 @@ factor2_kernel(const float *__restrict__ dip, float *__restrict__ F, int n1, int
     factor_sweep_pf<NW, 1, 8>(q, rev, n1, e, Fs);
 }
 
+// Filter taps of every slope sample, stored like a factor (the chains load
+// them instead of forming them from the slope at every step).
+template <int NW>
+__global__ void taps2_kernel(const float *__restrict__ dip, float *__restrict__ T, long n) {
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    constexpr int NB = 2 * NW;
+    float b[NB + 1];
+    apf_pass<NW>(dip[e], b);
+    fac_put<NB>(fac_out_at(T, NB, n, e, 0), 0, b[NB], b);
+}
+
 template <int NW, int PFD>
 __global__ void __launch_bounds__(TB)
 spray2_fwd(const float *__restrict__ x, const float *__restrict__ dip, const float *__restrict__ F,
@@ -398,7 +410,8 @@ spray2_fwd(const float *__restrict__ x, const float *__restrict__ dip, const flo
         Acc qs[1] = {q};
         bool rv[1] = {dir == 1};
         const Factor Fv = fac_at(F + (long)dir * fac_plane_floats(NB, n), NB, n, i3 * slice + ip, n2);
-        solve_factored_pf<NW, 1, PFD>(in, qs, rv, Fv, y, n1, e, out);
+        const Factor Tq = fac_at(F + 2 * fac_plane_floats(NB, n), NB, n, i3 * slice + (dir == 0 ? ip : ip - 1), n2);
+        solve_factored_pf<NW, 1, PFD, true, true>(in, qs, rv, Fv, y, n1, e, out, &Tq);
         cur = Acc{out.p, n2};
     }
 }
@@ -433,8 +446,9 @@ spray2_adj(const float *__restrict__ U, const float *__restrict__ dip, const flo
         Acc u{U + (long)slot * plane_slot + i3 * slice + ip, n2};
         Acc q{ds + (dir == 1 ? ip - 1 : ip), n2};
         const Factor Fv = fac_at(F + (long)(dir == 1 ? 1 : 0) * fac_plane_floats(NB, n), NB, n, i3 * slice + ip, n2);
+        const Factor Tq = fac_at(F + 2 * fac_plane_floats(NB, n), NB, n, i3 * slice + (dir == 1 ? ip - 1 : ip), n2);
         // t = S^{-1}-adjoint prediction of (t + u), loads issued PFD samples ahead
-        predict_adj_factored_pf<NW, PFD>(Acc{t.p, t.s}, &u, q, dir == 1, Fv, y, n1, e, t);
+        predict_adj_factored_pf<NW, PFD, true, true>(Acc{t.p, t.s}, &u, q, dir == 1, Fv, y, n1, e, t, &Tq);
     }
     Sink c{C + (long)dir * plane_slot + i3 * slice + i, n2};
     for (int k = 0; k < n1; k++) c[k] = t[k];
@@ -727,16 +741,15 @@ int spray2_tb() {
 }
 
 // Prefetch distance (samples) of the chain kernels' load rings, 8 or 16
-// (SH_SPRAY2_PF sets both); identical results.  The chains run at about
-// one wave per SIMD, so the deeper ring's registers cost no occupancy.
-// Measured (sint3d 300x100x100, somean2d 512x256): 16 takes the forward
-// chains 10.0 -> 9.6 ms and 2.01 -> 1.95 ms, the adjoint chains
-// 12.9 -> 13.4 ms, so the forward chains use 16 and the adjoint ones 8.
+// (SH_SPRAY2_PF); identical results.  The chains run at about one wave per
+// SIMD and issue-bound: with the taps loaded from the taps plane (five
+// values per ring slot) a distance of 16 no longer fits the registers
+// (sint3d 300x100x100: chains 17.1 ms at 8, 21.5 ms at 16).
 int spray2_pf(bool forward) {
+    (void)forward;
     const char *e = getenv("SH_SPRAY2_PF");
-    if (e && atoi(e) == 8) return 8;
     if (e && atoi(e) == 16) return 16;
-    return forward ? 16 : 8;
+    return 8;
 }
 
 #define SPRAY2_PF_DISPATCH(PF, FWD, ...)         \
@@ -751,7 +764,9 @@ int spray2_pf(bool forward) {
     } while (0)
 
 // Factor cache of P into caller-provided buffers: scr holds 2*nthr*n1
-// floats (solver scratch, then the running trace planes), F 2*(2*order+1)*n.
+// floats (solver scratch, then the running trace planes), F
+// 3 * fac_plane_floats(2 * order, n): the two factor variants, then the
+// filter taps of the slopes.
 int pws2_prepare_bufs(sh_ctx *ctx, Pwsmooth2 &P, float *scr, float *F) {
     long n = (long)P.n1 * P.n2 * P.n3;
     P.nthr = pws2_nthr(P.n2, P.n3);
@@ -765,7 +780,11 @@ int pws2_prepare_bufs(sh_ctx *ctx, Pwsmooth2 &P, float *scr, float *F) {
                    hipLaunchKernelGGL((factor2_kernel<NW>), grid, dim3(TB2), 0, ctx->stream, P.dip, P.F, P.n1, P.n2,
                                       P.n3, P.e));
     PROF_B(ctx, "factor2_kernel", 4.0 * (double)n);
-    return sh_launch_check(ctx, "factor2_kernel");
+    SH_TRY(sh_launch_check(ctx, "factor2_kernel"));
+    SH_DISPATCH_NW(ctx, P.order, NW,
+                   hipLaunchKernelGGL((taps2_kernel<NW>), dim3(ceil_div(n, TB)), dim3(TB), 0, ctx->stream, P.dip,
+                                      P.F + 2 * fac_plane_floats(2 * NW, n), n));
+    return sh_launch_check(ctx, "taps2_kernel");
 }
 
 int pws2_prepare(sh_ctx *ctx, Pwsmooth2 &P, int slot_scr, int slot_F) {
@@ -773,7 +792,7 @@ int pws2_prepare(sh_ctx *ctx, Pwsmooth2 &P, int slot_scr, int slot_F) {
     const long nthr = pws2_nthr(P.n2, P.n3);
     float *scr = nullptr, *F = nullptr;
     SH_TRY(sh_slot(ctx, slot_scr, (size_t)nthr * P.n1 * 2 * sizeof(float), (void **)&scr));
-    SH_TRY(sh_slot(ctx, slot_F, (size_t)2 * fac_plane_floats(2 * P.order, n) * sizeof(float), (void **)&F));
+    SH_TRY(sh_slot(ctx, slot_F, (size_t)3 * fac_plane_floats(2 * P.order, n) * sizeof(float), (void **)&F));
     return pws2_prepare_bufs(ctx, P, scr, F);
 }
 
@@ -1213,7 +1232,7 @@ Somf2dWs somf2d_ws(int n1, int n2, int n3, int ns, int order) {
     w.b = up(3 * n);
     w.U = up(n * (2 * ns + 1));
     w.scr = up((size_t)pws2_nthr(n2, n3) * n1 * 2);
-    w.F = up((size_t)2 * fac_plane_floats(2 * order, (long)n));
+    w.F = up((size_t)3 * fac_plane_floats(2 * order, (long)n));
     w.total = w.b + w.U + w.scr + w.F;
     return w;
 }
