@@ -672,6 +672,15 @@ def main():
         c5_ms, _, _ = timed(cold5_step, args.steps, False)
     info["cold5_ms_per_step"] = round(c5_ms, 3)
     info["cold5_samples_per_s"] = round(n / (c5_ms / 1e3), 1)
+    # the cold start at the warm start's error against the true slopes
+    kt = errs.get("cold niter at the warm start's true-slope error")
+    if use_warm and kt:
+        def cold_true_step():
+            dip.dip_device(d, shape, niter=kt, out=dips, **kw)
+            filters.somf3d_device(d, dips[:n], dips[n:], shape, 4, 4, order=2, out=out)
+        ct_ms, _, _ = timed(cold_true_step, args.steps, False)
+        info["cold_at_warm_true_error_ms_per_step"] = round(ct_ms, 3)
+        info["speedup_warm_vs_cold_at_true_error"] = round(ct_ms / ms_step, 3)
     # Per-kernel times: a separate pass of the same steps with the library's
     # HIP-event timers, outside the timed region.  dip3d's two directions
     # run on two streams; their kernels overlap and would each look slower,
