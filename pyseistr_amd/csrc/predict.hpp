@@ -111,6 +111,19 @@ SH_HD inline FactorOut fac_out_at(float *Fv, int nb, long n, long e0, long fs) {
     return FactorOut{Fv + (long)nb * n + e0, Fv + (long)nb * e0, fs};
 }
 
+// Filter taps of every sample of a slope plane of n elements, stored like a
+// factor (chain kernels load them instead of forming them from the slope at
+// every step).
+template <int NW>
+__global__ void taps_plane_kernel(const float *__restrict__ dip, float *__restrict__ T, long n) {
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    constexpr int NB = 2 * NW;
+    float b[NB + 1];
+    apf_pass<NW>(dip[e], b);
+    fac_put<NB>(fac_out_at(T, NB, n, e, 0), 0, b[NB], b);
+}
+
 // Taps of the filter at sample kk (zero outside the trace), reversed when rev.
 template <int NW>
 SH_HD inline void taps_at(const Acc &q, int kk, int n1, bool rev, float *a) {

@@ -352,18 +352,6 @@ factor2_kernel(const float *__restrict__ dip, float *__restrict__ F, int n1, int
     factor_sweep_pf<NW, 1, 8>(q, rev, n1, e, Fs);
 }
 
-// Filter taps of every slope sample, stored like a factor (the chains load
-// them instead of forming them from the slope at every step).
-template <int NW>
-__global__ void taps2_kernel(const float *__restrict__ dip, float *__restrict__ T, long n) {
-    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= n) return;
-    constexpr int NB = 2 * NW;
-    float b[NB + 1];
-    apf_pass<NW>(dip[e], b);
-    fac_put<NB>(fac_out_at(T, NB, n, e, 0), 0, b[NB], b);
-}
-
 template <int NW, int PFD>
 __global__ void __launch_bounds__(TB)
 spray2_fwd(const float *__restrict__ x, const float *__restrict__ dip, const float *__restrict__ F,
@@ -782,9 +770,9 @@ int pws2_prepare_bufs(sh_ctx *ctx, Pwsmooth2 &P, float *scr, float *F) {
     PROF_B(ctx, "factor2_kernel", 4.0 * (double)n);
     SH_TRY(sh_launch_check(ctx, "factor2_kernel"));
     SH_DISPATCH_NW(ctx, P.order, NW,
-                   hipLaunchKernelGGL((taps2_kernel<NW>), dim3(ceil_div(n, TB)), dim3(TB), 0, ctx->stream, P.dip,
+                   hipLaunchKernelGGL((taps_plane_kernel<NW>), dim3(ceil_div(n, TB)), dim3(TB), 0, ctx->stream, P.dip,
                                       P.F + 2 * fac_plane_floats(2 * NW, n), n));
-    return sh_launch_check(ctx, "taps2_kernel");
+    return sh_launch_check(ctx, "taps_plane_kernel");
 }
 
 int pws2_prepare(sh_ctx *ctx, Pwsmooth2 &P, int slot_scr, int slot_F) {
