@@ -628,7 +628,7 @@ def main():
         filters.somf3d_device(d, dips[:n], dips[n:], shape, 4, 4, order=2, out=out)
 
     def warm_step():
-        net.predict_device(d, shape, out=init_buf)  # includes the input RMS (one host sync)
+        net.predict_device(d, shape, out=init_buf)  # input scaled by 1/RMS on the device
         dip.dip_device(d, shape, niter=nw, init=init_buf, out=dips, **kw)
         filters.somf3d_device(d, dips[:n], dips[n:], shape, 4, 4, order=2, out=out)
 

@@ -284,7 +284,9 @@ int sh_somf2d_batched(sh_ctx *ctx, const float *data, const float *dip, const lo
 long sh_cnn_nparams(void);
 /* dil (nullable: all 1): dilation of each of the six layers, 1..8 (tap t
  * then covers the offset dil * (t%3-1, (t/3)%3-1, t/9-1)). */
-/* dip[2N] = inline || crossline prediction for the volume x[N] * xscale. */
+/* dip[2N] = inline || crossline prediction for the volume x[N] * xscale;
+ * xscale <= 0: the scale is 1 / RMS of x (1 for an all-zero x), computed
+ * on the device, so the call needs no host round trip. */
 int sh_cnn_forward(sh_ctx *ctx, const float *params, const float *x, float xscale,
                    const int *dil, int n1, int n2, int n3, float *dip);
 /* loss = sum wt (y - target)^2 / (2N) over both components (target[2N],

@@ -85,7 +85,8 @@ __global__ void pack_weights(const float *__restrict__ P, __bf16 *__restrict__ W
 // the FMAs as scalar operands (no LDS traffic per FMA).
 __global__ void __launch_bounds__(TB) conv1_fwd(const float *__restrict__ x, float xscale, const float *__restrict__ P,
                                                 __bf16 *__restrict__ out, int n1, int n2, int n3, int dil,
-                                                FastDiv f1, FastDiv f2) {
+                                                FastDiv f1, FastDiv f2, const float *__restrict__ xsp) {
+    if (xsp) xscale = xsp[0];  // the scale computed on the device
     const float *__restrict__ w = P + OFF_W1;
     const float *__restrict__ b = P + OFF_B1;
     long n = (long)n1 * n2 * n3;
@@ -728,6 +729,31 @@ __global__ void adam_kernel(float *__restrict__ P, const float *__restrict__ G, 
     P[i] -= lr * (m / c1) / (sqrtf(v / c2) + eps);
 }
 
+// 1 / RMS of x on the device (1 when x is all zero), in two launches: block
+// partial sums of x^2 in double, then one block adds them in a fixed order.
+__global__ void __launch_bounds__(TB) sumsq_partials(const float *__restrict__ x, long n, double *__restrict__ part) {
+    __shared__ double sh[16];
+    double s = 0.0;
+    for (long i = (long)blockIdx.x * TB + threadIdx.x; i < n; i += (long)gridDim.x * TB) {
+        const double v = x[i];
+        s += v * v;
+    }
+    const double t = block_sum(s, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+__global__ void __launch_bounds__(1024) inv_rms(const double *__restrict__ part, int nblk, long n,
+                                                float *__restrict__ xs) {
+    __shared__ double sh[16];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < nblk; i += blockDim.x) s += part[i];
+    const double t = block_sum(s, sh);
+    if (threadIdx.x == 0) {
+        const double rms = sqrt(t / (double)n);
+        xs[0] = rms > 0.0 ? (float)(1.0 / rms) : 1.0f;
+    }
+}
+
 // (y interleaved [v][2]) -> dips [2][v]
 __global__ void split_dips(const float *__restrict__ y, float *__restrict__ out, long n) {
     long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -826,14 +852,14 @@ int conv_launch(sh_ctx *ctx, int mode, const __bf16 *in, const __bf16 *Wg, const
 }
 
 int cnn_forward(sh_ctx *ctx, const float *P, const float *x, float xscale, const int *D, int n1, int n2, int n3,
-                CnnWork &W, bool train) {
+                CnnWork &W, bool train, const float *xsp = nullptr) {
     long n = (long)n1 * n2 * n3;
     long per = (long)NT * C * C;
     hipLaunchKernelGGL(pack_weights, dim3(ceil_div(5 * per, TB)), dim3(TB), 0, ctx->stream, P, W.Wf, W.Wb);
     SH_TRY(sh_launch_check(ctx, "pack_weights"));
     PROF_A(ctx);
     hipLaunchKernelGGL(conv1_fwd, dim3(ceil_div(n, TB)), dim3(TB), 0, ctx->stream, x, xscale, P, W.act[1], n1, n2, n3,
-                       D[0], FastDiv((unsigned)n1), FastDiv((unsigned)n2));
+                       D[0], FastDiv((unsigned)n1), FastDiv((unsigned)n2), xsp);
     PROF_B(ctx, "conv1_fwd", (double)n * (4.0 + 64.0));
     SH_TRY(sh_launch_check(ctx, "conv1_fwd"));
     __bf16 *cur = W.act[1];
@@ -874,7 +900,17 @@ int sh_cnn_forward(sh_ctx *ctx, const float *params, const float *x, float xscal
     SH_TRY(read_dil(ctx, dil, D));
     CnnWork W{};
     SH_TRY(cnn_work(ctx, n, false, W, n1, n2, n3));
-    SH_TRY(cnn_forward(ctx, params, x, xscale, D, n1, n2, n3, W, false));
+    float *xsp = nullptr;
+    if (!(xscale > 0.0f)) {  // scale by 1 / RMS of x, computed here (no host round trip)
+        const int nb = (int)std::min<long>(ceil_div(n, TB), 4096L);
+        double *part = nullptr;
+        SH_TRY(sh_slot(ctx, SLOT_INTERP0 + 16, (size_t)nb * sizeof(double) + 256, (void **)&part));
+        xsp = (float *)(part + nb);
+        hipLaunchKernelGGL(sumsq_partials, dim3(nb), dim3(TB), 0, ctx->stream, x, n, part);
+        hipLaunchKernelGGL(inv_rms, dim3(1), dim3(1024), 0, ctx->stream, part, nb, n, xsp);
+        SH_TRY(sh_launch_check(ctx, "inv_rms"));
+    }
+    SH_TRY(cnn_forward(ctx, params, x, xscale, D, n1, n2, n3, W, false, xsp));
     hipLaunchKernelGGL(split_dips, dim3(ceil_div(n, TB)), dim3(TB), 0, ctx->stream, W.y, dip, n);
     return sh_launch_check(ctx, "split_dips");
 }

@@ -132,10 +132,9 @@ class WarmStart:
         return self._dev
 
     def predict_device(self, d, shape, out=None, xscale=None):
-        """Initial dips [2N] for the flat Fortran-order device volume d."""
-        if xscale is None:
-            xscale = input_scale(d)
-        return cnn_forward(self.params, d, shape, xscale, out, dil=self.dil)
+        """Initial dips [2N] for the flat Fortran-order device volume d
+        (xscale None: the library scales d by 1 / RMS on the device)."""
+        return cnn_forward(self.params, d, shape, 0.0 if xscale is None else xscale, out, dil=self.dil)
 
     def dip_device(self, d, shape, niter=2, liter=10, order=2, rect=(5, 5, 5), out=None, init=None, report=False):
         """Warm-started dip3d: CNN prediction, then niter GN iterations."""

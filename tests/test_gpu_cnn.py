@@ -179,6 +179,23 @@ def test_training_reduces_loss(ws):
     assert np.isfinite(l1) and l1 < 0.3 * l0
 
 
+def test_device_input_scale(ws):
+    """xscale <= 0 makes the library scale the input by 1 / RMS on the
+    device: the same prediction as the host-computed scale (to float
+    rounding of the scale), and scale 1 for an all-zero volume."""
+    import torch
+    shape = (40, 18, 12)
+    rng = np.random.default_rng(8)
+    p = _dev(K.init_params(2) + 0.02 * rng.standard_normal(K.NPARAM).astype(np.float32))
+    d = torch.from_numpy((3.0 * rng.standard_normal(int(np.prod(shape)))).astype(np.float32)).cuda()
+    host = ws.cnn_forward(p, d, shape, ws.input_scale(d)).cpu().numpy()
+    dev = ws.cnn_forward(p, d, shape, 0.0).cpu().numpy()
+    assert np.allclose(dev, host, rtol=1e-4, atol=1e-5), np.abs(dev - host).max()
+    z = torch.zeros_like(d)
+    np.testing.assert_array_equal(ws.cnn_forward(p, z, shape, 0.0).cpu().numpy(),
+                                  ws.cnn_forward(p, z, shape, 1.0).cpu().numpy())
+
+
 def test_errors(ws):
     import torch
     from pyseistr_amd import _lib
