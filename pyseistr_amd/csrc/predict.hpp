@@ -61,6 +61,8 @@ SH_HD inline void fac_row(const Factor &F, long k, float *ok) {
     }
 }
 
+SH_HD inline float fac_d(const Factor &F, long k) { return F.d[k * F.fs]; }
+
 template <int NB>
 SH_HD inline void fac_put(const FactorOut &F, long k, float dk, const float *ok) {
     F.d[k * F.fs] = dk;
@@ -78,6 +80,20 @@ SH_HD inline void fac_put(const FactorOut &F, long k, float dk, const float *ok)
 // Filter taps of a slope plane stored like a factor (taps[0..NB) of
 // sample k side by side at o[NB * k * fs], taps[NB] at d[k * fs]): a[j],
 // reversed when rev, zero outside the trace (taps_at on the stored values).
+// a = b, or b reversed when rev.  Written as a branch on rev (uniform in
+// every caller): as a select per tap the compiler turns the pair of array
+// reads into one read at a variable index, a chain of selects per tap.
+template <int NA>
+SH_HD inline void tap_order(const float *b, bool rev, float *a) {
+    if (rev) {
+#pragma unroll
+        for (int j = 0; j < NA; j++) a[j] = b[NA - 1 - j];
+    } else {
+#pragma unroll
+        for (int j = 0; j < NA; j++) a[j] = b[j];
+    }
+}
+
 template <int NW>
 SH_HD inline void tap_row(const Factor &T, long k, float *b) {
     constexpr int NB = 2 * NW;
@@ -87,9 +103,12 @@ SH_HD inline void tap_row(const Factor &T, long k, float *b) {
 template <int NW>
 SH_HD inline void taps_stored(const float *b, int kk, int n1, bool rev, float *a) {
     constexpr int NA = 2 * NW + 1;
-    const bool in = kk >= 0 && kk < n1;
+    if (kk >= 0 && kk < n1) {
+        tap_order<NA>(b, rev, a);
+    } else {
 #pragma unroll
-    for (int j = 0; j < NA; j++) a[j] = in ? (rev ? b[NA - 1 - j] : b[j]) : 0.0f;
+        for (int j = 0; j < NA; j++) a[j] = 0.0f;
+    }
 }
 
 // compile-time flag passed to the step lambdas (interior samples: no
@@ -125,8 +144,8 @@ __global__ void taps_plane_kernel(const float *__restrict__ dip, float *__restri
 }
 
 // Taps of the filter at sample kk (zero outside the trace), reversed when rev.
-template <int NW>
-SH_HD inline void taps_at(const Acc &q, int kk, int n1, bool rev, float *a) {
+template <int NW, class QA = Acc>
+SH_HD inline void taps_at(const QA &q, int kk, int n1, bool rev, float *a) {
     constexpr int NA = 2 * NW + 1;
     if (kk < 0 || kk >= n1) {
 #pragma unroll
@@ -135,8 +154,7 @@ SH_HD inline void taps_at(const Acc &q, int kk, int n1, bool rev, float *a) {
     }
     float b[NA];
     apf_pass<NW>(q[kk], b);
-#pragma unroll
-    for (int j = 0; j < NA; j++) a[j] = rev ? b[NA - 1 - j] : b[j];
+    tap_order<NA>(b, rev, a);
 }
 
 // Regulariser row k (regularization, sof_cfuns.c:548-565).
@@ -531,8 +549,7 @@ SH_HD inline void taps_from(float qv, int kk, int n1, bool rev, float *a) {
     }
     float b[NA];
     apf_pass<NW>(qv, b);
-#pragma unroll
-    for (int j = 0; j < NA; j++) a[j] = rev ? b[NA - 1 - j] : b[j];
+    tap_order<NA>(b, rev, a);
 }
 
 // factor_sweep with the slope loads software-pipelined D samples ahead (a
@@ -755,9 +772,13 @@ SH_HD void solve_factored(const Acc *in, const Acc *q, const bool *rev, const Fa
 // their index into the trace (they are only consumed where the unclamped
 // index is valid).  The arithmetic is solve_factored's, operation for
 // operation, so the results are identical.
-template <int NW, int NF, int D, bool TP = false, bool MIDR = false>
-SH_HD void solve_factored_pf(const Acc *in, const Acc *q, const bool *rev, const Factor &F, Sink y, int n1, float e,
-                             Sink out, const Factor *tq = nullptr) {
+// The input, slope, factor and scratch / output accessors are template
+// types (IA, FA, SK): Acc / Factor / Sink, or the buffer-resource ones of
+// the 3-D level kernels; the arithmetic does not depend on them.
+template <int NW, int NF, int D, bool TP = false, bool MIDR = false, class IA = Acc, class FA = Factor,
+          class SK = Sink>
+SH_HD void solve_factored_pf(const IA *in, const IA *q, const bool *rev, const FA &F, SK y, int n1, float e,
+                             SK out, const Factor *tq = nullptr) {
     constexpr int NA = 2 * NW + 1, NB = 2 * NW;
     float tb[4];
     if (NF == 1) {
@@ -807,7 +828,7 @@ SH_HD void solve_factored_pf(const Acc *in, const Acc *q, const bool *rev, const
     float rd[D], ro[D][NB], rq[D][NF][NQ], ri[D][NF];
     auto fetch = [&](int s, int k) {
         const long kc = k < n1 ? k : n1 - 1;
-        rd[s] = F.d[kc * F.fs];
+        rd[s] = fac_d(F, kc);
         fac_row<NB>(F, kc, ro[s]);
         const int kq = k + 1 + NW < n1 ? k + 1 + NW : n1 - 1;
         const int ki = k + 1 + 2 * NW < n1 ? k + 1 + 2 * NW : n1 - 1;
@@ -869,8 +890,7 @@ SH_HD void solve_factored_pf(const Acc *in, const Acc *q, const bool *rev, const
             }
             if constexpr (TP) {
                 if (MID) {
-#pragma unroll
-                    for (int j = 0; j < NA; j++) A[f][NA - 1][j] = rev[f] ? rq[s][f][NA - 1 - j] : rq[s][f][j];
+                    tap_order<NA>(rq[s][f], rev[f], A[f][NA - 1]);
                 } else {
                     taps_stored<NW>(rq[s][f], k + 1 + NW, n1, rev[f], A[f][NA - 1]);
                 }
@@ -1307,8 +1327,7 @@ SH_HD void predict_adj_factored_pf(const Acc &a, const Acc *b, const Acc &q, boo
             }
             if constexpr (TP) {
                 if (MID) {
-#pragma unroll
-                    for (int j = 0; j < NA; j++) A[NA - 1][j] = rev ? rq[r][NA - 1 - j] : rq[r][j];
+                    tap_order<NA>(rq[r], rev, A[NA - 1]);
                 } else {
                     taps_stored<NW>(rq[r], k + 1 + NW, n1, rev, A[NA - 1]);
                 }

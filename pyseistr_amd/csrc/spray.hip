@@ -159,12 +159,83 @@ __device__ inline int level_block(int xcd, int cnt, long gx8, long &jb) {
     return (int)(b / gx8);
 }
 
+// Buffer-resource accessors of the level kernels (every plane below 4 GiB):
+// the plane's base and the sample row's byte offset (k * row, the
+// instruction's scalar offset) are wave-uniform, only the trace's byte
+// offset is a per-lane register, so a load or store needs no per-lane
+// address arithmetic.  The solve's arithmetic is the same as with pointers.
+__device__ inline __amdgpu_buffer_rsrc_t plane_rsrc(const void *p, unsigned long bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)(unsigned)bytes, 0x00020000);
+}
+struct BTrace {
+    __amdgpu_buffer_rsrc_t r;
+    unsigned lane, row;
+    __device__ float operator[](long k) const {
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)lane, (int)((unsigned)k * row), 0));
+    }
+};
+struct BSink {
+    __amdgpu_buffer_rsrc_t r;
+    unsigned lane, row;
+    struct Ref {
+        __amdgpu_buffer_rsrc_t r;
+        unsigned lane, off;
+        __device__ operator float() const {
+            return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)lane, (int)off, 0));
+        }
+        __device__ void operator=(float v) const {
+            __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)lane, (int)off, 0);
+        }
+    };
+    __device__ Ref operator[](long k) const { return Ref{r, lane, (unsigned)k * row}; }
+};
+// factor of one trace (layout of struct Factor): d plane and the rows of
+// NB off-diagonal entries side by side
+struct BFactor {
+    __amdgpu_buffer_rsrc_t rd, ro;
+    unsigned lane_d, lane_o, row_d, row_o;
+};
+__device__ inline float fac_d(const BFactor &F, long k) {
+    return __builtin_bit_cast(float,
+                              __builtin_amdgcn_raw_buffer_load_b32(F.rd, (int)F.lane_d, (int)((unsigned)k * F.row_d), 0));
+}
+// The row's 16- or 8-byte load is bit-cast to a float vector as a whole
+// before its lanes are read: read lane by lane from the load builtin's
+// integer vector, the load was compiled into one 4-byte load whose value
+// every lane received.
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+template <int NB>
+__device__ inline void fac_row(const BFactor &F, long k, float *ok) {
+    const int so = (int)((unsigned)k * F.row_o);
+    if constexpr (NB % 4 == 0) {
+#pragma unroll
+        for (int c = 0; c < NB / 4; c++) {
+            const f32x4 v =
+                __builtin_bit_cast(f32x4, __builtin_amdgcn_raw_buffer_load_b128(F.ro, (int)(F.lane_o + 16u * c), so, 0));
+#pragma unroll
+            for (int i = 0; i < 4; i++) ok[4 * c + i] = v[i];
+        }
+    } else {
+#pragma unroll
+        for (int c = 0; c < NB / 2; c++) {
+            const f32x2 v =
+                __builtin_bit_cast(f32x2, __builtin_amdgcn_raw_buffer_load_b64(F.ro, (int)(F.lane_o + 8u * c), so, 0));
+            ok[2 * c] = v[0];
+            ok[2 * c + 1] = v[1];
+        }
+    }
+}
+
 // One launch per (level, number of source traces NF): every thread predicts
 // one slot at one target trace with the shared factor of the target.  A
 // slot whose source lies outside the volume is written as zeros there
-// (sof3d_cfuns.c:1478-1503 skips it; the slot stays zero).
-template <int NW, int NF, int PF>
-__global__ void __launch_bounds__(TB)
+// (sof3d_cfuns.c:1478-1503 skips it; the slot stays zero).  BUF: planes,
+// factor and scratch through buffer resources (scratch row = the launch's
+// thread count), else through pointers (scratch row = nthr); BUF with
+// PF == 0 is the pointer path.
+template <int NW, int NF, int PF, bool BUF>
+__global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(2)))
 spray3_level(const float *__restrict__ pi, const float *__restrict__ px, float *__restrict__ U,
              const float *__restrict__ F, const Level3 nodes, int cnt, int xcd, int n1, int n2, int n3, int np2,
              float e, float *__restrict__ scr, long nthr) {
@@ -183,6 +254,37 @@ spray3_level(const float *__restrict__ pi, const float *__restrict__ px, float *
         return;
     }
     long tid = (long)node * gx8 * TB + j;
+    constexpr int NB = 2 * NW;
+    if constexpr (BUF && PF > 0) {
+        // NF == 2: axis 2 then axis 3; NF == 1: the node's one axis
+        const unsigned row = (unsigned)ntr * 4u;
+        BTrace in[2], q[2];
+        bool rev[2];
+        const bool ax2 = (nd.update & 1) != 0;
+#pragma unroll
+        for (int f = 0; f < NF; f++) {
+            const bool a2 = NF == 2 ? f == 0 : ax2;
+            const int up = a2 ? nd.up2 : nd.up3;
+            const long d = a2 ? 1 : n2;
+            const int ks = nd.slot + (up ? -1 : 1) * (a2 ? 1 : np2);
+            in[f] = BTrace{plane_rsrc(U + (long)ks * n, (unsigned long)n * 4), (unsigned)(up ? j - d : j + d) * 4u,
+                           row};
+            q[f] = BTrace{plane_rsrc(a2 ? pi : px, (unsigned long)n * 4), (unsigned)(up ? j - d : j) * 4u, row};
+            rev[f] = up != 0;
+        }
+        const float *Fp = F + (long)node_variant(nd.update, nd.up2, nd.up3) * fac_plane_floats(NB, n);
+        const BFactor Fv{plane_rsrc(Fp + (long)NB * n, (unsigned long)n * 4),
+                         plane_rsrc(Fp, (unsigned long)n * NB * 4),
+                         (unsigned)j * 4u,
+                         (unsigned)j * (4u * NB),
+                         row,
+                         row * NB};
+        const unsigned lt = (unsigned)(cnt * gx8 * TB);  // threads of this launch
+        const BSink y{plane_rsrc(scr, (unsigned long)lt * n1 * 4), (unsigned)tid * 4u, lt * 4u};
+        const BSink o{plane_rsrc(U + (long)nd.slot * n, (unsigned long)n * 4), (unsigned)j * 4u, row};
+        solve_factored_pf<NW, NF, PF>(in, q, rev, Fv, y, n1, e, o);
+        return;
+    }
     Sink y{scr + tid, nthr};
     Acc in[2], q[2];
     bool rev[2];
@@ -203,7 +305,6 @@ spray3_level(const float *__restrict__ pi, const float *__restrict__ px, float *
         rev[nf] = nd.up3 != 0;
         nf++;
     }
-    constexpr int NB = 2 * NW;
     const Factor Fv = fac_at(F + (long)node_variant(nd.update, nd.up2, nd.up3) * fac_plane_floats(NB, n), NB, n, j,
                              ntr);
     if constexpr (PF > 0)
@@ -609,6 +710,7 @@ int spray3_run(sh_ctx *ctx, int order, int n1, int n2, int n3, int ns2, int ns3,
     // warm step: two sources 9.45 / 8.85 for d = 4 / 6 (12.0 at 8 with the
     // factor in planes), one source 4.63 / 4.42 for d = 4 / 8.
     const char *pe = getenv("SH_SPRAY_PF");
+    const char *be = getenv("SH_SPRAY_BUF");
     for (size_t L = 1; L < levels.size(); L++) {
         if (grouped) {  // two-source slots by (quadrant, group size)
             std::vector<Group3> groups;
@@ -663,11 +765,23 @@ int spray3_run(sh_ctx *ctx, int order, int n1, int n2, int n3, int ns2, int ns3,
             if (!cnt) continue;
             dim3 grid((unsigned)(gx8 * cnt));
             PROF_A(ctx);
-#define SL(NF_, PF_)                                                                                          \
-    SH_DISPATCH_NW(ctx, order, NW,                                                                            \
-                   hipLaunchKernelGGL((spray3_level<NW, NF_, PF_>), grid, dim3(TB), 0, ctx->stream, piT, pxT, U, \
-                                      F, lv, cnt, xcd, n1, n2, n3, np2, e, scr, nthr))
             const int pf = pe ? atoi(pe) : (nf == 1 ? 8 : 6);
+            // buffer-resource accessors when every plane, the factor's
+            // off-diagonal block and this launch's scratch stay below 4 GiB
+            // (and a prefetching solve); SH_SPRAY_BUF=0 keeps pointers
+            const double lim = 4294967296.0;
+            const bool buf = !(be && be[0] == '0') && pf > 0 && (double)n * 4 * (2 * order) < lim &&
+                             (double)gx8 * TB * cnt * n1 * 4 < lim;
+#define SL(NF_, PF_)                                                                                           \
+    if (buf) {                                                                                                 \
+        SH_DISPATCH_NW(ctx, order, NW,                                                                         \
+                       hipLaunchKernelGGL((spray3_level<NW, NF_, PF_, true>), grid, dim3(TB), 0, ctx->stream,   \
+                                          piT, pxT, U, F, lv, cnt, xcd, n1, n2, n3, np2, e, scr, nthr));        \
+    } else {                                                                                                   \
+        SH_DISPATCH_NW(ctx, order, NW,                                                                         \
+                       hipLaunchKernelGGL((spray3_level<NW, NF_, PF_, false>), grid, dim3(TB), 0, ctx->stream,  \
+                                          piT, pxT, U, F, lv, cnt, xcd, n1, n2, n3, np2, e, scr, nthr));        \
+    }
             // timer names carry the kernel's trailing template arguments
             // (sources, prefetch distance)
             if (nf == 1) {

@@ -129,6 +129,29 @@ def test_spray2_prefetch_distances_identical(ps, monkeypatch):
         same(a, b, f"call {k}")
 
 
+def test_spray3_level_paths_identical(ps, monkeypatch):
+    """The 3-D spray levels give the same bits through buffer resources (the
+    default) and through pointers (SH_SPRAY_BUF=0), at every prefetch
+    distance (SH_SPRAY_PF) and with two-source slots grouped per thread
+    (SH_SPRAY_GROUP): pwspray3d, somean3dc and somf3dc, PWD orders 1-3."""
+    d, _, p, q = _case3((52, 17, 14), 31)
+
+    def calls():
+        return [ps.pwspray3d(d, p, q, 2, 3, 2, 0.01), ps.pwspray3d(d, p, q, 1, 2, 1, 0.05),
+                ps.pwspray3d(d, p, q, 2, 1, 3, 0.01), ps.somean3dc(d, p, q, 3, 2, 0.01, 2),
+                ps.somf3dc(d, p, q, 4, 4, 0.01, 2)]
+
+    ref = calls()
+    for env in ({"SH_SPRAY_BUF": "0"}, {"SH_SPRAY_PF": "4"}, {"SH_SPRAY_PF": "8"},
+                {"SH_SPRAY_PF": "2", "SH_SPRAY_BUF": "0"}, {"SH_SPRAY_PF": "0"}, {"SH_SPRAY_GROUP": "2"}):
+        for k, v in env.items():
+            monkeypatch.setenv(k, v)
+        for k, (a, b) in enumerate(zip(calls(), ref)):
+            same(a, b, f"{env} call {k}")
+        for k in env:
+            monkeypatch.delenv(k)
+
+
 def test_pwd_virtual_blocks_identical(ps, monkeypatch):
     """soint's operator kernels give the same bits with 1, 2 or 4 groups of
     samples per block (the partial-sum partition is unchanged), with and
