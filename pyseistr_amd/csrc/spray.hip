@@ -233,9 +233,11 @@ __device__ inline void fac_row(const BFactor &F, long k, float *ok) {
 // (sof3d_cfuns.c:1478-1503 skips it; the slot stays zero).  BUF: planes,
 // factor and scratch through buffer resources (scratch row = the launch's
 // thread count), else through pointers (scratch row = nthr); BUF with
-// PF == 0 is the pointer path.
+// PF == 0 is the pointer path.  Orders 1 and 2 are held to two waves per
+// SIMD (256 registers: the two-source solve at prefetch distance 6 fits
+// without spills); orders 3 and 4 need more registers than that.
 template <int NW, int NF, int PF, bool BUF>
-__global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(2)))
+__global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(NW <= 2 ? 2 : 1)))
 spray3_level(const float *__restrict__ pi, const float *__restrict__ px, float *__restrict__ U,
              const float *__restrict__ F, const Level3 nodes, int cnt, int xcd, int n1, int n2, int n3, int np2,
              float e, float *__restrict__ scr, long nthr) {
