@@ -23,6 +23,8 @@ struct Sink {
     long s;
     SH_HD float &operator[](long k) const { return p[k * s]; }
 };
+// read access to what a sink holds
+SH_HD inline Acc sink_acc(const Sink &y) { return Acc{y.p, y.s}; }
 
 // LDL^T factor of one trace's banded matrix, rows k = 0 .. n1-1: d[k] at
 // d[k * fs]; the NB off-diagonal entries o[0..NB) of row k side by side at
@@ -38,6 +40,7 @@ struct FactorOut {
     float *d, *o;
     long fs;
 };
+SH_HD inline float fac_d(const Factor &F, long k) { return F.d[k * F.fs]; }
 
 template <int NB>
 SH_HD inline void fac_row(const Factor &F, long k, float *ok) {
@@ -60,8 +63,6 @@ SH_HD inline void fac_row(const Factor &F, long k, float *ok) {
         }
     }
 }
-
-SH_HD inline float fac_d(const Factor &F, long k) { return F.d[k * F.fs]; }
 
 template <int NB>
 SH_HD inline void fac_put(const FactorOut &F, long k, float dk, const float *ok) {
@@ -94,11 +95,11 @@ SH_HD inline void tap_order(const float *b, bool rev, float *a) {
     }
 }
 
-template <int NW>
-SH_HD inline void tap_row(const Factor &T, long k, float *b) {
+template <int NW, class FT>
+SH_HD inline void tap_row(const FT &T, long k, float *b) {
     constexpr int NB = 2 * NW;
     fac_row<NB>(T, k, b);
-    b[NB] = T.d[k * T.fs];
+    b[NB] = fac_d(T, k);
 }
 template <int NW>
 SH_HD inline void taps_stored(const float *b, int kk, int n1, bool rev, float *a) {
@@ -776,9 +777,9 @@ SH_HD void solve_factored(const Acc *in, const Acc *q, const bool *rev, const Fa
 // types (IA, FA, SK): Acc / Factor / Sink, or the buffer-resource ones of
 // the 3-D level kernels; the arithmetic does not depend on them.
 template <int NW, int NF, int D, bool TP = false, bool MIDR = false, class IA = Acc, class FA = Factor,
-          class SK = Sink>
+          class SK = Sink, class TQ = Factor>
 SH_HD void solve_factored_pf(const IA *in, const IA *q, const bool *rev, const FA &F, SK y, int n1, float e,
-                             SK out, const Factor *tq = nullptr) {
+                             SK out, const TQ *tq = nullptr) {
     constexpr int NA = 2 * NW + 1, NB = 2 * NW;
     float tb[4];
     if (NF == 1) {
@@ -1170,9 +1171,10 @@ SH_HD void predict_adj_factored(const Acc &in, const Acc &q, bool rev, const Fac
 // running trace plus the slot it gathers, added in that order).  Same
 // float operations in the same order as predict_adj_factored; y holds the
 // solved trace, out may alias a.
-template <int NW, int D, bool TP = false, bool MIDR = false>
-SH_HD void predict_adj_factored_pf(const Acc &a, const Acc *b, const Acc &q, bool rev, const Factor &F, Sink y,
-                                   int n1, float e, Sink out, const Factor *tq = nullptr) {
+template <int NW, int D, bool TP = false, bool MIDR = false, class IA = Acc, class FA = Factor, class SK = Sink,
+          class TQ = Factor>
+SH_HD void predict_adj_factored_pf(const IA &a, const IA *b, const IA &q, bool rev, const FA &F, SK y, int n1,
+                                   float e, SK out, const TQ *tq = nullptr) {
     constexpr int NB = 2 * NW, NA = 2 * NW + 1;
     // forward substitution: ring of d[k], o[m][k] and the input at k
     {
@@ -1186,7 +1188,7 @@ SH_HD void predict_adj_factored_pf(const Acc &a, const Acc *b, const Acc &q, boo
         float rd[D], ro[D][NB], ri[D];
         auto fetch = [&](int s, int k) {
             const long kc = k < n1 ? k : n1 - 1;
-            rd[s] = F.d[kc * F.fs];
+            rd[s] = fac_d(F, kc);
             fac_row<NB>(F, kc, ro[s]);
             ri[s] = b ? (a[kc] + (*b)[kc]) : a[kc];
         };
@@ -1268,7 +1270,7 @@ SH_HD void predict_adj_factored_pf(const Acc &a, const Acc *b, const Acc &q, boo
     // out = A^T A s + e * boundary(s) (pred_adj_apply): ring of the slope
     // at k + 1 + NW and the solved sample at k + 1 + 2 NW
     {
-        const Acc s{y.p, y.s};
+        const auto s = sink_acc(y);
         float A[NA][NA], S[NA], T[NA];
 #pragma unroll
         for (int w = 0; w < NA; w++) {

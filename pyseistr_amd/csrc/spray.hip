@@ -167,11 +167,18 @@ __device__ inline int level_block(int xcd, int cnt, long gx8, long &jb) {
 __device__ inline __amdgpu_buffer_rsrc_t plane_rsrc(const void *p, unsigned long bytes) {
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)(unsigned)bytes, 0x00020000);
 }
+// Scalar offset of sample row k: k is the same in every active lane at each
+// call (the solves walk samples in lockstep); read from the first active
+// lane so that the compiler keeps it in a scalar register even where it
+// cannot prove it uniform (it would otherwise loop over lanes per load).
+__device__ inline int row_off(long k, unsigned row) {
+    return (int)__builtin_amdgcn_readfirstlane((unsigned)k * row);
+}
 struct BTrace {
     __amdgpu_buffer_rsrc_t r;
     unsigned lane, row;
     __device__ float operator[](long k) const {
-        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)lane, (int)((unsigned)k * row), 0));
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)lane, row_off(k, row), 0));
     }
 };
 struct BSink {
@@ -187,8 +194,10 @@ struct BSink {
             __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)lane, (int)off, 0);
         }
     };
-    __device__ Ref operator[](long k) const { return Ref{r, lane, (unsigned)k * row}; }
+    __device__ Ref operator[](long k) const { return Ref{r, lane, (unsigned)row_off(k, row)}; }
 };
+__device__ inline BTrace sink_acc(const BSink &y) { return BTrace{y.r, y.lane, y.row}; }
+
 // factor of one trace (layout of struct Factor): d plane and the rows of
 // NB off-diagonal entries side by side
 struct BFactor {
@@ -197,7 +206,7 @@ struct BFactor {
 };
 __device__ inline float fac_d(const BFactor &F, long k) {
     return __builtin_bit_cast(float,
-                              __builtin_amdgcn_raw_buffer_load_b32(F.rd, (int)F.lane_d, (int)((unsigned)k * F.row_d), 0));
+                              __builtin_amdgcn_raw_buffer_load_b32(F.rd, (int)F.lane_d, row_off(k, F.row_d), 0));
 }
 // The row's 16- or 8-byte load is bit-cast to a float vector as a whole
 // before its lanes are read: read lane by lane from the load builtin's
@@ -207,7 +216,7 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef float f32x2 __attribute__((ext_vector_type(2)));
 template <int NB>
 __device__ inline void fac_row(const BFactor &F, long k, float *ok) {
-    const int so = (int)((unsigned)k * F.row_o);
+    const int so = row_off(k, F.row_o);
     if constexpr (NB % 4 == 0) {
 #pragma unroll
         for (int c = 0; c < NB / 4; c++) {
@@ -236,7 +245,7 @@ __device__ inline void fac_row(const BFactor &F, long k, float *ok) {
 // PF == 0 is the pointer path.  Orders 1 and 2 are held to two waves per
 // SIMD (256 registers: the two-source solve at prefetch distance 6 fits
 // without spills); orders 3 and 4 need more registers than that.
-template <int NW, int NF, int PF, bool BUF>
+template <bool BUF, int NW, int NF, int PF>
 __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(NW <= 2 ? 2 : 1)))
 spray3_level(const float *__restrict__ pi, const float *__restrict__ px, float *__restrict__ U,
              const float *__restrict__ F, const Level3 nodes, int cnt, int xcd, int n1, int n2, int n3, int np2,
@@ -455,7 +464,37 @@ factor2_kernel(const float *__restrict__ dip, float *__restrict__ F, int n1, int
     factor_sweep_pf<NW, 1, 8>(q, rev, n1, e, Fs);
 }
 
-template <int NW, int PFD>
+// Accessors of the 2-D chain kernels over whole buffers (BUF: buffer
+// resources with the trace's byte offset per lane and the sample row k * n2
+// as the scalar offset; the direction a lane runs differs within a wave,
+// so it only enters the lane offset), or plain pointers.
+// (buffers of nel floats; element offset off, sample stride in elements)
+template <bool BUF>
+struct Chain2 {
+    int n2;
+    __device__ auto rd(const float *b, long nel, long off, long stride) const {
+        if constexpr (BUF) return BTrace{plane_rsrc(b, nel * 4), (unsigned)off * 4u, (unsigned)stride * 4u};
+        else return Acc{b + off, stride};
+    }
+    __device__ auto wr(float *b, long nel, long off, long stride) const {
+        if constexpr (BUF) return BSink{plane_rsrc(b, nel * 4), (unsigned)off * 4u, (unsigned)stride * 4u};
+        else return Sink{b + off, stride};
+    }
+    // factor (or taps) plane v of F (planes of pf floats, n per sample
+    // plane) at element e0 of a sample plane
+    template <int NB>
+    __device__ auto fac(const float *F, long pf, int nplanes, long n, int v, long e0) const {
+        if constexpr (BUF) {
+            const __amdgpu_buffer_rsrc_t r = plane_rsrc(F, pf * nplanes * 4);
+            return BFactor{r, r, (unsigned)(v * pf + NB * n + e0) * 4u, (unsigned)(v * pf + NB * e0) * 4u,
+                           (unsigned)n2 * 4u, (unsigned)n2 * (4u * NB)};
+        } else {
+            return fac_at(F + (long)v * pf, NB, n, e0, n2);
+        }
+    }
+};
+
+template <bool BUF, int NW, int PFD>
 __global__ void __launch_bounds__(TB)
 spray2_fwd(const float *__restrict__ x, const float *__restrict__ dip, const float *__restrict__ F,
            float *__restrict__ U, int n1, int n2, int n3, int ns, float e, float *__restrict__ scr, long nthr,
@@ -473,11 +512,12 @@ spray2_fwd(const float *__restrict__ x, const float *__restrict__ dip, const flo
     int i = (int)(gid % n2);
     long slice = (long)n1 * n2;
     const float *xs = x + i3 * slice;
-    const float *ds = dip + i3 * slice;
     long plane_slot = slice * n3, n = plane_slot;
     constexpr int NB = 2 * NW;
+    const long pf = fac_plane_floats(NB, n);
+    const Chain2<BUF> C2{n2};
     long tid = (long)i3 * 2 * n2 + gid;
-    Sink y{scr + tid, nthr};
+    const auto y = C2.wr(scr, nthr * n1, tid, nthr);
     if (dir == 0) {  // centre slot written once
         Sink c{U + (long)ns * plane_slot + i3 * slice + i, n2};
         for (int k = 0; k < n1; k++) c[k] = xs[(long)k * n2 + i];
@@ -490,20 +530,21 @@ spray2_fwd(const float *__restrict__ x, const float *__restrict__ dip, const flo
         Sink z{U + (long)slot * plane_slot + i3 * slice + i, n2};
         for (int k = 0; k < n1; k++) z[k] = 0.0f;
     }
-    Acc cur{xs + i, n2};
+    const long nU = (2L * ns + 1) * n;
+    auto cur = C2.rd(x, n, i3 * slice + i, n2);
     for (int is = 0; is < ns; is++) {
         int ip = dir == 0 ? i - is - 1 : i + is + 1;
         if (ip < 0 || ip >= n2) break;
         int slot = dir == 0 ? ns - is - 1 : ns + is + 1;
-        Acc q{ds + (dir == 0 ? ip : ip - 1), n2};
-        Sink out{U + (long)slot * plane_slot + i3 * slice + ip, n2};
-        Acc in[1] = {cur};
-        Acc qs[1] = {q};
+        const long eo = (long)slot * plane_slot + i3 * slice + ip;
+        const auto out = C2.wr(U, nU, eo, n2);
+        decltype(cur) in[1] = {cur};
+        decltype(cur) qs[1] = {C2.rd(dip, n, i3 * slice + (dir == 0 ? ip : ip - 1), n2)};
         bool rv[1] = {dir == 1};
-        const Factor Fv = fac_at(F + (long)dir * fac_plane_floats(NB, n), NB, n, i3 * slice + ip, n2);
-        const Factor Tq = fac_at(F + 2 * fac_plane_floats(NB, n), NB, n, i3 * slice + (dir == 0 ? ip : ip - 1), n2);
+        const auto Fv = C2.template fac<NB>(F, pf, 3, n, dir, i3 * slice + ip);
+        const auto Tq = C2.template fac<NB>(F, pf, 3, n, 2, i3 * slice + (dir == 0 ? ip : ip - 1));
         solve_factored_pf<NW, 1, PFD, true, true>(in, qs, rv, Fv, y, n1, e, out, &Tq);
-        cur = Acc{out.p, n2};
+        cur = C2.rd(U, nU, eo, n2);
     }
 }
 
@@ -511,7 +552,7 @@ spray2_fwd(const float *__restrict__ x, const float *__restrict__ dip, const flo
 // chain dir 1 gathers slots ns+is+1 of traces i+is+1 (is = ns-1..0) through
 // adjoint predictions; dir 0 likewise from the left.  Results go to
 // C[dir][i3][k][i]; the centre slot is added by spray2_adj_combine.
-template <int NW, int PFD>
+template <bool BUF, int NW, int PFD>
 __global__ void __launch_bounds__(TB)
 spray2_adj(const float *__restrict__ U, const float *__restrict__ dip, const float *__restrict__ F,
            float *__restrict__ C, int n1, int n2, int n3, int ns, float e, float *__restrict__ scr,
@@ -524,25 +565,32 @@ spray2_adj(const float *__restrict__ U, const float *__restrict__ dip, const flo
     int i = (int)(gid % n2);
     long slice = (long)n1 * n2;
     long plane_slot = slice * n3, n = plane_slot;
-    const float *ds = dip + i3 * slice;
     constexpr int NB = 2 * NW;
+    const long pf = fac_plane_floats(NB, n);
+    const Chain2<BUF> C2{n2};
     long tid = (long)i3 * 2 * n2 + gid;
-    Sink y{scr + tid, nthr};
-    Sink t{tr + tid, nthr};   // running trace
-    for (int k = 0; k < n1; k++) t[k] = 0.0f;
+    const auto y = C2.wr(scr, nthr * n1, tid, nthr);
+    const auto t = C2.wr(tr, nthr * n1, tid, nthr);  // running trace
+    const auto ta = C2.rd(tr, nthr * n1, tid, nthr);
+    {
+        Sink z{tr + tid, nthr};
+        for (int k = 0; k < n1; k++) z[k] = 0.0f;
+    }
+    const long nU = (2L * ns + 1) * n;
     for (int is = ns - 1; is >= 0; is--) {
         int ip = dir == 1 ? i + is + 1 : i - is - 1;
         if (ip < 0 || ip >= n2) continue;
         int slot = dir == 1 ? ns + is + 1 : ns - is - 1;
-        Acc u{U + (long)slot * plane_slot + i3 * slice + ip, n2};
-        Acc q{ds + (dir == 1 ? ip - 1 : ip), n2};
-        const Factor Fv = fac_at(F + (long)(dir == 1 ? 1 : 0) * fac_plane_floats(NB, n), NB, n, i3 * slice + ip, n2);
-        const Factor Tq = fac_at(F + 2 * fac_plane_floats(NB, n), NB, n, i3 * slice + (dir == 1 ? ip - 1 : ip), n2);
+        const auto u = C2.rd(U, nU, (long)slot * plane_slot + i3 * slice + ip, n2);
+        const auto q = C2.rd(dip, n, i3 * slice + (dir == 1 ? ip - 1 : ip), n2);
+        const auto Fv = C2.template fac<NB>(F, pf, 3, n, dir == 1 ? 1 : 0, i3 * slice + ip);
+        const auto Tq = C2.template fac<NB>(F, pf, 3, n, 2, i3 * slice + (dir == 1 ? ip - 1 : ip));
         // t = S^{-1}-adjoint prediction of (t + u), loads issued PFD samples ahead
-        predict_adj_factored_pf<NW, PFD, true, true>(Acc{t.p, t.s}, &u, q, dir == 1, Fv, y, n1, e, t, &Tq);
+        predict_adj_factored_pf<NW, PFD, true, true>(ta, &u, q, dir == 1, Fv, y, n1, e, t, &Tq);
     }
     Sink c{C + (long)dir * plane_slot + i3 * slice + i, n2};
-    for (int k = 0; k < n1; k++) c[k] = t[k];
+    const float *tp = tr + tid;
+    for (int k = 0; k < n1; k++) c[k] = tp[(long)k * nthr];
 }
 
 // x = ((x or 0) + chain_right) + chain_left + centre: the accumulation
@@ -777,11 +825,11 @@ int spray3_run(sh_ctx *ctx, int order, int n1, int n2, int n3, int ns2, int ns3,
 #define SL(NF_, PF_)                                                                                           \
     if (buf) {                                                                                                 \
         SH_DISPATCH_NW(ctx, order, NW,                                                                         \
-                       hipLaunchKernelGGL((spray3_level<NW, NF_, PF_, true>), grid, dim3(TB), 0, ctx->stream,   \
+                       hipLaunchKernelGGL((spray3_level<true, NW, NF_, PF_>), grid, dim3(TB), 0, ctx->stream,   \
                                           piT, pxT, U, F, lv, cnt, xcd, n1, n2, n3, np2, e, scr, nthr));        \
     } else {                                                                                                   \
         SH_DISPATCH_NW(ctx, order, NW,                                                                         \
-                       hipLaunchKernelGGL((spray3_level<NW, NF_, PF_, false>), grid, dim3(TB), 0, ctx->stream,  \
+                       hipLaunchKernelGGL((spray3_level<false, NW, NF_, PF_>), grid, dim3(TB), 0, ctx->stream,  \
                                           piT, pxT, U, F, lv, cnt, xcd, n1, n2, n3, np2, e, scr, nthr));        \
     }
             // timer names carry the kernel's trailing template arguments
@@ -867,6 +915,28 @@ int spray2_pf(bool forward) {
         }                                        \
     } while (0)
 
+// Buffer-resource accessors in the chain kernels when every buffer they
+// address (input and slopes, slot volume, factor planes, scratch planes)
+// stays below 4 GiB; SH_SPRAY2_BUF=0 keeps pointers (identical results).
+bool spray2_buf(int n1, long n, int ns, int order, long nthr) {
+    const char *e = getenv("SH_SPRAY2_BUF");
+    if (e && e[0] == '0') return false;
+    const double lim = 4294967295.0;
+    return 4.0 * n * (2 * ns + 1) < lim && 4.0 * 3 * fac_plane_floats(2 * order, n) < lim &&
+           4.0 * 2 * nthr * n1 < lim;
+}
+
+#define SPRAY2_BUF_DISPATCH(BUFV, cond, ...) \
+    do {                                     \
+        if (cond) {                          \
+            constexpr bool BUFV = true;      \
+            __VA_ARGS__;                     \
+        } else {                             \
+            constexpr bool BUFV = false;     \
+            __VA_ARGS__;                     \
+        }                                    \
+    } while (0)
+
 // Factor cache of P into caller-provided buffers: scr holds 2*nthr*n1
 // floats (solver scratch, then the running trace planes), F
 // 3 * fac_plane_floats(2 * order, n): the two factor variants, then the
@@ -907,9 +977,10 @@ int pws2_spray(sh_ctx *ctx, Pwsmooth2 &P, const float *x, const float *x2 = null
     const int TB2 = spray2_tb();
     dim3 grid(ceil_div(2L * P.n2, TB2), P.n3, x2 ? 2 : 1);
     PROF_A(ctx);
-    SPRAY2_PF_DISPATCH(PF, true, SH_DISPATCH_NW(ctx, P.order, NW,
-                   hipLaunchKernelGGL((spray2_fwd<NW, PF>), grid, dim3(TB2), 0, ctx->stream, x, P.dip, P.F, P.U, P.n1,
-                                      P.n2, P.n3, P.ns, P.e, P.scr, P.nthr, P.stop, x2, U2)));
+    const bool buf = spray2_buf(P.n1, (long)P.n1 * P.n2 * P.n3, P.ns, P.order, P.nthr);
+    SPRAY2_BUF_DISPATCH(BUF, buf, SPRAY2_PF_DISPATCH(PF, true, SH_DISPATCH_NW(ctx, P.order, NW,
+                   hipLaunchKernelGGL((spray2_fwd<BUF, NW, PF>), grid, dim3(TB2), 0, ctx->stream, x, P.dip, P.F, P.U,
+                                      P.n1, P.n2, P.n3, P.ns, P.e, P.scr, P.nthr, P.stop, x2, U2))));
     PROF_B(ctx, "spray2_fwd", (x2 ? 2 : 1) * 12.0 * 2 * P.ns * (double)P.n1 * P.n2 * P.n3);
     return sh_launch_check(ctx, "spray2_fwd");
 }
@@ -981,9 +1052,10 @@ int pws2_apply(sh_ctx *ctx, Pwsmooth2 &P, int adj, int add, float *trace, float 
     dim3 grid(ceil_div(2L * P.n2, TB2), P.n3);
     {
         PROF_A(ctx);
-        SPRAY2_PF_DISPATCH(PF, false, SH_DISPATCH_NW(ctx, P.order, NW,
-                       hipLaunchKernelGGL((spray2_adj<NW, PF>), grid, dim3(TB2), 0, ctx->stream, P.U, P.dip, P.F, P.C,
-                                          P.n1, P.n2, P.n3, P.ns, P.e, P.scr, P.tr, P.nthr, P.stop)));
+        const bool buf = spray2_buf(P.n1, n, P.ns, P.order, P.nthr);
+        SPRAY2_BUF_DISPATCH(BUF, buf, SPRAY2_PF_DISPATCH(PF, false, SH_DISPATCH_NW(ctx, P.order, NW,
+                       hipLaunchKernelGGL((spray2_adj<BUF, NW, PF>), grid, dim3(TB2), 0, ctx->stream, P.U, P.dip, P.F,
+                                          P.C, P.n1, P.n2, P.n3, P.ns, P.e, P.scr, P.tr, P.nthr, P.stop))));
         PROF_B(ctx, "spray2_adj", 12.0 * 2 * P.ns * (double)n);
     }
     SH_TRY(sh_launch_check(ctx, "spray2_adj"));
@@ -1272,9 +1344,10 @@ int shd_pwspray2d(sh_ctx *ctx, int adj, const float *dip, int n1, int n2, int ns
     }
     const int TB2 = spray2_tb();
     dim3 grid(ceil_div(2L * n2, TB2), 1);
-    SPRAY2_PF_DISPATCH(PF, false, SH_DISPATCH_NW(ctx, order, NW,
-                   hipLaunchKernelGGL((spray2_adj<NW, PF>), grid, dim3(TB2), 0, ctx->stream, P.U, P.dip, P.F, P.C, n1, n2, 1,
-                                      ns, P.e, P.scr, P.tr, P.nthr, nullptr)));
+    const bool buf = spray2_buf(n1, n, ns, order, P.nthr);
+    SPRAY2_BUF_DISPATCH(BUF, buf, SPRAY2_PF_DISPATCH(PF, false, SH_DISPATCH_NW(ctx, order, NW,
+                   hipLaunchKernelGGL((spray2_adj<BUF, NW, PF>), grid, dim3(TB2), 0, ctx->stream, P.U, P.dip, P.F, P.C,
+                                      n1, n2, 1, ns, P.e, P.scr, P.tr, P.nthr, nullptr))));
     SH_TRY(sh_launch_check(ctx, "spray2_adj"));
     hipLaunchKernelGGL(spray2_adj_combine, dim3(ceil_div(n, TB)), dim3(TB), 0, ctx->stream, P.C, P.U, xT, n, ns, 0);
     SH_TRY(sh_launch_check(ctx, "spray2_adj_combine"));

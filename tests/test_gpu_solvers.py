@@ -105,8 +105,9 @@ def test_tiled_permutation_equals_plain(ps, monkeypatch):
 
 
 def test_spray2_prefetch_distances_identical(ps, monkeypatch):
-    """The 2-D chain kernels' load rings at 8 and 16 samples ahead give the
-    same bits (spray, adjoint spray, somean2d and sint3d's shaping CG)."""
+    """The 2-D chain kernels' load rings at 8 and 16 samples ahead, and their
+    buffer-resource and pointer accessors (SH_SPRAY2_BUF=0), give the same
+    bits (spray, adjoint spray, somean2d and sint3d's shaping CG)."""
     d, m, p, q = _case3((60, 23, 9), 11)
     d2, p2 = d[:, :, 0], p[:, :, 0]
     rng = np.random.default_rng(3)
@@ -125,8 +126,12 @@ def test_spray2_prefetch_distances_identical(ps, monkeypatch):
         monkeypatch.setenv("SH_SPRAY2_PF", pf)
         out[pf] = calls()
     monkeypatch.delenv("SH_SPRAY2_PF")
-    for k, (a, b) in enumerate(zip(out["8"], out["16"])):
+    monkeypatch.setenv("SH_SPRAY2_BUF", "0")
+    out["ptr"] = calls()
+    monkeypatch.delenv("SH_SPRAY2_BUF")
+    for k, (a, b, c) in enumerate(zip(out["8"], out["16"], out["ptr"])):
         same(a, b, f"call {k}")
+        same(a, c, f"call {k}, pointers")
 
 
 def test_spray3_level_paths_identical(ps, monkeypatch):
