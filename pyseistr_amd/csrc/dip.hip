@@ -352,6 +352,92 @@ smooth_fused(const float *__restrict__ in, float *__restrict__ out, int n1, int 
     sm_partials<EPI>(acc, part, gridDim.x * gridDim.y, blockIdx.x + gridDim.x * blockIdx.y, sh);
 }
 
+// Axes 2 and 3 in one pass: a block owns a 64 (i1) x P_T2 (i2) tile and
+// marches along i3 over a segment of planes.  Each input plane's rows (with
+// the i2 halo, folded at the edges) go through LDS, double-buffered so the
+// next plane's loads are in flight while the current one is smoothed along
+// i2; every thread keeps the i2-smoothed values of its columns for the last
+// 2*H3+1 planes in a register ring and applies the i3 stencil from it.  The
+// per-axis sums are those of the separable passes in the same order.
+constexpr int P_HMAX2 = 8;
+
+template <int EPI, int H3, int P_T2>
+__global__ void __launch_bounds__(256)
+smooth_pair23(const float *__restrict__ in, float *__restrict__ out, int n1, int n2, int n3, int r2, int seg,
+              CgArgs A, const DipScal *sc, double *part, int first) {
+    if (sc->cg_done) return;
+    extern __shared__ float lds[];  // two planes of [P_T2 + 2 h2][64]
+    __shared__ float w2[P_HMAX2 + 1], w3[H3 + 1];
+    __shared__ double sh[16];
+    constexpr int NC = P_T2 / 4, RING = 2 * H3 + 1, P_KR = (P_T2 + 2 * P_HMAX2) / 4;
+    const int h2 = r2 - 1, rows = P_T2 + 2 * h2;
+    const int tx = threadIdx.x, ty = threadIdx.y;  // 64 x 4
+    const int ntile1 = (n1 + 63) / 64;
+    const int x0 = (blockIdx.x % ntile1) * 64, y0 = (blockIdx.x / ntile1) * P_T2;
+    const int z0 = blockIdx.y * seg, z1 = min(n3, z0 + seg);
+    const long n12 = (long)n1 * n2;
+    const bool pend = (EPI == EPI_GP) ? (sc->pending != 0) : false;
+    const float a2 = pend ? (float)sc->alpha2 : 0.0f;
+    const float a1 = (EPI == EPI_S) ? (float)sc->alpha : 0.0f;
+    {
+        const int t = tx + 64 * ty;
+        if (t <= P_HMAX2) w2[t] = (t < r2) ? (float)((double)(r2 - t) / ((double)r2 * r2)) : 0.0f;
+        if (t <= H3) w3[t] = (float)((double)(H3 + 1 - t) / ((double)(H3 + 1) * (H3 + 1)));
+    }
+    const int ix = x0 + tx;
+    const bool xin = ix < n1;
+    // rows this thread loads: r = ty + 4 k (row y0 - h2 + r; rows past
+    // n2 - 1 + h2 are never read by an output inside the volume)
+    long roff[P_KR];
+#pragma unroll
+    for (int k = 0; k < P_KR; k++) {
+        const int r = ty + 4 * k, yy = y0 - h2 + r;
+        roff[k] = (xin && r < rows && yy < n2 + h2) ? ix + (long)n1 * fold_idx(yy, n2) : -1;
+    }
+    float pre[P_KR];
+    auto fetch = [&](int q) {
+        const long po = n12 * fold_idx(q, n3);
+#pragma unroll
+        for (int k = 0; k < P_KR; k++) pre[k] = roff[k] >= 0 ? in[po + roff[k]] : 0.0f;
+    };
+    float ring[NC][RING];
+#pragma unroll
+    for (int c = 0; c < NC; c++)
+#pragma unroll
+        for (int m = 0; m < RING; m++) ring[c][m] = 0.0f;
+    double acc[3] = {0.0, 0.0, 0.0};
+    const int qa = z0 - H3, qb = z1 + H3;
+    fetch(qa);
+    for (int q = qa; q < qb; q++) {
+        float *L = lds + (q & 1) * rows * 64;
+#pragma unroll
+        for (int k = 0; k < P_KR; k++) {
+            const int r = ty + 4 * k;
+            if (r < rows) L[r * 64 + tx] = pre[k];
+        }
+        __syncthreads();
+        if (q + 1 < qb) fetch(q + 1);
+        const int o = q - H3;
+#pragma unroll
+        for (int c = 0; c < NC; c++) {
+            const float *cc = L + (ty + 4 * c + h2) * 64 + tx;
+            float s = w2[0] * cc[0];
+            for (int k = 1; k <= h2; k++) s += w2[k] * (cc[-k * 64] + cc[k * 64]);
+#pragma unroll
+            for (int m = 0; m < RING - 1; m++) ring[c][m] = ring[c][m + 1];
+            ring[c][RING - 1] = s;
+            const int iy = y0 + ty + 4 * c;
+            if (o >= z0 && xin && iy < n2) {
+                float t = w3[0] * ring[c][H3];
+#pragma unroll
+                for (int k = 1; k <= H3; k++) t += w3[k] * (ring[c][H3 - k] + ring[c][H3 + k]);
+                sm_store<EPI>(t, out, ix + (long)n1 * iy + n12 * o, A, pend, a2, a1, first, acc);
+            }
+        }
+    }
+    sm_partials<EPI>(acc, part, gridDim.x * gridDim.y, blockIdx.x + gridDim.x * blockIdx.y, sh);
+}
+
 // ---------------------------------------------------------------------------
 // Stand-alone smoothing (smoothcf, dip_cfuns.c:2006-2123): every line along
 // axis `ax` smoothed `repeat` times by ps_smooth (adj: fold -> doubint ->
@@ -564,31 +650,61 @@ int fused_segments(const DipGeom &g, int h3) {
 
 int smooth_apply(sh_ctx *ctx, const DipGeom &g, const SmoothPlan &sp, const float *in, float *out, int pro,
                  int epi, const CgArgs &A, DipWork &W, int first, int *nblk_out) {
-    if (sp.fused) {
+    // the fused 2.5-D pass over axes with radii (r1, r[1], r[2])
+    auto fused = [&](int P, int E, int r1, const float *src, float *dst) -> int {
         int seg = fused_segments(g, sp.r[2] - 1);
         dim3 grid((unsigned)((long)ceil_div(g.n1, F_TI1) * ceil_div(g.n2, F_TI2)), ceil_div(g.n3, seg));
         int nblk = grid.x * grid.y;
         if (nblk > W.nblk_max) return sh_fail(ctx, SH_EINVAL, "internal: partial buffer too small");
-        int h1 = sp.r[0] - 1, h2 = sp.r[1] - 1, h3 = sp.r[2] - 1;
+        int h1 = r1 - 1, h2 = sp.r[1] - 1, h3 = sp.r[2] - 1;
         size_t lds = sizeof(float) * ((size_t)(F_TI2 + 2 * h2) * (F_TI1 + 2 * h1) + (size_t)(F_TI2 + 2 * h2) * F_TI1 +
                                       (size_t)(2 * h3 + 1) * F_TI2 * F_TI1);
-        double bps = pass_bytes(pro, epi, first != 0, true);
+        double bps = pass_bytes(P, E, first != 0, true);
         static const char *names[2][3] = {{"smooth_fused<0,0>", "smooth_fused<0,1>", "smooth_fused<0,2>"},
                                           {"smooth_fused<1,0>", "smooth_fused<1,1>", "smooth_fused<1,2>"}};
         PROF_A(ctx);
 #define SM_F(PP, EE)                                                                                   \
-    if (pro == PP && epi == EE)                                                                        \
-        hipLaunchKernelGGL((smooth_fused<PP, EE>), grid, dim3(64, 4), lds, ctx->stream, in, out, g.n1, g.n2, g.n3, \
-                           sp.r[0], sp.r[1], sp.r[2], seg, A, W.sc, W.part, first);
+    if (P == PP && E == EE)                                                                            \
+        hipLaunchKernelGGL((smooth_fused<PP, EE>), grid, dim3(64, 4), lds, ctx->stream, src, dst, g.n1, g.n2, g.n3, \
+                           r1, sp.r[1], sp.r[2], seg, A, W.sc, W.part, first);
         SM_F(0, 0) SM_F(0, 1) SM_F(0, 2) SM_F(1, 0) SM_F(1, 1) SM_F(1, 2)
 #undef SM_F
-        PROF_B(ctx, names[pro][epi], bps * (double)g.n);
+        PROF_B(ctx, names[P][E], bps * (double)g.n);
         SH_TRY(sh_launch_check(ctx, "smooth_fused"));
         *nblk_out = nblk;
         return SH_OK;
-    }
+    };
+    if (sp.fused == 1) return fused(pro, epi, sp.r[0], in, out);
+    // axes 2 and 3 in one pass (smooth_pair23) from src into dst
+    auto pair = [&](int E, const float *src, float *dst) -> int {
+        constexpr int P_T2 = 16;
+        const long tiles = (long)ceil_div(g.n1, 64) * ceil_div(g.n2, P_T2);
+        const int nseg = (int)std::max<long>(1, std::min<long>(g.n3 / 16, (2048 + tiles - 1) / tiles));
+        const int seg = ceil_div(g.n3, nseg);
+        dim3 grid((unsigned)tiles, ceil_div(g.n3, seg));
+        const int nblk = grid.x * grid.y;
+        if (nblk > W.nblk_max) return sh_fail(ctx, SH_EINVAL, "internal: partial buffer too small");
+        const size_t lds = sizeof(float) * 2 * (size_t)(P_T2 + 2 * (sp.r[1] - 1)) * 64;
+        static const char *names[3] = {"smooth_pair23<0>", "smooth_pair23<1>", "smooth_pair23<2>"};
+        PROF_A(ctx);
+#define SM_P(EE, HH)                                                                                       \
+    if (E == EE && sp.r[2] - 1 == HH)                                                                      \
+        hipLaunchKernelGGL((smooth_pair23<EE, HH, P_T2>), grid, dim3(64, 4), lds, ctx->stream, src, dst, g.n1, \
+                           g.n2, g.n3, sp.r[1], seg, A, W.sc, W.part, first);
+#define SM_PH(EE) SM_P(EE, 1) SM_P(EE, 2) SM_P(EE, 3) SM_P(EE, 4) SM_P(EE, 5) SM_P(EE, 6)
+        SM_PH(0) SM_PH(1) SM_PH(2)
+#undef SM_PH
+#undef SM_P
+        PROF_B(ctx, names[E], pass_bytes(PRO_PLAIN, E, first != 0, false) * (double)g.n);
+        SH_TRY(sh_launch_check(ctx, "smooth_pair23"));
+        *nblk_out = nblk;
+        return SH_OK;
+    };
+    // pair (fused == 3): axis 1 as a separable pass into t1, then axes 2
+    // and 3 in one pass (smooth_pair23)
+    const int nsep = sp.fused == 3 ? 1 : sp.naxes;
     const float *src = in;
-    for (int a = 0; a < sp.naxes; a++) {
+    for (int a = 0; a < nsep; a++) {
         bool is_first = a == 0, is_last = a == sp.naxes - 1;
         int P = is_first ? pro : PRO_PLAIN;
         int E = is_last ? epi : EPI_PLAIN;
@@ -655,6 +771,7 @@ int smooth_apply(sh_ctx *ctx, const DipGeom &g, const SmoothPlan &sp, const floa
         if (is_last) *nblk_out = nblk;
         src = dst;
     }
+    if (sp.fused == 3) return pair(epi, src, out);
     return SH_OK;
 }
 
@@ -759,7 +876,15 @@ extern "C" int shd_dip(sh_ctx *ctx, const float *data, const float *mask, const 
         size_t lds = sizeof(float) * ((size_t)(F_TI2 + 2 * h2) * (F_TI1 + 2 * h1) + (size_t)(F_TI2 + 2 * h2) * F_TI1 +
                                       (size_t)(2 * h3 + 1) * F_TI2 * F_TI1);
         const char *env = getenv("SH_SMOOTH_FUSED");
-        plan.fused = (lds <= 60 * 1024) && (env && env[0] == '1');
+        plan.fused = (lds <= 60 * 1024) && (env && env[0] == '1') ? 1 : 0;
+        // SH_SMOOTH_FUSED=3: axis 1 separable, axes 2 and 3 in smooth_pair23
+        // (all three axes active, radii r2 <= 9 and 2 <= r3 <= 7).  Measured
+        // at 512x256x256 with rect 5,5,5: 8.4 + 9.2 ms per warm step for the
+        // two pair passes against 6.7 + 7.8 ms for the four separable ones
+        // they replace (tiles of 16 or 32 rows, 4 to 16 segments along i3:
+        // no better), so it stays opt-in.
+        if (env && env[0] == '3' && plan.naxes == 3 && plan.r[1] - 1 <= P_HMAX2 && plan.r[2] >= 2 && plan.r[2] <= 7)
+            plan.fused = 3;
     }
     if (plan.naxes == 0) {  // no smoothing: identity through a 1-tap pass
         plan.naxes = 1;

@@ -205,15 +205,18 @@ def test_bad_arguments_raise(ps):
         ps.somf3dc(w, w, w, 2, 2, 0.01, 5)  # predictor orders 1..4
 
 
-def test_fused_smoothing_matches_separable(ps, monkeypatch):
-    """The opt-in fused 2.5-D smoothing pass (SH_SMOOTH_FUSED=1) computes the
-    same per-axis stencils in the same order as the separable passes; only
-    the block partition of the double-precision CG sums differs, so the dips
-    agree to 1e-5 samples/trace."""
+@pytest.mark.parametrize("mode", ["1", "3"])
+def test_fused_smoothing_matches_separable(ps, monkeypatch, mode):
+    """The opt-in fused smoothing passes (SH_SMOOTH_FUSED=1: all axes in the
+    2.5-D kernel; 3: axis 1 separable, axes 2 and 3 in smooth_pair23)
+    compute the same per-axis
+    stencils in the same order as the separable passes; only the block
+    partition of the double-precision CG sums differs, so the dips agree to
+    1e-5 samples/trace."""
     from pyseistr_amd import synth
     v, _ = synth.volume3d((96, 40, 36), seed=9, n_events=3, smax=2.0)
     a = np.stack(ps.dip3dc(v, niter=3, liter=10, rect=[5, 5, 5]))
-    monkeypatch.setenv("SH_SMOOTH_FUSED", "1")
+    monkeypatch.setenv("SH_SMOOTH_FUSED", mode)
     b = np.stack(ps.dip3dc(v, niter=3, liter=10, rect=[5, 5, 5]))
     monkeypatch.delenv("SH_SMOOTH_FUSED")
     assert np.abs(a - b).max() <= 1e-5
