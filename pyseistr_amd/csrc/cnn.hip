@@ -130,6 +130,66 @@ __global__ void __launch_bounds__(TB) conv1_fwd(const float *__restrict__ x, flo
     }
 }
 
+// conv1_fwd for dilation 1 with the input staged in LDS: a block owns a
+// 64 (i1) x 4 (i2) tile and marches along i3 over a segment of planes,
+// keeping the scaled input planes o-1, o, o+1 (with their one-sample halo,
+// zero outside the volume) in a ring of three LDS planes, so each input
+// sample is read from HBM about once instead of through 27 global loads
+// per voxel.  Same operations per output as conv1_fwd.
+constexpr int C1_W = 66, C1_H = 6, C1_PL = C1_W * C1_H;
+
+__global__ void __launch_bounds__(TB) conv1_fwd_tile(const float *__restrict__ x, float xscale,
+                                                     const float *__restrict__ P, __bf16 *__restrict__ out, int n1,
+                                                     int n2, int n3, int seg, const float *__restrict__ xsp) {
+    if (xsp) xscale = xsp[0];
+    __shared__ float pl[3 * C1_PL];
+    const float *__restrict__ w = P + OFF_W1;
+    const float *__restrict__ b = P + OFF_B1;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;  // 64 x 4
+    const int ntile1 = (n1 + 63) / 64;
+    const int x0 = (blockIdx.x % ntile1) * 64, y0 = (blockIdx.x / ntile1) * 4;
+    const int z0 = blockIdx.y * seg, z1 = min(n3, z0 + seg);
+    const long n12 = (long)n1 * n2;
+    const int i1 = x0 + tx, i2 = y0 + ty;
+    for (int q = z0 - 1; q <= z1; q++) {
+        // plane q into ring slot (q + 3) % 3
+        float *dst = pl + ((q + 3) % 3) * C1_PL;
+        for (int e = threadIdx.x; e < C1_PL; e += TB) {
+            const int r = e / C1_W, c = e - r * C1_W;
+            const int j1 = x0 - 1 + c, j2 = y0 - 1 + r;
+            const bool in = q >= 0 && q < n3 && j1 >= 0 && j1 < n1 && j2 >= 0 && j2 < n2;
+            dst[e] = in ? x[j1 + (long)n1 * j2 + n12 * q] * xscale : 0.0f;
+        }
+        __syncthreads();
+        const int o = q - 1;
+        if (o >= z0 && i1 < n1 && i2 < n2) {
+            float xv[NT];
+#pragma unroll
+            for (int t = 0; t < NT; t++) {
+                const int d1 = t % 3 - 1, d2 = (t / 3) % 3 - 1, d3 = t / 9 - 1;
+                xv[t] = pl[((o + d3 + 3) % 3) * C1_PL + (ty + 1 + d2) * C1_W + tx + 1 + d1];
+            }
+            __bf16 ov[C];
+#pragma unroll
+            for (int co = 0; co < C; co++) {
+                float s = b[co];
+#pragma unroll
+                for (int t = 0; t < NT; t++) s = fmaf(w[t * C + co], xv[t], s);
+                ov[co] = (__bf16)fmaxf(s, 0.0f);
+            }
+            bf16x8 *dp = (bf16x8 *)(out + ((long)i1 + (long)n1 * i2 + n12 * o) * C);
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                bf16x8 pk;
+#pragma unroll
+                for (int i = 0; i < 8; i++) pk[i] = ov[k * 8 + i];
+                dp[k] = pk;
+            }
+        }
+        __syncthreads();  // the slot of plane o - 1 is overwritten next
+    }
+}
+
 // ---------------------------------------------------------------------------
 // 32-channel 3x3x3 convolution on the matrix cores.
 // Block = 4 waves, tile 64 (i1) x 4 (i2) voxels at one i3; wave w owns row
@@ -858,8 +918,19 @@ int cnn_forward(sh_ctx *ctx, const float *P, const float *x, float xscale, const
     hipLaunchKernelGGL(pack_weights, dim3(ceil_div(5 * per, TB)), dim3(TB), 0, ctx->stream, P, W.Wf, W.Wb);
     SH_TRY(sh_launch_check(ctx, "pack_weights"));
     PROF_A(ctx);
-    hipLaunchKernelGGL(conv1_fwd, dim3(ceil_div(n, TB)), dim3(TB), 0, ctx->stream, x, xscale, P, W.act[1], n1, n2, n3,
-                       D[0], FastDiv((unsigned)n1), FastDiv((unsigned)n2), xsp);
+    // SH_CNN_C1=g: the one-voxel-per-thread kernel with 27 global loads per
+    // voxel instead of the LDS-staged one (identical results)
+    const char *c1e = getenv("SH_CNN_C1");
+    if (D[0] == 1 && !(c1e && c1e[0] == 'g')) {
+        const long tiles = (long)ceil_div(n1, 64) * ceil_div(n2, 4);
+        const int nseg = (int)std::max<long>(1, std::min<long>(n3 / 8, (2048 + tiles - 1) / tiles));
+        const int seg = ceil_div(n3, nseg);
+        hipLaunchKernelGGL(conv1_fwd_tile, dim3((unsigned)tiles, ceil_div(n3, seg)), dim3(TB), 0, ctx->stream, x, xscale,
+                           P, W.act[1], n1, n2, n3, seg, xsp);
+    } else {
+        hipLaunchKernelGGL(conv1_fwd, dim3(ceil_div(n, TB)), dim3(TB), 0, ctx->stream, x, xscale, P, W.act[1], n1, n2,
+                           n3, D[0], FastDiv((unsigned)n1), FastDiv((unsigned)n2), xsp);
+    }
     PROF_B(ctx, "conv1_fwd", (double)n * (4.0 + 64.0));
     SH_TRY(sh_launch_check(ctx, "conv1_fwd"));
     __bf16 *cur = W.act[1];

@@ -117,19 +117,21 @@ def test_dilated_forward_and_gradient(ws, dil):
 def test_conv_kernels_identical(ws, monkeypatch, shape):
     """The three dilation-1 convolution kernels (plane-scatter, the
     default; wide tiles; plain) sum every output's taps in the same order,
-    so forward and gradient must be bit-identical.  (300, 10, 6) has an i1
-    tail tile and rows past n2; (200, 50, 45) has more tiles than blocks,
-    so a block's z-range crosses from one column into the next."""
+    so forward and gradient must be bit-identical; so do the two first-layer
+    kernels (input staged in LDS, the default; 27 global loads per voxel,
+    SH_CNN_C1=g).  (300, 10, 6) has an i1 tail tile and rows past n2;
+    (200, 50, 45) has more tiles than blocks, so a block's z-range crosses
+    from one column into the next."""
     p, x, t, wt = _case(5, shape)
     xs = 0.4
     got = ws.cnn_forward(_dev(p), _dev(x), shape, xs).cpu().numpy()
     loss, g = ws.cnn_grad(_dev(p), _dev(x), shape, xs, _dev(t), _dev(wt))
     g = g.cpu().numpy()
-    for kind in ("n", "w"):
-        monkeypatch.setenv("SH_CNN_CONV", kind)
+    for var, kind in (("SH_CNN_CONV", "n"), ("SH_CNN_CONV", "w"), ("SH_CNN_C1", "g")):
+        monkeypatch.setenv(var, kind)
         got_k = ws.cnn_forward(_dev(p), _dev(x), shape, xs).cpu().numpy()
         loss_k, g_k = ws.cnn_grad(_dev(p), _dev(x), shape, xs, _dev(t), _dev(wt))
-        monkeypatch.delenv("SH_CNN_CONV")
+        monkeypatch.delenv(var)
         assert np.array_equal(got, got_k), kind
         assert loss == loss_k and np.array_equal(g, g_k.cpu().numpy()), kind
     if np.prod(shape) <= 20000:
