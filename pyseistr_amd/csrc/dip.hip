@@ -176,6 +176,76 @@ smooth_contig(const float *__restrict__ in, float *__restrict__ out, int n1, lon
     sm_partials<EPI>(acc, part, gridDim.x, blockIdx.x, sh);
 }
 
+// The axis-1 pass without a CG epilogue (plain, or with the CG prologue)
+// with one wave per task: each wave stages its chunk (and halo) in its own
+// LDS row, so no block-wide barrier is needed, and loads the next task's
+// samples into registers while it smooths the current one (up to 12 per
+// lane).  Same operations as smooth_contig<PRO, EPI_PLAIN>, including the
+// prologue's write-back of the owned samples.
+template <int PRO>
+__global__ void __launch_bounds__(256)
+smooth_contig_wave(const float *__restrict__ in, float *__restrict__ out, int n1, long ntask, int nb, CgArgs A,
+                   const DipScal *sc) {
+    if (sc->cg_done) return;
+    constexpr int LW = SM_CH1 + 2 * SM_HMAX, KS = (LW + 63) / 64;
+    constexpr int NR = (PRO == PRO_PLAIN) ? 1 : 5;
+    __shared__ float tile[4][LW];
+    __shared__ float wts[SM_HMAX + 1];
+    const int h = nb - 1;
+    const int nch = (n1 + SM_CH1 - 1) / SM_CH1;
+    const bool pend = (PRO == PRO_CG) ? (sc->pending != 0) : false;
+    const float a2 = pend ? (float)sc->alpha2 : 0.0f;
+    const int cur = (PRO == PRO_CG) ? sc->cur : 0;
+    sm_weights(wts, nb);
+    __syncthreads();
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    float *tl = tile[wv];
+    const long per = (ntask + gridDim.x - 1) / gridDim.x;
+    const long t0 = (long)blockIdx.x * per, t1 = min(ntask, t0 + per);
+    float raw[KS][NR];
+    long gi[KS];
+    bool own[KS];
+    auto fetch = [&](long task) {
+        const long base = (task / nch) * n1;
+        const int c0 = (int)(task % nch) * SM_CH1;
+        const int len = min(SM_CH1, n1 - c0);
+#pragma unroll
+        for (int k = 0; k < KS; k++) {
+            const int t = lane + 64 * k;
+            if (t < len + 2 * h) {
+                const int pos = c0 - h + t;
+                gi[k] = base + fold_idx(pos, n1);
+                own[k] = pos >= c0 && pos < c0 + len;
+                sm_fetch<PRO>(in, gi[k], A, pend, cur, raw[k]);
+            }
+        }
+    };
+    if (t0 + wv < t1) fetch(t0 + wv);
+    for (long task = t0 + wv; task < t1; task += 4) {
+        const long base = (task / nch) * n1;
+        const int c0 = (int)(task % nch) * SM_CH1;
+        const int len = min(SM_CH1, n1 - c0);
+        // the wave's reads of its previous task are issued before these
+        // writes, and LDS instructions of one wave complete in order
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int k = 0; k < KS; k++) {
+            const int t = lane + 64 * k;
+            if (t < len + 2 * h) tl[t] = sm_finish<PRO>(raw[k], gi[k], own[k], A, pend, a2, cur);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (task + 4 < t1) fetch(task + 4);
+        for (int t = lane; t < len; t += 64) {
+            const float *c = tl + t + h;
+            float v = wts[0] * c[0];
+            for (int k = 1; k <= h; k++) v += wts[k] * (c[-k] + c[k]);
+            out[base + c0 + t] = v;
+        }
+    }
+}
+
 // Axes 2/3 (strided): task = tw = blockDim.x consecutive i1 x a chunk of chs
 // samples along the axis x one position of the other strided axis (ntx x
 // nty x nother tasks, i1 chunk fastest); index = i1 + o*ostride + q*stride.
@@ -728,7 +798,18 @@ int smooth_apply(sh_ctx *ctx, const DipGeom &g, const SmoothPlan &sp, const floa
     if (P == PP && E == EE)                                                                   \
         hipLaunchKernelGGL((smooth_contig<PP, EE>), grid, dim3(256), 0, ctx->stream, src, dst, g.n1, ntask, nb, A, \
                            W.sc, W.part, first);
-            SM_C(0, 0) SM_C(0, 1) SM_C(0, 2) SM_C(1, 0) SM_C(1, 1) SM_C(1, 2)
+            // the plain pass: one wave per task (SH_SMOOTH_WAVE=0: one block
+            // per task).  Measured at 512x256x256: warm step 48.2 -> 47.7 ms;
+            // with the CG prologue (smooth_contig_wave<1>, 122 VGPRs) that
+            // pass went 5.97 -> 6.94 ms per warm step, so it keeps the block
+            // form.
+            const char *we = getenv("SH_SMOOTH_WAVE");
+            if (P == PRO_PLAIN && E == EPI_PLAIN && !(we && we[0] == '0')) {
+                hipLaunchKernelGGL((smooth_contig_wave<PRO_PLAIN>), grid, dim3(256), 0, ctx->stream, src, dst, g.n1,
+                                   ntask, nb, A, W.sc);
+            } else {
+                SM_C(0, 0) SM_C(0, 1) SM_C(0, 2) SM_C(1, 0) SM_C(1, 1) SM_C(1, 2)
+            }
 #undef SM_C
         } else {
             int naxis = ax == 1 ? g.n2 : g.n3;
