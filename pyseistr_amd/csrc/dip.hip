@@ -246,6 +246,47 @@ smooth_contig_wave(const float *__restrict__ in, float *__restrict__ out, int n1
     }
 }
 
+// The plain strided pass (axis 2 or 3) with one wave per task and each
+// lane owning one i1 column of it: the lane keeps the task's CH samples
+// along the axis and their halo in registers (no LDS, no barrier).  Same
+// stencil operations as smooth_strided<PRO_PLAIN, EPI_PLAIN>.
+template <int H, int CH>
+__global__ void __launch_bounds__(256)
+smooth_strided_col(const float *__restrict__ in, float *__restrict__ out, int n1, int naxis, long stride,
+                   long ostride, int nother, const DipScal *sc) {
+    if (sc->cg_done) return;
+    constexpr int nb = H + 1;
+    float wts[H + 1];
+#pragma unroll
+    for (int k = 0; k <= H; k++) wts[k] = (float)((double)(nb - k) / ((double)nb * nb));
+    const int lane = threadIdx.x & 63;
+    const long wave = (long)blockIdx.x * 4 + (threadIdx.x >> 6), nwave = (long)gridDim.x * 4;
+    const int ntx = (n1 + 63) / 64, nty = (naxis + CH - 1) / CH;
+    const long ntask = (long)ntx * nty * nother;
+    for (long task = wave; task < ntask; task += nwave) {
+        const int bx = (int)(task % ntx);
+        const long rest = task / ntx;
+        const int by = (int)(rest % nty), bz = (int)(rest / nty);
+        const int i1 = bx * 64 + lane;
+        const int c0 = by * CH, len = min(CH, naxis - c0);
+        const long base = (long)bz * ostride + i1;
+        if (i1 >= n1) continue;
+        float v[CH + 2 * H];
+#pragma unroll
+        for (int r = 0; r < CH + 2 * H; r++)
+            v[r] = (r < len + 2 * H) ? in[base + (long)fold_idx(c0 - H + r, naxis) * stride] : 0.0f;
+#pragma unroll
+        for (int r = 0; r < CH; r++) {
+            if (r < len) {
+                float t = wts[0] * v[r + H];
+#pragma unroll
+                for (int k = 1; k <= H; k++) t += wts[k] * (v[r + H - k] + v[r + H + k]);
+                out[base + (long)(c0 + r) * stride] = t;
+            }
+        }
+    }
+}
+
 // Axes 2/3 (strided): task = tw = blockDim.x consecutive i1 x a chunk of chs
 // samples along the axis x one position of the other strided axis (ntx x
 // nty x nother tasks, i1 chunk fastest); index = i1 + o*ostride + q*stride.
@@ -839,12 +880,30 @@ int smooth_apply(sh_ctx *ctx, const DipGeom &g, const SmoothPlan &sp, const floa
             dim3 grid((unsigned)std::min<long>(ntask, DIP_GRID_CAP));
             nblk = grid.x;
             size_t lds = (size_t)(chs + 2 * (nb - 1)) * tw * sizeof(float);
+            // the plain pass with registers per i1 column (SH_SMOOTH_COL=1;
+            // radii 2..9).  Measured at 512x256x256: the pass itself 3.82 ->
+            // 3.39 ms per warm step, but the warm step (two dip directions
+            // side by side) 47.7 -> 48.1 ms, the passes after it slower; so
+            // it stays opt-in.
+            const char *ce = getenv("SH_SMOOTH_COL");
+            const bool col = P == PRO_PLAIN && E == EPI_PLAIN && ce && ce[0] == '1' && nb >= 2 && nb <= 9;
+            if (col) {
+                const long ct = (long)ceil_div(g.n1, 64) * ceil_div(naxis, 32) * nother;
+                dim3 cg((unsigned)std::min<long>(ceil_div(ct, 4), 8192));
+#define SM_COL(HH)                                                                                            \
+    if (nb - 1 == HH)                                                                                         \
+        hipLaunchKernelGGL((smooth_strided_col<HH, 32>), cg, dim3(256), 0, ctx->stream, src, dst, g.n1, naxis, \
+                           stride, ostride, nother, W.sc);
+                SM_COL(1) SM_COL(2) SM_COL(3) SM_COL(4) SM_COL(5) SM_COL(6) SM_COL(7) SM_COL(8)
+#undef SM_COL
+            } else {
 #define SM_S(PP, EE)                                                                          \
     if (P == PP && E == EE)                                                                   \
         hipLaunchKernelGGL((smooth_strided<PP, EE>), grid, dim3(tw, 256 / tw), lds, ctx->stream, src, dst, g.n1, \
                            naxis, stride, ostride, nother, nb, chs, A, W.sc, W.part, first);
             SM_S(0, 0) SM_S(0, 1) SM_S(0, 2) SM_S(1, 0) SM_S(1, 1) SM_S(1, 2)
 #undef SM_S
+            }
         }
         PROF_B(ctx, names[ax != 0][P][E], bps * (double)g.n);
         SH_TRY(sh_launch_check(ctx, "smooth"));

@@ -207,9 +207,11 @@ def test_bad_arguments_raise(ps):
 
 def test_wave_axis1_pass_identical(ps, monkeypatch):
     """The plain axis-1 smoothing pass with one wave per trace chunk (the
-    default) and with one block per chunk (SH_SMOOTH_WAVE=0) run the same
-    stencils and carry no sums, so the dips are bit-identical; n1 = 700
-    splits each trace into two chunks."""
+    default) and with one block per chunk (SH_SMOOTH_WAVE=0), and the plain
+    strided pass from LDS tiles (the default) and from registers per i1
+    column (SH_SMOOTH_COL=1), run the same stencils and carry no sums, so
+    the dips are bit-identical; n1 = 700 splits each trace into two
+    chunks."""
     from pyseistr_amd import synth
     for shape, rect in (((96, 40, 36), [5, 5, 5]), ((700, 12, 5), [9, 3, 2])):
         v, _ = synth.volume3d(shape, seed=4, n_events=3, smax=2.0)
@@ -217,7 +219,11 @@ def test_wave_axis1_pass_identical(ps, monkeypatch):
         monkeypatch.setenv("SH_SMOOTH_WAVE", "0")
         b = np.stack(ps.dip3dc(v, niter=2, liter=6, rect=rect))
         monkeypatch.delenv("SH_SMOOTH_WAVE")
+        monkeypatch.setenv("SH_SMOOTH_COL", "1")
+        c = np.stack(ps.dip3dc(v, niter=2, liter=6, rect=rect))
+        monkeypatch.delenv("SH_SMOOTH_COL")
         assert np.array_equal(a, b), shape
+        assert np.array_equal(a, c), shape
 
 
 @pytest.mark.parametrize("mode", ["1", "3"])
