@@ -241,7 +241,7 @@ smooth_contig_wave(const float *__restrict__ in, float *__restrict__ out, int n1
             const float *c = tl + t + h;
             float v = wts[0] * c[0];
             for (int k = 1; k <= h; k++) v += wts[k] * (c[-k] + c[k]);
-            out[base + c0 + t] = v;
+            st_nt(&out[base + c0 + t], v);
         }
     }
 }
@@ -281,7 +281,7 @@ smooth_strided_col(const float *__restrict__ in, float *__restrict__ out, int n1
                 float t = wts[0] * v[r + H];
 #pragma unroll
                 for (int k = 1; k <= H; k++) t += wts[k] * (v[r + H - k] + v[r + H + k]);
-                out[base + (long)(c0 + r) * stride] = t;
+                st_nt(&out[base + (long)(c0 + r) * stride], t);
             }
         }
     }

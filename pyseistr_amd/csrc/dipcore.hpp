@@ -6,6 +6,12 @@
 
 namespace dipcore {
 
+// Output stores of the plain smoothing passes (the intermediate arrays
+// between the axis passes, read only by the next pass): non-temporal.
+// Measured at 512x256x256: warm step 47.6 -> 47.4 ms; the same for every
+// array the dip kernels write made it slower (47.9 ms).
+__device__ inline void st_nt(float *p, float v) { __builtin_nontemporal_store(v, p); }
+
 constexpr int SM_CH1 = 512;   // samples of a line per block, axis-1 pass
 constexpr int SM_CHS = 64;    // samples along the axis per task, strided pass
 constexpr int SM_TW = 64;     // consecutive i1 samples per row of a strided pass
@@ -170,7 +176,7 @@ template <int EPI>
 __device__ inline void sm_store(float s, float *out, long gi, const CgArgs &A, bool pend, float a2, float a1,
                                 int first, double *acc) {
     if (EPI == EPI_PLAIN) {
-        out[gi] = s;
+        st_nt(&out[gi], s);
     } else if (EPI == EPI_GP) {  // gp = eps*p + S(gx), p updated lazily
         float pv = A.pc[gi];
         if (pend) {
