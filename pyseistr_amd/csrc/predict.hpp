@@ -777,9 +777,9 @@ SH_HD void solve_factored(const Acc *in, const Acc *q, const bool *rev, const Fa
 // types (IA, FA, SK): Acc / Factor / Sink, or the buffer-resource ones of
 // the 3-D level kernels; the arithmetic does not depend on them.
 template <int NW, int NF, int D, bool TP = false, bool MIDR = false, class IA = Acc, class FA = Factor,
-          class SK = Sink, class TQ = Factor>
+          class SK = Sink, class TQ = Factor, class SO = SK>
 SH_HD void solve_factored_pf(const IA *in, const IA *q, const bool *rev, const FA &F, SK y, int n1, float e,
-                             SK out, const TQ *tq = nullptr) {
+                             SO out, const TQ *tq = nullptr) {
     constexpr int NA = 2 * NW + 1, NB = 2 * NW;
     float tb[4];
     if (NF == 1) {

@@ -181,7 +181,10 @@ struct BTrace {
         return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)lane, row_off(k, row), 0));
     }
 };
-struct BSink {
+// AUX: the store's cache policy (2: non-temporal, for outputs that only a
+// later launch reads)
+template <int AUX>
+struct BSinkT {
     __amdgpu_buffer_rsrc_t r;
     unsigned lane, row;
     struct Ref {
@@ -191,12 +194,15 @@ struct BSink {
             return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)lane, (int)off, 0));
         }
         __device__ void operator=(float v) const {
-            __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)lane, (int)off, 0);
+            __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)lane, (int)off, AUX);
         }
     };
     __device__ Ref operator[](long k) const { return Ref{r, lane, (unsigned)row_off(k, row)}; }
 };
-__device__ inline BTrace sink_acc(const BSink &y) { return BTrace{y.r, y.lane, y.row}; }
+using BSink = BSinkT<0>;
+using BSinkNT = BSinkT<2>;
+template <int AUX>
+__device__ inline BTrace sink_acc(const BSinkT<AUX> &y) { return BTrace{y.r, y.lane, y.row}; }
 
 // factor of one trace (layout of struct Factor): d plane and the rows of
 // NB off-diagonal entries side by side
@@ -292,7 +298,8 @@ spray3_level(const float *__restrict__ pi, const float *__restrict__ px, float *
                          row * NB};
         const unsigned lt = (unsigned)(cnt * gx8 * TB);  // threads of this launch
         const BSink y{plane_rsrc(scr, (unsigned long)lt * n1 * 4), (unsigned)tid * 4u, lt * 4u};
-        const BSink o{plane_rsrc(U + (long)nd.slot * n, (unsigned long)n * 4), (unsigned)j * 4u, row};
+        // the slot is read only by later levels and the median: non-temporal
+        const BSinkNT o{plane_rsrc(U + (long)nd.slot * n, (unsigned long)n * 4), (unsigned)j * 4u, row};
         solve_factored_pf<NW, NF, PF>(in, q, rev, Fv, y, n1, e, o);
         return;
     }
