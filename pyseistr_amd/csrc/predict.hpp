@@ -64,17 +64,25 @@ SH_HD inline void fac_row(const Factor &F, long k, float *ok) {
     }
 }
 
+// Factor rows are read only by later launches: non-temporal stores.
+typedef float fac_f4 __attribute__((ext_vector_type(4)));
+typedef float fac_f2 __attribute__((ext_vector_type(2)));
 template <int NB>
 SH_HD inline void fac_put(const FactorOut &F, long k, float dk, const float *ok) {
-    F.d[k * F.fs] = dk;
+    __builtin_nontemporal_store(dk, &F.d[k * F.fs]);
     float *p = F.o + (long)NB * k * F.fs;
     if constexpr (NB % 4 == 0) {
 #pragma unroll
-        for (int c = 0; c < NB / 4; c++)
-            reinterpret_cast<float4 *>(p)[c] = make_float4(ok[4 * c], ok[4 * c + 1], ok[4 * c + 2], ok[4 * c + 3]);
+        for (int c = 0; c < NB / 4; c++) {
+            const fac_f4 v = {ok[4 * c], ok[4 * c + 1], ok[4 * c + 2], ok[4 * c + 3]};
+            __builtin_nontemporal_store(v, reinterpret_cast<fac_f4 *>(p) + c);
+        }
     } else {
 #pragma unroll
-        for (int c = 0; c < NB / 2; c++) reinterpret_cast<float2 *>(p)[c] = make_float2(ok[2 * c], ok[2 * c + 1]);
+        for (int c = 0; c < NB / 2; c++) {
+            const fac_f2 v = {ok[2 * c], ok[2 * c + 1]};
+            __builtin_nontemporal_store(v, reinterpret_cast<fac_f2 *>(p) + c);
+        }
     }
 }
 
