@@ -113,7 +113,7 @@ cg_result_kernel(const float *__restrict__ xa, const float *__restrict__ xb, con
 
 // Axis 1 (contiguous): task = one chunk of one trace; a block runs a
 // contiguous range of tasks (ntask = chunks * traces).
-template <int PRO, int EPI>
+template <int PRO, int EPI, bool NT = false>
 __global__ void __launch_bounds__(256)
 smooth_contig(const float *__restrict__ in, float *__restrict__ out, int n1, long ntask, int nb, CgArgs A,
               const DipScal *sc, double *part, int first) {
@@ -170,7 +170,7 @@ smooth_contig(const float *__restrict__ in, float *__restrict__ out, int n1, lon
             const float *c = tile + t + h;
             float s = wts[0] * c[0];
             for (int k = 1; k <= h; k++) s += wts[k] * (c[-k] + c[k]);
-            sm_store<EPI>(s, out, base + c0 + t, A, pend, a2, a1, first, acc);
+            sm_store<EPI, NT>(s, out, base + c0 + t, A, pend, a2, a1, first, acc);
         }
     }
     sm_partials<EPI>(acc, part, gridDim.x, blockIdx.x, sh);
@@ -182,7 +182,7 @@ smooth_contig(const float *__restrict__ in, float *__restrict__ out, int n1, lon
 // samples into registers while it smooths the current one (up to 12 per
 // lane).  Same operations as smooth_contig<PRO, EPI_PLAIN>, including the
 // prologue's write-back of the owned samples.
-template <int PRO>
+template <int PRO, bool NT>
 __global__ void __launch_bounds__(256)
 smooth_contig_wave(const float *__restrict__ in, float *__restrict__ out, int n1, long ntask, int nb, CgArgs A,
                    const DipScal *sc) {
@@ -241,7 +241,8 @@ smooth_contig_wave(const float *__restrict__ in, float *__restrict__ out, int n1
             const float *c = tl + t + h;
             float v = wts[0] * c[0];
             for (int k = 1; k <= h; k++) v += wts[k] * (c[-k] + c[k]);
-            st_nt(&out[base + c0 + t], v);
+            if constexpr (NT) st_nt(&out[base + c0 + t], v);
+            else out[base + c0 + t] = v;
         }
     }
 }
@@ -281,7 +282,7 @@ smooth_strided_col(const float *__restrict__ in, float *__restrict__ out, int n1
                 float t = wts[0] * v[r + H];
 #pragma unroll
                 for (int k = 1; k <= H; k++) t += wts[k] * (v[r + H - k] + v[r + H + k]);
-                st_nt(&out[base + (long)(c0 + r) * stride], t);
+                out[base + (long)(c0 + r) * stride] = t;
             }
         }
     }
@@ -290,7 +291,7 @@ smooth_strided_col(const float *__restrict__ in, float *__restrict__ out, int n1
 // Axes 2/3 (strided): task = tw = blockDim.x consecutive i1 x a chunk of chs
 // samples along the axis x one position of the other strided axis (ntx x
 // nty x nother tasks, i1 chunk fastest); index = i1 + o*ostride + q*stride.
-template <int PRO, int EPI>
+template <int PRO, int EPI, bool NT = false>
 __global__ void __launch_bounds__(256)
 smooth_strided(const float *__restrict__ in, float *__restrict__ out, int n1, int naxis, long stride,
                long ostride, int nother, int nb, int chs, CgArgs A, const DipScal *sc, double *part, int first) {
@@ -365,7 +366,7 @@ smooth_strided(const float *__restrict__ in, float *__restrict__ out, int n1, in
                 const float *c = tile + (row + h) * tw + tx;
                 float s = wts[0] * c[0];
                 for (int k = 1; k <= h; k++) s += wts[k] * (c[-k * tw] + c[k * tw]);
-                sm_store<EPI>(s, out, base + (long)(c0 + row) * stride, A, pend, a2, a1, first, acc);
+                sm_store<EPI, NT>(s, out, base + (long)(c0 + row) * stride, A, pend, a2, a1, first, acc);
             }
         }
     }
@@ -814,6 +815,9 @@ int smooth_apply(sh_ctx *ctx, const DipGeom &g, const SmoothPlan &sp, const floa
     // pair (fused == 3): axis 1 as a separable pass into t1, then axes 2
     // and 3 in one pass (smooth_pair23)
     const int nsep = sp.fused == 3 ? 1 : sp.naxes;
+    // plain passes of volumes of 2^24 samples or more write their outputs
+    // with non-temporal stores (their arrays do not stay in the L2)
+    const bool nt = g.n >= (1L << 24);
     const float *src = in;
     for (int a = 0; a < nsep; a++) {
         bool is_first = a == 0, is_last = a == sp.naxes - 1;
@@ -835,10 +839,15 @@ int smooth_apply(sh_ctx *ctx, const DipGeom &g, const SmoothPlan &sp, const floa
             long ntask = (long)ceil_div(g.n1, SM_CH1) * g.n2 * g.n3;
             dim3 grid((unsigned)std::min<long>(ntask, DIP_GRID_CAP));
             nblk = grid.x;
-#define SM_C(PP, EE)                                                                          \
-    if (P == PP && E == EE)                                                                   \
-        hipLaunchKernelGGL((smooth_contig<PP, EE>), grid, dim3(256), 0, ctx->stream, src, dst, g.n1, ntask, nb, A, \
-                           W.sc, W.part, first);
+#define SM_C(PP, EE)                                                                                        \
+    if (P == PP && E == EE) {                                                                               \
+        if (EE == EPI_PLAIN && nt)                                                                          \
+            hipLaunchKernelGGL((smooth_contig<PP, EE, true>), grid, dim3(256), 0, ctx->stream, src, dst, g.n1, ntask, \
+                               nb, A, W.sc, W.part, first);                                                 \
+        else                                                                                                \
+            hipLaunchKernelGGL((smooth_contig<PP, EE, false>), grid, dim3(256), 0, ctx->stream, src, dst, g.n1,      \
+                               ntask, nb, A, W.sc, W.part, first);                                          \
+    }
             // the plain pass: one wave per task (SH_SMOOTH_WAVE=0: one block
             // per task).  Measured at 512x256x256: warm step 48.2 -> 47.7 ms;
             // with the CG prologue (smooth_contig_wave<1>, 122 VGPRs) that
@@ -846,8 +855,12 @@ int smooth_apply(sh_ctx *ctx, const DipGeom &g, const SmoothPlan &sp, const floa
             // form.
             const char *we = getenv("SH_SMOOTH_WAVE");
             if (P == PRO_PLAIN && E == EPI_PLAIN && !(we && we[0] == '0')) {
-                hipLaunchKernelGGL((smooth_contig_wave<PRO_PLAIN>), grid, dim3(256), 0, ctx->stream, src, dst, g.n1,
-                                   ntask, nb, A, W.sc);
+                if (nt)
+                    hipLaunchKernelGGL((smooth_contig_wave<PRO_PLAIN, true>), grid, dim3(256), 0, ctx->stream, src, dst,
+                                       g.n1, ntask, nb, A, W.sc);
+                else
+                    hipLaunchKernelGGL((smooth_contig_wave<PRO_PLAIN, false>), grid, dim3(256), 0, ctx->stream, src, dst,
+                                       g.n1, ntask, nb, A, W.sc);
             } else {
                 SM_C(0, 0) SM_C(0, 1) SM_C(0, 2) SM_C(1, 0) SM_C(1, 1) SM_C(1, 2)
             }
@@ -897,10 +910,15 @@ int smooth_apply(sh_ctx *ctx, const DipGeom &g, const SmoothPlan &sp, const floa
                 SM_COL(1) SM_COL(2) SM_COL(3) SM_COL(4) SM_COL(5) SM_COL(6) SM_COL(7) SM_COL(8)
 #undef SM_COL
             } else {
-#define SM_S(PP, EE)                                                                          \
-    if (P == PP && E == EE)                                                                   \
-        hipLaunchKernelGGL((smooth_strided<PP, EE>), grid, dim3(tw, 256 / tw), lds, ctx->stream, src, dst, g.n1, \
-                           naxis, stride, ostride, nother, nb, chs, A, W.sc, W.part, first);
+#define SM_S(PP, EE)                                                                                        \
+    if (P == PP && E == EE) {                                                                               \
+        if (EE == EPI_PLAIN && nt)                                                                          \
+            hipLaunchKernelGGL((smooth_strided<PP, EE, true>), grid, dim3(tw, 256 / tw), lds, ctx->stream, src, dst, \
+                               g.n1, naxis, stride, ostride, nother, nb, chs, A, W.sc, W.part, first);      \
+        else                                                                                                \
+            hipLaunchKernelGGL((smooth_strided<PP, EE, false>), grid, dim3(tw, 256 / tw), lds, ctx->stream, src,    \
+                               dst, g.n1, naxis, stride, ostride, nother, nb, chs, A, W.sc, W.part, first); \
+    }
             SM_S(0, 0) SM_S(0, 1) SM_S(0, 2) SM_S(1, 0) SM_S(1, 1) SM_S(1, 2)
 #undef SM_S
             }

@@ -7,9 +7,11 @@
 namespace dipcore {
 
 // Output stores of the plain smoothing passes (the intermediate arrays
-// between the axis passes, read only by the next pass): non-temporal.
-// Measured at 512x256x256: warm step 47.6 -> 47.4 ms; the same for every
-// array the dip kernels write made it slower (47.9 ms).
+// between the axis passes, read only by the next pass): non-temporal for
+// volumes of 2^24 samples or more (the kernels' NT argument).  Measured at 512x256x256:
+// warm step 47.6 -> 47.4 ms; the same for every array the dip kernels
+// write made it slower (47.9 ms), and on a 512x256 gather, whose arrays
+// stay in the L2, non-temporal outputs cost 5 %.
 __device__ inline void st_nt(float *p, float v) { __builtin_nontemporal_store(v, p); }
 
 constexpr int SM_CH1 = 512;   // samples of a line per block, axis-1 pass
@@ -172,11 +174,12 @@ __device__ inline float sm_finish(const float *raw, long gi, bool own, const CgA
     return (-xv) + rv * raw[2];
 }
 
-template <int EPI>
+template <int EPI, bool NT = false>
 __device__ inline void sm_store(float s, float *out, long gi, const CgArgs &A, bool pend, float a2, float a1,
                                 int first, double *acc) {
     if (EPI == EPI_PLAIN) {
-        st_nt(&out[gi], s);
+        if constexpr (NT) st_nt(&out[gi], s);
+        else out[gi] = s;
     } else if (EPI == EPI_GP) {  // gp = eps*p + S(gx), p updated lazily
         float pv = A.pc[gi];
         if (pend) {

@@ -64,24 +64,38 @@ SH_HD inline void fac_row(const Factor &F, long k, float *ok) {
     }
 }
 
-// Factor rows are read only by later launches: non-temporal stores.
+// NT: the rows are read only by later launches over a large volume, so
+// they are written with non-temporal stores (for small volumes the L2
+// keeps them for the readers).
 typedef float fac_f4 __attribute__((ext_vector_type(4)));
 typedef float fac_f2 __attribute__((ext_vector_type(2)));
-template <int NB>
+template <int NB, bool NT = false>
 SH_HD inline void fac_put(const FactorOut &F, long k, float dk, const float *ok) {
-    __builtin_nontemporal_store(dk, &F.d[k * F.fs]);
     float *p = F.o + (long)NB * k * F.fs;
-    if constexpr (NB % 4 == 0) {
+    if constexpr (NT) {
+        __builtin_nontemporal_store(dk, &F.d[k * F.fs]);
+        if constexpr (NB % 4 == 0) {
 #pragma unroll
-        for (int c = 0; c < NB / 4; c++) {
-            const fac_f4 v = {ok[4 * c], ok[4 * c + 1], ok[4 * c + 2], ok[4 * c + 3]};
-            __builtin_nontemporal_store(v, reinterpret_cast<fac_f4 *>(p) + c);
+            for (int c = 0; c < NB / 4; c++) {
+                const fac_f4 v = {ok[4 * c], ok[4 * c + 1], ok[4 * c + 2], ok[4 * c + 3]};
+                __builtin_nontemporal_store(v, reinterpret_cast<fac_f4 *>(p) + c);
+            }
+        } else {
+#pragma unroll
+            for (int c = 0; c < NB / 2; c++) {
+                const fac_f2 v = {ok[2 * c], ok[2 * c + 1]};
+                __builtin_nontemporal_store(v, reinterpret_cast<fac_f2 *>(p) + c);
+            }
         }
     } else {
+        F.d[k * F.fs] = dk;
+        if constexpr (NB % 4 == 0) {
 #pragma unroll
-        for (int c = 0; c < NB / 2; c++) {
-            const fac_f2 v = {ok[2 * c], ok[2 * c + 1]};
-            __builtin_nontemporal_store(v, reinterpret_cast<fac_f2 *>(p) + c);
+            for (int c = 0; c < NB / 4; c++)
+                reinterpret_cast<float4 *>(p)[c] = make_float4(ok[4 * c], ok[4 * c + 1], ok[4 * c + 2], ok[4 * c + 3]);
+        } else {
+#pragma unroll
+            for (int c = 0; c < NB / 2; c++) reinterpret_cast<float2 *>(p)[c] = make_float2(ok[2 * c], ok[2 * c + 1]);
         }
     }
 }
@@ -563,7 +577,7 @@ SH_HD inline void taps_from(float qv, int kk, int n1, bool rev, float *a) {
 
 // factor_sweep with the slope loads software-pipelined D samples ahead (a
 // register ring; loop unrolled by D); same arithmetic, identical factor.
-template <int NW, int NF, int D>
+template <int NW, int NF, int D, bool NT = false>
 SH_HD void factor_sweep_pf(const Acc *q, const bool *rev, int n1, float e, FactorOut F) {
     constexpr int NA = 2 * NW + 1, NB = 2 * NW;
     float A[NF][NA][NA];
@@ -629,7 +643,7 @@ SH_HD void factor_sweep_pf(const Acc *q, const bool *rev, int n1, float e, Facto
                 ok[nidx] = sm / dk;
             }
         }
-        fac_put<NB>(F, k, dk, ok);
+        fac_put<NB, NT>(F, k, dk, ok);
 #pragma unroll
         for (int r = NB - 1; r > 0; r--) {
             Ld[r] = Ld[r - 1];

@@ -137,8 +137,10 @@ factor3_kernel(const float *__restrict__ pi, const float *__restrict__ px, float
     constexpr int NB = 2 * NW;
     const FactorOut Fs = fac_out_at(F + (long)v * fac_plane_floats(NB, n), NB, n, j, ntr);
     // slope loads pipelined 8 samples ahead (identical factor)
-    if (nf == 1) factor_sweep_pf<NW, 1, 8>(q, rev, n1, e, Fs);
-    else factor_sweep_pf<NW, 2, 8>(q, rev, n1, e, Fs);
+    // (the factor planes of a 3-D volume are read by the later level
+    // launches only: non-temporal stores)
+    if (nf == 1) factor_sweep_pf<NW, 1, 8, true>(q, rev, n1, e, Fs);
+    else factor_sweep_pf<NW, 2, 8, true>(q, rev, n1, e, Fs);
 }
 
 // Block -> (trace block, node) of a level launch over a 1-D grid of
