@@ -183,8 +183,10 @@ struct BTrace {
         return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)lane, row_off(k, row), 0));
     }
 };
-// AUX: the store's cache policy (2: non-temporal, for outputs that only a
-// later launch reads)
+// AUX: the cache policy of the stores and loads (2: non-temporal, for
+// outputs that only a later launch reads and for the substitution scratch,
+// whose rows come back to the same thread after the caches have turned
+// over anyway)
 template <int AUX>
 struct BSinkT {
     __amdgpu_buffer_rsrc_t r;
@@ -193,7 +195,7 @@ struct BSinkT {
         __amdgpu_buffer_rsrc_t r;
         unsigned lane, off;
         __device__ operator float() const {
-            return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)lane, (int)off, 0));
+            return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)lane, (int)off, AUX));
         }
         __device__ void operator=(float v) const {
             __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)lane, (int)off, AUX);
@@ -299,7 +301,9 @@ spray3_level(const float *__restrict__ pi, const float *__restrict__ px, float *
                          row,
                          row * NB};
         const unsigned lt = (unsigned)(cnt * gx8 * TB);  // threads of this launch
-        const BSink y{plane_rsrc(scr, (unsigned long)lt * n1 * 4), (unsigned)tid * 4u, lt * 4u};
+        // scratch written and read with non-temporal accesses (keeps the
+        // shared factor and slope lines in the L2)
+        const BSinkNT y{plane_rsrc(scr, (unsigned long)lt * n1 * 4), (unsigned)tid * 4u, lt * 4u};
         // the slot is read only by later levels and the median: non-temporal
         const BSinkNT o{plane_rsrc(U + (long)nd.slot * n, (unsigned long)n * 4), (unsigned)j * 4u, row};
         solve_factored_pf<NW, NF, PF>(in, q, rev, Fv, y, n1, e, o);
