@@ -58,6 +58,14 @@ int sh_count_nonfinite(sh_ctx *ctx, const float *a, long n, long *nbad);
 int sh_ctx_set_stream(sh_ctx *ctx, void *hip_stream);
 /* Release cached device workspace. */
 int sh_ctx_trim(sh_ctx *ctx);
+/* Layout helper.  The reference's Python wrappers flatten C-order numpy
+ * arrays in Fortran order on the host before calling C
+ * (pyseistr/dip3d.py:100, pyseistr/somf3d.py:89-91); this does the same
+ * reorder on the device.  in: a C-order (n1, n2, n3) array (n3 fastest;
+ * a 2-D (n1, n2) array is (n1, 1, n2)); out: the same samples in Fortran
+ * order (n1 fastest), the layout every other entry point takes.  Device
+ * pointers, not overlapping; n2 <= 65535, n1 <= 4194240. */
+int sh_c_to_f(sh_ctx *ctx, const float *in, float *out, int n1, int n2, int n3);
 const char *sh_last_error(const sh_ctx *ctx);
 int sh_version(void);
 /* Block until all work queued on the context's stream has finished. */

@@ -78,7 +78,37 @@ int sh_aux_stream(sh_ctx *ctx) {
     return SH_OK;
 }
 
+// C-order (n1, n2, n3) -> Fortran order: per i2 slab, 64 x 64 tiles of
+// the (n1, n3) matrix go through LDS, each wave reading one row of 64
+// consecutive i3 samples and writing one run of 64 consecutive i1 samples.
+__global__ void __launch_bounds__(256) c_to_f_kernel(const float *__restrict__ in, float *__restrict__ out, int n1,
+                                                     int n2, int n3) {
+    __shared__ float t[64][65];
+    const int i3b = blockIdx.x * 64, i1b = blockIdx.y * 64, i2 = blockIdx.z;
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    for (int r = ty; r < 64; r += 4) {
+        const int i1 = i1b + r, i3 = i3b + tx;
+        if (i1 < n1 && i3 < n3) t[r][tx] = in[((long)i1 * n2 + i2) * n3 + i3];
+    }
+    __syncthreads();
+    for (int r = ty; r < 64; r += 4) {
+        const int i3 = i3b + r, i1 = i1b + tx;
+        if (i1 < n1 && i3 < n3) out[((long)i3 * n2 + i2) * n1 + i1] = t[tx][r];
+    }
+}
+
 extern "C" {
+
+int sh_c_to_f(sh_ctx *ctx, const float *in, float *out, int n1, int n2, int n3) {
+    if (!ctx) return SH_ENULL;
+    if (!in || !out) return sh_fail(ctx, SH_ENULL, "sh_c_to_f: NULL pointer");
+    if (n1 < 1 || n2 < 1 || n3 < 1) return sh_fail(ctx, SH_EINVAL, "sh_c_to_f: bad shape %d x %d x %d", n1, n2, n3);
+    if (n2 > 65535 || (n1 + 63) / 64 > 65535)
+        return sh_fail(ctx, SH_EUNSUP, "sh_c_to_f: shape %d x %d x %d exceeds the grid", n1, n2, n3);
+    const dim3 grid((unsigned)((n3 + 63) / 64), (unsigned)((n1 + 63) / 64), (unsigned)n2);
+    hipLaunchKernelGGL(c_to_f_kernel, grid, dim3(64, 4), 0, ctx->stream, in, out, n1, n2, n3);
+    return sh_launch_check(ctx, "c_to_f");
+}
 
 int sh_prof_enable(sh_ctx *ctx, int on) {
     if (!ctx) return SH_ENULL;

@@ -4,7 +4,8 @@
  * SH_FLAG_CHECK_FINITE: a NaN or Inf in an input fails the call with
    SH_ENONFINITE naming that input, before anything else runs; finite
    inputs run unchanged;
- * sh_dip_report.elapsed_ms is the call's device time."""
+ * sh_dip_report.elapsed_ms is the call's device time;
+ * sh_c_to_f, the device-side Fortran-order flatten of C-order arrays."""
 import ctypes
 
 import numpy as np
@@ -144,3 +145,30 @@ def test_check_finite_drop_in():
             ps.dip2dc(v)
     finally:
         ps.check_finite(False)
+
+
+@pytest.mark.parametrize("shape", [(67, 9, 131), (41, 130, 33), (300, 257), (64, 1, 1024), (1025, 65, 3)])
+def test_c_order_upload_is_fortran_flatten(shape):
+    """sh_c_to_f (used by the drop-ins for C-order arrays of 2^16 samples or
+    more) gives exactly numpy's Fortran-order flatten, which is what the
+    reference's wrappers pass to C; a drop-in run on the C-order array and
+    on its Fortran-order copy gives the same bits."""
+    a = np.random.default_rng(len(shape) + shape[0]).standard_normal(shape).astype(np.float32)
+    assert a.flags.c_contiguous and a.size >= _lib.C_TO_F_MIN
+    got = _lib.to_device(a).cpu().numpy()
+    np.testing.assert_array_equal(got, a.ravel(order="F"))
+    if len(shape) == 3 and shape == (41, 130, 33):
+        import pyseistr_amd as ps
+        f = np.asfortranarray(a)
+        x = ps.somean3dc(a, 0.3 * a, -0.2 * a, 2, 2, 0.01, 2)
+        y = ps.somean3dc(f, np.asfortranarray(0.3 * a), np.asfortranarray(-0.2 * a), 2, 2, 0.01, 2)
+        np.testing.assert_array_equal(x, y)
+
+
+def test_c_to_f_rejects_bad_shapes(fresh):
+    lib, ctx = fresh
+    import torch
+    t = torch.zeros(8, device="cuda")
+    assert lib.sh_c_to_f(ctx, _lib.ptr(t), _lib.ptr(t), 0, 1, 8) != 0
+    assert lib.sh_c_to_f(ctx, _lib.ptr(t), _lib.ptr(t), 1, 70000, 1) != 0
+    assert lib.sh_c_to_f(ctx, None, _lib.ptr(t), 1, 1, 8) != 0
