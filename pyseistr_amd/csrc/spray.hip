@@ -1170,7 +1170,7 @@ __global__ void svmf_traces(const float *__restrict__ U, float *__restrict__ out
 // Each step is exact, so the result equals svmf_one's.
 template <int N>
 __global__ void __launch_bounds__(TB)
-svmf_first(const float *__restrict__ U, float *__restrict__ a, long n, int np, int n1, int nfw, long ntr, long total) {
+svmf_first(const float *__restrict__ U, float *__restrict__ a, long ss, int np, int n1, int nfw, long ntr, long total) {
     const int nf = nfw + 2, m2 = (nfw - 1) / 2, pad = (N - nf) / 2;
     for (long g = (long)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += (long)gridDim.x * blockDim.x) {
         const long t = g % ntr;
@@ -1182,7 +1182,7 @@ svmf_first(const float *__restrict__ U, float *__restrict__ a, long n, int np, i
             if (q < nf) {
                 int col = s + q - m2;
                 col = col < 0 ? 0 : (col > np - 1 ? np - 1 : col);
-                v[q] = U[(long)col * n + (long)k * ntr + t];
+                v[q] = U[(long)col * ss + (long)k * ntr + t];
             } else {
                 v[q] = (q - nf < pad) ? -INFINITY : INFINITY;
             }
@@ -1218,14 +1218,14 @@ svmf_mean(const float *__restrict__ a, float *__restrict__ avg, int np, int n1, 
 
 template <int N>
 __global__ void __launch_bounds__(TB)
-svmf_final(const float *__restrict__ U, const float *__restrict__ avg, float *__restrict__ out, long n, int np,
-           int nfw, long ntr) {
+svmf_final(const float *__restrict__ U, const float *__restrict__ avg, float *__restrict__ out, long n, long ss,
+           int np, int nfw, long ntr) {
     const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= n) return;
     const long t = g % ntr;
     const long k = g / ntr;
     const int c = (np - 1) / 2;
-    const float av = fabsf(U[(long)c * n + g]), mean = avg[t];
+    const float av = fabsf(U[(long)c * ss + g]), mean = avg[t];
     int len;
     if (av < mean) len = (av < mean / 2) ? nfw + 2 : nfw;
     else len = (av > mean * 2) ? nfw - 4 : nfw - 2;
@@ -1236,7 +1236,7 @@ svmf_final(const float *__restrict__ U, const float *__restrict__ avg, float *__
         if (q < len) {
             int col = c + q - h;
             col = col < 0 ? 0 : (col > np - 1 ? np - 1 : col);
-            v[q] = U[(long)col * n + k * ntr + t];
+            v[q] = U[(long)col * ss + k * ntr + t];
         } else {
             v[q] = (q - len < pad) ? -INFINITY : INFINITY;
         }
@@ -1253,8 +1253,9 @@ svmf_final(const float *__restrict__ U, const float *__restrict__ avg, float *__
 
 // svmf over the trace-fast slot volume U[np][n1][ntr] into out[n1][ntr];
 // the one-thread-per-trace kernel where the window exceeds the networks.
-int svmf3(sh_ctx *ctx, const float *U, float *out, int n1, long ntr, int np, int nfw) {
+int svmf3(sh_ctx *ctx, const float *U, float *out, int n1, long ntr, int np, int nfw, long ss = 0) {
     const long n = (long)n1 * ntr;
+    if (ss == 0) ss = n;  // slot stride of U (the section size unless the slots hold several sections)
     const int nf = nfw + 2;
     if (nf > 81) {
         hipLaunchKernelGGL(svmf_traces, dim3(ceil_div(ntr, 64)), dim3(64), 0, ctx->stream, U, out, n, np, n1, nfw, ntr,
@@ -1269,7 +1270,7 @@ int svmf3(sh_ctx *ctx, const float *U, float *out, int n1, long ntr, int np, int
     {
         PROF_A(ctx);
 #define SV1(NN) \
-    if (nf <= NN) { hipLaunchKernelGGL((svmf_first<NN>), dim3(g1), dim3(TB), 0, ctx->stream, U, a, n, np, n1, nfw, ntr, total); } else
+    if (nf <= NN) { hipLaunchKernelGGL((svmf_first<NN>), dim3(g1), dim3(TB), 0, ctx->stream, U, a, ss, np, n1, nfw, ntr, total); } else
         SV1(9) SV1(13) SV1(17) SV1(33) SV1(49) SV1(65) SV1(81) {}
 #undef SV1
         PROF_B(ctx, "svmf_first", 4.0 * (double)(np + 1) * n);
@@ -1279,7 +1280,7 @@ int svmf3(sh_ctx *ctx, const float *U, float *out, int n1, long ntr, int np, int
     SH_TRY(sh_launch_check(ctx, "svmf_mean"));
     PROF_A(ctx);
 #define SV3(NN) \
-    if (nf <= NN) { hipLaunchKernelGGL((svmf_final<NN>), dim3(ceil_div(n, TB)), dim3(TB), 0, ctx->stream, U, avg, out, n, np, nfw, ntr); } else
+    if (nf <= NN) { hipLaunchKernelGGL((svmf_final<NN>), dim3(ceil_div(n, TB)), dim3(TB), 0, ctx->stream, U, avg, out, n, ss, np, nfw, ntr); } else
     SV3(9) SV3(13) SV3(17) SV3(33) SV3(49) SV3(65) SV3(81) {}
 #undef SV3
     PROF_B(ctx, "svmf_final", 4.0 * (double)(nf + 1) * n);
@@ -1456,9 +1457,20 @@ int somf2d_core(sh_ctx *ctx, const float *dn, const float *dip, int n1, int n2, 
     int m = (nmf - 1) / 2;
     if (option == 2) {
         const long ntraces = (long)n2 * n3;
-        hipLaunchKernelGGL(svmf_traces, dim3(ceil_div(ntraces, 64)), dim3(64), 0, ctx->stream, P.U, yT, n, np, n1, nmf,
-                           ntraces, n2, 1);
-        SH_TRY(sh_launch_check(ctx, "svmf_traces"));
+        if (!ws && nmf + 2 <= 81) {
+            // the slots are U[slot][section][sample][trace]: each section
+            // runs the grid-wide kernels of the 3-D filter with the slot
+            // stride of the whole volume (exact, the same result)
+            const long sec = (long)n1 * n2;
+            for (int i3 = 0; i3 < n3; i3++)
+                SH_TRY(svmf3(ctx, P.U + i3 * sec, yT + i3 * sec, n1, n2, np, nmf, n));
+        } else {
+            // the fixed workspace or windows beyond the selection networks:
+            // one thread per trace
+            hipLaunchKernelGGL(svmf_traces, dim3(ceil_div(ntraces, 64)), dim3(64), 0, ctx->stream, P.U, yT, n, np, n1,
+                               nmf, ntraces, n2, 1);
+            SH_TRY(sh_launch_check(ctx, "svmf_traces"));
+        }
     } else {
         SH_TRY(launch_median(ctx, P.U, yT, n, ns - m, nmf));
     }

@@ -479,7 +479,7 @@ def test_somf3d_space_varying_median(ps, shape, r):
     same(got, O.somf3d(v, pi, px, shape, r, r, order=2, option=2))
 
 
-@pytest.mark.parametrize("shape,ns", [((60, 30, 2), 3), ((48, 20, 1), 5)])
+@pytest.mark.parametrize("shape,ns", [((60, 30, 2), 3), ((48, 20, 1), 5), ((512, 256, 1), 8), ((256, 64, 3), 6)])
 def test_somf2d_space_varying_median(ps, shape, ns):
     from oracle import oracle as O
     rng = np.random.default_rng(ns)
